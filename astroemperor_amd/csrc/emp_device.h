@@ -1,0 +1,296 @@
+// Device-side numerics for the Keplerian RV likelihood on gfx950 (CDNA4).
+//
+// Everything here is wave64 code: one parameter vector (theta) is owned by a
+// group of 1..8 wavefronts, and the epochs are strided over the group's
+// lanes.  Per-theta quantities are computed redundantly by every lane and
+// kept in LDS; per-epoch quantities live in VGPRs.
+//
+// Kepler solve.  The reference calls kepler.solve (kepler.py 0.0.7) inside
+// support/models/kep0*.model.  Here the root of E - e sin E = M is found
+// with Markley's non-iterative method (F. L. Markley, "Kepler Equation
+// Solver", Celest. Mech. Dyn. Astron. 63, 101-111, 1995): a cubic starter
+// with ~4e-4 rad worst-case error followed by one fifth-order Householder
+// correction, which reaches machine precision for 0 <= e < 1.
+//
+// Projection.  The templates compute f = 2 atan(sqrt((1+e)/(1-e)) tan(E/2))
+// and A (cos(f + w) + e cos w).  With cos f = (cos E - e)/(1 - e cos E) and
+// sin f = sqrt(1-e^2) sin E/(1 - e cos E) this is the same function
+//     mu = [A (1 - e^2) cos w cos E - A sqrt(1 - e^2) sin w sin E] / (1 - e cos E),
+// which needs one reciprocal per planet-epoch and no atan/tan.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace emp {
+
+constexpr double PI_D      = 3.141592653589793116;        // nearest double to pi
+constexpr double TWO_PI_HI = 6.283185307179586232;        // nearest double to 2 pi
+constexpr double TWO_PI_LO = 2.4492935982947064e-16;      // 2 pi - TWO_PI_HI
+constexpr double INV_TWO_PI = 0.15915494309189535;
+// Markley's alpha(M, e) = (3 pi^2 + 1.6 pi (pi - |M|)/(1 + e)) / (pi^2 - 6)
+constexpr double MK_A0 = 3.0 * PI_D * PI_D / (PI_D * PI_D - 6.0);
+constexpr double MK_A1 = 1.6 * PI_D / (PI_D * PI_D - 6.0);
+
+// ----------------------------------------------------------------------------
+// Wave helpers (wave64).
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// FP64 reciprocal for arguments in a normal range: hardware estimate plus two
+// Newton-Raphson refinements (each doubles the correct bits).
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// Reduce an FP64 angle to [-pi, pi] with a two-term (Cody-Waite) 2 pi.
+__device__ __forceinline__ double reduce_2pi(double M) {
+    double n = __builtin_rint(M * INV_TWO_PI);
+    double r = __builtin_fma(-n, TWO_PI_HI, M);
+    return __builtin_fma(-n, TWO_PI_LO, r);
+}
+
+// ----------------------------------------------------------------------------
+// Per-planet constants, computed once per theta (per planet) and kept in LDS.
+// ----------------------------------------------------------------------------
+struct PlanetC {
+    double freq;     // 2 pi / P
+    double phase;    // M = freq * t + phase         (tp_mode == 0)
+    double tp;       // M = freq * (t - tp)          (tp_mode == 1)
+    double e;        // eccentricity
+    double c1;       // A (1 - e^2) cos w
+    double c2;       // -A sqrt(1 - e^2) sin w
+    double mk_a1;    // MK_A1 / (1 + e)
+    double mk_d0;    // 3 (1 - e)
+    double K;        // amplitude (for gradients)
+    double cw, sw;   // cos w, sin w (for gradients)
+    int tp_mode;
+    int bad;         // NaN-producing parameter set (reference would give NaN)
+};
+
+// Builds PlanetC from the 5 block parameters, following the reference's
+// per-parameterisation template (support/models/kep0p.model, Appendix
+// "derived" column): P from lnP for 2/6/7, K = As^2+Ac^2 and
+// phase = arccos(Ac/sqrt K) for 2, e = S^2+C^2 and w = arccos(C/sqrt e)
+// (2 pi - w when S < 0) for 1/2/4/7 with w forced to 0 below 1e-6 (1/7) or
+// 1e-5 (2/4) unless `use_c` (clib/ckep0*.model has no guard).
+__device__ inline PlanetC planet_setup(const double* th, int param, int use_c) {
+    PlanetC pc;
+    double per, A, e, cw = 1.0, sw = 0.0;
+    pc.tp_mode = 0;
+    pc.phase = 0.0;
+    pc.tp = 0.0;
+    pc.bad = 0;
+    bool sc = (param == 1 || param == 2 || param == 4 || param == 7);
+    if (param == 2 || param == 6 || param == 7) per = exp(th[0]); else per = th[0];
+    if (param == 2) {
+        A = th[1] * th[1] + th[2] * th[2];
+        double ph = acos(th[2] / sqrt(A));
+        if (th[1] < 0) ph = 2.0 * PI_D - ph;
+        pc.phase = ph;
+    } else {
+        A = th[1];
+        if (param == 3 || param == 4) { pc.tp_mode = 1; pc.tp = th[2]; }
+        else pc.phase = th[2];
+    }
+    if (sc) {
+        e = th[3] * th[3] + th[4] * th[4];
+        double thr = (param == 2 || param == 4) ? 1e-5 : 1e-6;
+        if (!use_c && e < thr) {
+            cw = 1.0; sw = 0.0;
+        } else {
+            double se = sqrt(e);
+            cw = th[4] / se;
+            sw = th[3] / se;
+            if (!(cw >= -1.0 && cw <= 1.0)) pc.bad = 1;   // arccos domain error -> NaN
+        }
+    } else {
+        e = th[3];
+        sincos(th[4], &sw, &cw);
+    }
+    if (!(e >= 0.0 && e < 1.0)) pc.bad = 1;
+    double q = sqrt((1.0 - e) * (1.0 + e));
+    pc.freq = 2.0 * PI_D / per;
+    pc.e = e;
+    pc.c1 = A * (1.0 - e) * (1.0 + e) * cw;
+    pc.c2 = -A * q * sw;
+    pc.mk_a1 = MK_A1 / (1.0 + e);
+    pc.mk_d0 = 3.0 * (1.0 - e);
+    pc.K = A;
+    pc.cw = cw;
+    pc.sw = sw;
+    if (pc.bad) {   // the reference evaluates to NaN here; make that explicit
+        pc.c1 = __builtin_nan("");
+        pc.c2 = __builtin_nan("");
+    }
+    return pc;
+}
+
+__device__ __forceinline__ double planet_M(const PlanetC& pc, double t) {
+    return pc.tp_mode ? (t - pc.tp) * pc.freq : pc.freq * t + pc.phase;
+}
+
+// ----------------------------------------------------------------------------
+// FP64 Kepler solve (Markley 1995).  Input: M reduced to [-pi, pi].
+// Output: sin E, cos E.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, double& sE, double& cE) {
+    const double e = pc.e;
+    const double Ma = __builtin_fabs(M);
+    const double ome = 1.0 - e;
+    const double alpha = __builtin_fma(pc.mk_a1, PI_D - Ma, MK_A0);
+    const double d = __builtin_fma(alpha, e, pc.mk_d0);
+    const double ad = alpha * d;
+    const double M2 = Ma * Ma;
+    const double q = __builtin_fma(2.0 * ad, ome, -M2);
+    const double r = __builtin_fma(3.0 * ad * (d - ome), Ma, M2 * Ma);
+    const double q2 = q * q;
+    double w = cbrt(__builtin_fabs(r) + sqrt(__builtin_fma(q2, q, r * r)));
+    w = w * w;
+    const double E1 = (2.0 * r * w / (__builtin_fma(w, w + q, q2)) + Ma) / d;
+    double s, c;
+    sincos(E1, &s, &c);
+    // Fifth-order Householder correction (Markley eqs. 20-22).
+    const double f0 = E1 - __builtin_fma(e, s, Ma);
+    const double f1 = __builtin_fma(-e, c, 1.0);
+    const double f2 = e * s;
+    const double f3 = 1.0 - f1;
+    const double d3 = -f0 / (f1 - 0.5 * f0 * f2 / f1);
+    const double d4 = -f0 / (f1 + 0.5 * d3 * f2 + d3 * d3 * f3 * (1.0 / 6.0));
+    const double d5 = -f0 / (f1 + 0.5 * d4 * f2 + d4 * d4 * f3 * (1.0 / 6.0)
+                             - d4 * d4 * d4 * f2 * (1.0 / 24.0));
+    // Rotate (s, c) by d5; |d5| < 1e-3 so three Taylor terms are exact to 1e-19.
+    const double dd = d5 * d5;
+    const double sd = d5 * __builtin_fma(-dd, 1.0 / 6.0, 1.0);
+    const double cd = __builtin_fma(dd, __builtin_fma(dd, 1.0 / 24.0, -0.5), 1.0);
+    sE = __builtin_fma(s, cd, c * sd);
+    cE = __builtin_fma(c, cd, -s * sd);
+    if (M < 0.0) sE = -sE;
+}
+
+// ----------------------------------------------------------------------------
+// Mixed-precision Kepler solve.  The FP32 part (Markley starter with
+// hardware log2/exp2/rcp/sqrt and a third-order FP32 correction with the
+// hardware sin/cos) buys the iteration count; one FP64 Halley step from the
+// FP32 root (cubic convergence from ~1e-6) restores FP64 accuracy, so the
+// likelihood stays within 1e-6 of the FP64 path.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void kepler_mixed(double M, const PlanetC& pc, double& sE, double& cE) {
+    const float ef = (float)pc.e;
+    const double Ma = __builtin_fabs(M);
+    const float Mf = (float)Ma;
+    const float omef = 1.0f - ef;
+    const float alpha = __builtin_fmaf((float)pc.mk_a1, (float)PI_D - Mf, (float)MK_A0);
+    const float d = __builtin_fmaf(alpha, ef, (float)pc.mk_d0);
+    const float ad = alpha * d;
+    const float M2 = Mf * Mf;
+    const float q = __builtin_fmaf(2.0f * ad, omef, -M2);
+    const float r = __builtin_fmaf(3.0f * ad * (d - omef), Mf, M2 * Mf);
+    const float q2 = q * q;
+    const float x = __builtin_fabsf(r) + __builtin_amdgcn_sqrtf(__builtin_fmaf(q2, q, r * r));
+    // w = x^(2/3) = 2^(2/3 log2 x)
+    const float w = __builtin_amdgcn_exp2f(0.66666667f * __builtin_amdgcn_logf(x));
+    float E = (2.0f * r * w * __builtin_amdgcn_rcpf(__builtin_fmaf(w, w + q, q2)) + Mf)
+              * __builtin_amdgcn_rcpf(d);
+    {   // FP32 Halley step with hardware sin/cos (argument in revolutions).
+        const float rev = E * 0.15915494f;
+        const float s = __builtin_amdgcn_sinf(rev), c = __builtin_amdgcn_cosf(rev);
+        const float f0 = E - __builtin_fmaf(ef, s, Mf);
+        const float f1 = __builtin_fmaf(-ef, c, 1.0f);
+        const float dn = -f0 * __builtin_amdgcn_rcpf(f1);
+        E += -f0 * __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f * dn, ef * s, f1));
+    }
+    // FP64 Halley step from the FP32 root.
+    const double E0 = (double)E;
+    double s, c;
+    sincos(E0, &s, &c);
+    const double e = pc.e;
+    const double f0 = E0 - __builtin_fma(e, s, Ma);
+    const double f1 = __builtin_fma(-e, c, 1.0);
+    const double r1 = rcp64(f1);
+    const double dn = -f0 * r1;
+    const double dl = dn * __builtin_fma(-0.5 * dn * e * s, r1, 1.0);
+    const double dd = dl * dl;
+    const double sd = dl * __builtin_fma(-dd, 1.0 / 6.0, 1.0);
+    const double cd = __builtin_fma(dd, __builtin_fma(dd, 1.0 / 24.0, -0.5), 1.0);
+    sE = __builtin_fma(s, cd, c * sd);
+    cE = __builtin_fma(c, cd, -s * sd);
+    if (M < 0.0) sE = -sE;
+}
+
+template <int PREC>
+__device__ __forceinline__ double planet_mu(const PlanetC& pc, double t) {
+    const double M = reduce_2pi(planet_M(pc, t));
+    double sE, cE;
+    if (PREC == 0) kepler_fp64(M, pc, sE, cE);
+    else kepler_mixed(M, pc, sE, cE);
+    const double den = __builtin_fma(-pc.e, cE, 1.0);
+    return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64(den);
+}
+
+// ----------------------------------------------------------------------------
+// Counter-based RNG: Philox-4x32-10 (Salmon, Moraes, Dror & Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11).
+// ----------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)M0 * c.x;
+        uint64_t p1 = (uint64_t)M1 * c.z;
+        U4 n;
+        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        n.y = (uint32_t)p1;
+        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        n.w = (uint32_t)p0;
+        c = n;
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Uniform double in (0, 1) from two 32-bit words (53 random bits).
+__host__ __device__ inline double u01(uint32_t hi, uint32_t lo) {
+    uint64_t v = (((uint64_t)hi << 32) | lo) >> 11;
+    return ((double)v + 0.5) * 1.1102230246251565e-16;   // 2^-53
+}
+
+// Keyed bijection of [0, n) (4-round Feistel on the enclosing power of four,
+// cycle-walking back into range).  Used to pair walkers between adjacent
+// temperatures independently of their states.
+__host__ __device__ inline uint32_t feistel_perm(uint32_t i, uint32_t n, uint32_t key0, uint32_t key1) {
+    uint32_t bits = 2;
+    while ((1u << bits) < n) bits += 2;
+    const uint32_t half = bits / 2, mask = (1u << half) - 1u;
+    uint32_t x = i;
+    do {
+        uint32_t L = x >> half, R = x & mask;
+        for (uint32_t r = 0; r < 4; ++r) {
+            uint32_t h = R * 0x9E3779B1u ^ (key0 + r * 0x85EBCA77u);
+            h ^= h >> 15; h *= 0x2C1B3C6Du ^ key1; h ^= h >> 12;
+            uint32_t nL = R;
+            R = (L ^ h) & mask;
+            L = nL;
+        }
+        x = (L << half) | R;
+    } while (x >= n);
+    return x;
+}
+
+}  // namespace emp

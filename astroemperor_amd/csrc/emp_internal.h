@@ -1,0 +1,80 @@
+// Internal declarations shared by the HIP translation units of libemp_mi355x.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+
+#include "../../include/emp_abi.h"
+#include "emp_device.h"
+
+namespace emp {
+
+constexpr int MAXD = 128;        // full parameter vector length supported by the kernels
+constexpr int KMAX = 8;          // Keplerians per model
+constexpr int IMAX = 16;         // instruments per model
+constexpr int AMAX = 4;          // acceleration order
+constexpr int BLOCK = 512;       // threads per workgroup (8 waves)
+constexpr int NWAVES = BLOCK / 64;
+constexpr int TILE = 1024;       // epochs staged in LDS per tile
+
+// Everything a kernel needs to know about a model, passed by value.
+struct ModelDev {
+    int N, I, K, param, use_c, acc, has_jit;
+    int ma_q, ma_global, ma_active;
+    int Dfull, D, nblocks;
+    int off_kep, off_acc, off_off, off_jit, off_ma;
+    const double* t;      // [N] device order
+    const double* y;      // [N]
+    const double* s2;     // [N] yerr^2
+    const double* dt;     // [N] t_i - t_{i-1} within an MA segment, -1 at a segment start
+    const int* ins;       // [N]
+    const int* inv_free;  // [Dfull] free index of each full parameter, -1 if fixed
+    const int* free_idx;  // [D]
+    const double* fixed_val;          // [Dfull]
+    const EmpPriorTerm* prior;        // [Dfull]
+    const int* block_start;           // [nblocks + 1]
+    const EmpPriorTerm* ecc_prior;    // [K]
+    double lnl_const;                 // -0.5 ln(2 pi) N
+    double t0;                        // X_[0] (acceleration origin)
+};
+
+// Per-slot LDS scratch of the likelihood kernels.
+struct SlotLDS {
+    double theta[MAXD];
+    PlanetC planet[KMAX];
+    double gam[IMAX], jit2[IMAX], ma_c[IMAX], ma_tau[IMAX], jit[IMAX];
+    double acc[AMAX];
+};
+
+}  // namespace emp
+
+struct EmpModel {
+    EmpModelDesc desc;        // scalars copied from the caller (pointers cleared)
+    emp::ModelDev dev;
+    std::vector<void*> allocations;
+    std::vector<int> order;   // device position -> caller (time-sorted) position
+    unsigned long long* d_nonfinite = nullptr;
+    double* d_stage_in = nullptr;     // staging for EMP_HOST_POINTERS
+    double* d_stage_out = nullptr;
+    size_t stage_in_cap = 0, stage_out_cap = 0;
+    int wpt_override = 0;
+    int device = 0;
+    int n_cu = 256;
+};
+
+// Error handling.
+void emp_set_error(const char* fmt, ...);
+int emp_hip_fail(hipError_t e, const char* what, const char* file, int line);
+bool emp_fault_injected();
+#define EMP_HIP(x)                                                              \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) return emp_hip_fail(e_, #x, __FILE__, __LINE__);  \
+    } while (0)
+
+// Likelihood launchers (emp_model.hip), used by the PT engine as well.
+int emp_pick_wpt(const EmpModel* m, int64_t B, int precision);
+int emp_launch_lnl(const EmpModel* m, const double* d_theta, int64_t B, int64_t ld,
+                   double* d_lnl, int precision, int wpt, hipStream_t s);
+int emp_launch_prior(const EmpModel* m, const double* d_theta, int64_t B, int64_t ld,
+                     double* d_lnp, hipStream_t s);

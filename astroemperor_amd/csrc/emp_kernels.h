@@ -1,0 +1,222 @@
+// Likelihood building blocks shared by the batched-lnL, model-eval and PT
+// kernels.
+//
+// Launch geometry: one workgroup = 8 wave64s.  The workgroup owns SLOTS =
+// 8 / WPT parameter vectors; each vector is processed by WPT waves whose
+// 64*WPT lanes stride over the epochs.  Epoch data (t, y, yerr^2, MA time
+// step, instrument tag) is staged into LDS in tiles of TILE epochs and shared
+// by all slots of the workgroup.
+#pragma once
+#include "emp_internal.h"
+
+namespace emp {
+
+constexpr double LN2 = 0.69314718055994530942;
+
+// Prior of one full parameter vector, sequentially and in the reference's
+// order so that the sum is bit-identical to the generated my_prior
+// (emp/emp.py:182-254): per block, add each free parameter's prior, return
+// -inf as soon as a block leaves the sum at -inf, then add the block's
+// derived eccentricity prior.
+__device__ inline double prior_term(const EmpPriorTerm& p, double x) {
+#pragma clang fp contract(off)
+    if (!(x >= p.lo && x <= p.hi)) return -__builtin_inf();
+    switch (p.kind) {
+        case EMP_PRIOR_UNIFORM:
+        case EMP_PRIOR_JEFFREYS:
+            return p.a0;
+        case EMP_PRIOR_NORMAL: {
+            double z = (x - p.a0) / p.a1;
+            return -0.5 * (z * z) - p.a3 - p.a2;
+        }
+        case EMP_PRIOR_BETA: {
+            double v = -p.a2;
+            if (p.a0 != 1.0) v += (p.a0 - 1.0) * log(x);
+            if (p.a1 != 1.0) v += (p.a1 - 1.0) * log1p(-x);
+            return v;
+        }
+        default:
+            return 0.0;
+    }
+}
+
+__device__ inline double prior_seq(const ModelDev& md, const double* th) {
+#pragma clang fp contract(off)
+    double lp = 0.0;
+    for (int b = 0; b < md.nblocks; ++b) {
+        for (int j = md.block_start[b]; j < md.block_start[b + 1]; ++j) {
+            if (md.inv_free[j] < 0) continue;
+            lp += prior_term(md.prior[j], th[j]);
+        }
+        if (lp == -__builtin_inf()) return lp;
+        if (b < md.K && md.ecc_prior[b].kind != EMP_PRIOR_NONE) {
+            const double* s = th + md.off_kep + 5 * b;
+            double x = s[3] * s[3] + s[4] * s[4];
+            lp += prior_term(md.ecc_prior[b], x);
+        }
+    }
+    return lp;
+}
+
+// Per-slot constants (planets, instruments, acceleration) into LDS.
+// Called by the slot's first wave; the caller synchronises afterwards.
+__device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
+    if (lane < md.K) sl.planet[lane] = planet_setup(sl.theta + md.off_kep + 5 * lane, md.param, md.use_c);
+    if (lane < md.I) {
+        sl.gam[lane] = sl.theta[md.off_off + lane];
+        double j = md.has_jit ? sl.theta[md.off_jit + lane] : 0.0;
+        sl.jit[lane] = j;
+        sl.jit2[lane] = j * j;
+        if (md.ma_active) {
+            int set = md.ma_global ? 0 : lane;
+            sl.ma_c[lane] = sl.theta[md.off_ma + 2 * md.ma_q * set];
+            sl.ma_tau[lane] = sl.theta[md.off_ma + 2 * md.ma_q * set + 1];
+        }
+    }
+    if (lane < md.acc) sl.acc[lane] = sl.theta[md.off_acc + lane];
+}
+
+struct TileLDS {
+    double t[TILE], y[TILE], s2[TILE], dt[TILE];
+    int ins[TILE];
+};
+
+__device__ __forceinline__ void tile_load(const ModelDev& md, TileLDS& tl, int tb, int tn) {
+    for (int j = threadIdx.x; j < tn; j += BLOCK) {
+        tl.t[j] = md.t[tb + j];
+        tl.y[j] = md.y[tb + j];
+        tl.s2[j] = md.s2[tb + j];
+        tl.ins[j] = md.ins[tb + j];
+        if (md.ma_active) tl.dt[j] = md.dt[tb + j];
+    }
+}
+
+// Model value (planets + offset + acceleration) of one epoch.
+template <int PREC>
+__device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl, double t, int k) {
+    double mu = sl.gam[k];
+    if (md.acc) {
+        const double x = t - md.t0;
+        double v = 0.0;
+        for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
+        mu += v * x;
+    }
+    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], t);
+    return mu;
+}
+
+// Affine map composition for the MA(1) recurrence eps_i = a_i eps_{i-1} + b_i:
+// applying (a1, b1) then (a2, b2) gives (a2 a1, a2 b1 + b2).
+__device__ __forceinline__ void wave_affine_scan(double& a, double& b, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        double pa = __shfl_up(a, o, 64);
+        double pb = __shfl_up(b, o, 64);
+        if (lane >= o) {
+            b = __builtin_fma(a, pb, b);
+            a = a * pa;
+        }
+    }
+}
+
+struct ScanLDS {
+    double tot[2][NWAVES][2];
+    double red[NWAVES][2];
+};
+
+// lnL of the slot's parameter vector.  All threads of the workgroup must
+// call this together (it contains workgroup barriers); `live` == false makes
+// a slot skip the arithmetic while still taking part in the barriers.
+// With WRITE the per-epoch model and err2 are stored (device order).
+template <int PREC, int WPT, bool WRITE>
+__device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
+                            bool live, int wsub, double* out_model, double* out_err2) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int G = 64 * WPT;
+    const int g = wsub * 64 + lane;
+    double chi = 0.0, prod = 1.0;
+    int pexp = 0;
+    double carry = 0.0;
+    int parity = 0;
+    for (int tb = 0; tb < md.N; tb += TILE) {
+        const int tn = min(TILE, md.N - tb);
+        __syncthreads();
+        tile_load(md, tl, tb, tn);
+        __syncthreads();
+        for (int cb = 0; cb < tn; cb += G) {
+            const int j = cb + g;
+            const bool valid = live && (j < tn);
+            double r = 0.0, err2 = 1.0, mu = 0.0;
+            int k = 0;
+            if (valid) {
+                k = tl.ins[j];
+                mu = epoch_mu<PREC>(md, sl, tl.t[j], k);
+                err2 = tl.s2[j] + sl.jit2[k];
+                r = tl.y[j] - mu;
+            }
+            double eps = r;
+            if (md.ma_active) {
+                double a = 1.0, b = 0.0;
+                if (valid) {
+                    const double dtj = tl.dt[j];
+                    a = dtj < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-dtj / sl.ma_tau[k]));
+                    b = r;
+                }
+                wave_affine_scan(a, b, lane);
+                if (WPT == 1) {
+                    eps = __builtin_fma(a, carry, b);
+                    carry = __shfl(eps, 63, 64);
+                } else {
+                    if (lane == 63) { sc.tot[parity][wave][0] = a; sc.tot[parity][wave][1] = b; }
+                    __syncthreads();
+                    const int w0 = wave - wsub;
+                    double pa = 1.0, pb = 0.0;
+                    for (int w = 0; w < WPT; ++w) {
+                        if (w == wsub) {
+                            eps = __builtin_fma(a, __builtin_fma(pa, carry, pb), b);
+                        }
+                        const double ta = sc.tot[parity][w0 + w][0], tbv = sc.tot[parity][w0 + w][1];
+                        pb = __builtin_fma(ta, pb, tbv);
+                        pa = ta * pa;
+                    }
+                    carry = __builtin_fma(pa, carry, pb);
+                    parity ^= 1;
+                }
+            }
+            if (valid) {
+                chi = __builtin_fma(eps * eps, rcp64(err2), chi);
+                prod *= err2;
+                pexp += __builtin_amdgcn_frexp_exp(prod);
+                prod = __builtin_amdgcn_frexp_mant(prod);
+                if (WRITE) {
+                    out_model[tb + j] = mu + (r - eps);
+                    out_err2[tb + j] = err2;
+                }
+            }
+        }
+    }
+    double ld = live ? log(prod) + (double)pexp * LN2 : 0.0;
+    chi = wave_sum(chi);
+    ld = wave_sum(ld);
+    if (WPT > 1) {
+        __syncthreads();
+        if (lane == 0) { sc.red[wave][0] = chi; sc.red[wave][1] = ld; }
+        __syncthreads();
+        chi = 0.0; ld = 0.0;
+        const int w0 = wave - wsub;
+        for (int w = 0; w < WPT; ++w) { chi += sc.red[w0 + w][0]; ld += sc.red[w0 + w][1]; }
+    }
+    return -0.5 * (chi + ld) + md.lnl_const;
+}
+
+// Expand the free vector (global memory, row `row`) into the slot's full
+// vector in LDS, re-inserting fixed values (emp/emp_model.py:709-711).
+__device__ __forceinline__ void expand_theta(const ModelDev& md, SlotLDS& sl, const double* row, int g, int G) {
+    for (int j = g; j < md.Dfull; j += G) {
+        const int f = md.inv_free[j];
+        sl.theta[j] = f >= 0 ? row[f] : md.fixed_val[j];
+    }
+}
+
+}  // namespace emp

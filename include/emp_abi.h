@@ -1,0 +1,230 @@
+/*
+ * emp_abi.h -- C ABI of the MI355X (gfx950) Keplerian-RV likelihood and
+ * parallel-tempering engine.
+ *
+ * This is the drop-in boundary for the reference's per-walker callables and
+ * its sampler engine.  In the reference those live in a Python script that
+ * EMPEROR writes at run time:
+ *   my_model(theta) -> (model, err2)   emp/emp_model.py:706-781
+ *   my_likelihood(theta) -> float      support/likelihoods/00.like:3-5
+ *   my_prior(theta) -> float           emp/emp.py:182-254
+ *   reddemcee.PTSampler(...)           emp/emp.py:576-593
+ *   sampler.run_mcmc(p1, nsweeps, nsteps)  support/endit_reddemcee.scr:3
+ *   sampler.get_chain/get_log_like/get_log_prob/get_tsw/get_smd
+ *                                      emp/emp.py:1401-1429, 1985-1990
+ * (emp/ = src/astroemperor/).  Each entry point below names the callable it
+ * replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative EMP_E* code on
+ *    failure; emp_last_error() then describes the failure (thread-local).
+ *  - No exceptions cross the boundary; the caller owns every buffer.
+ *  - theta buffers hold the FREE parameters (fixed ones are re-inserted
+ *    on device, as emp_model.py:709-711 does), row-major [B, ld], ld >= ndim.
+ *  - By default pointers are device pointers and work is enqueued on
+ *    `stream` (a hipStream_t, NULL = default stream) without synchronising.
+ *    With EMP_HOST_POINTERS the library copies through its own staging
+ *    buffers and returns after the results are on the host.
+ *  - A likelihood that is not finite (the reference would return NaN, e.g.
+ *    for e >= 1) is written as -inf and counted (emp_model_nonfinite_count).
+ *  - A handle must not be used from two threads at once.
+ */
+#ifndef EMP_ABI_H
+#define EMP_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMP_ABI_VERSION 1
+
+/* Error codes. */
+#define EMP_OK 0
+#define EMP_EINVAL (-1)     /* bad argument or descriptor */
+#define EMP_EHIP (-2)       /* HIP runtime error */
+#define EMP_ENOMEM (-3)     /* device allocation failed */
+#define EMP_EUNSUP (-4)     /* model feature outside what the kernels implement */
+#define EMP_EIO (-5)        /* checkpoint file error */
+#define EMP_EFAULT_INJECTED (-6) /* failure forced by EMP_FAULT_AT (testing) */
+
+/* Flags. */
+#define EMP_HOST_POINTERS 1
+
+/* Precision of the likelihood path. */
+#define EMP_FP64 0          /* every operation FP64: the drop-in path */
+#define EMP_MIXED 1         /* FP32 Kepler iterations + FP64 phase, final step, accumulation */
+
+/* Prior kinds (support/priors/<Kind>.prior). */
+#define EMP_PRIOR_NONE (-1)
+#define EMP_PRIOR_UNIFORM 0   /* a0 = log(1/(hi-lo))                        Uniform.prior:1-5 */
+#define EMP_PRIOR_NORMAL 1    /* a0 = mu, a1 = s, a2 = logZ, a3 = log(s*sqrt(2*pi))  Normal.prior:1-9 */
+#define EMP_PRIOR_JEFFREYS 2  /* a0 = log(1/(hi-lo)) (the reference's Jeffreys is uniform) Jeffreys.prior:1-6 */
+#define EMP_PRIOR_BETA 3      /* a0 = a, a1 = b, a2 = ln B(a, b)            Beta.prior:1-6 */
+#define EMP_PRIOR_FIXED 4     /* 0                                          Fixed.prior:1-2 */
+
+/* Semantics of per-instrument moving average (support/models/moav00.model). */
+#define EMP_SEM_REFERENCE 0   /* per-instrument MA is a no-op, as the reference computes it */
+#define EMP_SEM_INTENDED 1    /* moav01's recurrence within each instrument's time series */
+
+typedef struct EmpModel EmpModel;
+typedef struct EmpPT EmpPT;
+
+typedef struct EmpPriorTerm {
+    int32_t kind;      /* EMP_PRIOR_* */
+    int32_t block;     /* block index: the prior returns -inf after the first block whose sum is -inf */
+    double lo, hi;     /* inclusive support */
+    double a0, a1, a2, a3;
+} EmpPriorTerm;
+
+/* Model description: the contents of the generated script's data loader
+ * (emp/emp_model.py:406-433), model writer (706-781) and prior writer
+ * (emp/emp.py:182-254), in flat form. */
+typedef struct EmpModelDesc {
+    int32_t abi_version;        /* EMP_ABI_VERSION */
+    int32_t n_epochs;           /* N = ndat */
+    int32_t n_ins;              /* I: instruments (Flag 1..I) */
+    int32_t n_planets;          /* k Keplerian blocks */
+    int32_t parameterisation;   /* 0,1,2,3,4,6,7 (block_repo.py:23-31) */
+    int32_t use_c;              /* 1: clib/ckep0*.model variant (no small-e guard) */
+    int32_t acc_order;          /* Acceleration block order, 0 = none (acc.model) */
+    int32_t has_jitter;         /* Jitter block present (jitter00.model) */
+    int32_t ma_order;           /* MOAV order q, 0 = none */
+    int32_t ma_global;          /* 1: moav01.model (one set), 0: moav00.model (per instrument) */
+    int32_t semantics;          /* EMP_SEM_* */
+    int32_t ndim_full;          /* length of the full parameter vector */
+    int32_t ndim;               /* free parameters */
+    int32_t n_blocks;           /* number of parameter blocks */
+    int32_t off_kep, off_acc, off_offset, off_jitter, off_ma;  /* block starts in the full vector */
+    const double* t;            /* [N] X_ = BJD - common_t, time-sorted (host) */
+    const double* y;            /* [N] Y_ */
+    const double* yerr;         /* [N] YERR_ */
+    const int32_t* ins;         /* [N] instrument index 0..I-1 (Flag - 1) */
+    const int32_t* free_index;  /* [ndim] full index of each free parameter (C_) */
+    const double* fixed_value;  /* [ndim_full] values of fixed parameters (mod_fixed_) */
+    const EmpPriorTerm* prior;  /* [ndim_full] in block order (kind ignored for fixed) */
+    const EmpPriorTerm* ecc_prior; /* [n_planets] prior on e = S^2 + C^2 (EMP_PRIOR_NONE if absent) */
+} EmpModelDesc;
+
+/* Replaces: the generated module top level (data, masks, constants).
+ * Copies the data to device memory of the current HIP device. */
+int emp_model_create(const EmpModelDesc* desc, EmpModel** out);
+void emp_model_destroy(EmpModel* model);
+
+/* Replaces: my_likelihood(theta) (support/likelihoods/00.like:3-5), batched.
+ * lnl[b] = lnL(theta[b, :ndim]). precision: EMP_FP64 or EMP_MIXED. */
+int emp_lnl_batch(EmpModel* model, const double* theta, int64_t B, int64_t ld,
+                  double* lnl, int32_t precision, int32_t flags, void* stream);
+
+/* Replaces: my_prior(theta) (emp/emp.py:182-254), batched. */
+int emp_lnprior_batch(EmpModel* model, const double* theta, int64_t B, int64_t ld,
+                      double* lnp, int32_t flags, void* stream);
+
+/* Replaces: my_model(theta) -> (model0, err20) (emp/emp_model.py:706-781).
+ * Host pointers; one theta; FP64; outputs in the data's time order. */
+int emp_model_eval(EmpModel* model, const double* theta, double* model_out, double* err2_out);
+
+/* No reference counterpart: lnL and dlnL/dtheta[free]
+ * for a batch, by a hand-written adjoint kernel (FP64). grad is [B, ndim]. */
+int emp_lnl_grad_batch(EmpModel* model, const double* theta, int64_t B, int64_t ld,
+                       double* lnl, double* grad, int32_t flags, void* stream);
+
+/* Launch shape: waves per parameter vector (1, 2, 4 or 8); 0 = automatic. */
+int emp_model_set_waves_per_theta(EmpModel* model, int32_t waves_per_theta);
+int emp_model_get_waves_per_theta(EmpModel* model, int32_t precision, int32_t* out);
+
+/* Number of non-finite likelihoods mapped to -inf since creation. */
+int emp_model_nonfinite_count(EmpModel* model, int64_t* out);
+
+/* ---------------------------------------------------------------------------
+ * Parallel tempering (replaces reddemcee.PTSampler + run_mcmc).
+ * ------------------------------------------------------------------------ */
+typedef struct EmpPTDesc {
+    int32_t ntemps;             /* T */
+    int32_t nwalkers;           /* W (even) */
+    int32_t nsteps;             /* stretch steps per sweep (engine_config['setup'][3]) */
+    int32_t precision;          /* EMP_FP64 / EMP_MIXED */
+    double a;                   /* stretch scale (emcee StretchMove default 2.0) */
+    uint64_t seed;              /* Philox key */
+    int32_t adapt_mode;         /* -1: frozen ladder; 0: Vousden et al. 2016 dynamics */
+    int32_t record;             /* 1: keep chain/lnL/lnp history on device */
+    double adapt_tau;           /* engine_config['adapt_tau'] */
+    double adapt_nu;            /* engine_config['adapt_nu'] */
+    int64_t max_sweeps;         /* history capacity (sweeps) when record == 1 */
+} EmpPTDesc;
+
+/* What emp_pt_get returns. */
+#define EMP_PT_CHAIN 0      /* [S, T, W, D] positions after each sweep (history) */
+#define EMP_PT_LNL 1        /* [S, T, W] */
+#define EMP_PT_LNP 2        /* [S, T, W] log prior */
+#define EMP_PT_BETAS 3      /* [S, T] ladder after each sweep */
+#define EMP_PT_TSW 4        /* [S, T-1] swap acceptance per adjacent pair per sweep */
+#define EMP_PT_SMD 5        /* [S, T-1] mean |log swap ratio| per pair per sweep */
+#define EMP_PT_ACCEPTED 6   /* [T, W] accepted stretch moves (int64) */
+#define EMP_PT_STATE_X 7    /* [T, W, D] current positions */
+#define EMP_PT_STATE_LNL 8  /* [T, W] */
+#define EMP_PT_STATE_LNP 9  /* [T, W] */
+#define EMP_PT_STATE_BETAS 10 /* [T] */
+#define EMP_PT_NSWEEPS 11   /* [2] sweeps done, likelihoods evaluated (int64) */
+
+/* Replaces: reddemcee.PTSampler(nwalkers, ndim, my_likelihood, my_prior,
+ * ntemps=, betas=, adapt_tau=, adapt_nu=, adapt_mode=) (emp/emp.py:576-593)
+ * and the initial evaluation of p1 = test_init() (emp/emp.py:650-682).
+ * betas: [T] host, p0: [T, W, ndim] host. */
+int emp_pt_create(EmpModel* model, const EmpPTDesc* desc, const double* betas,
+                  const double* p0, EmpPT** out);
+void emp_pt_destroy(EmpPT* pt);
+
+/* Replaces: sampler.run_mcmc(p1, nsweeps, nsteps) (support/endit_reddemcee.scr:3).
+ * Each sweep = nsteps red/blue stretch steps at every temperature, one round
+ * of adjacent swaps (hottest pair first), an optional ladder update and the
+ * history record.  use_graph: capture one sweep into a hipGraph and replay it.
+ * Enqueued on the engine's own stream; returns after completion. */
+int emp_pt_run(EmpPT* pt, int64_t nsweeps, int32_t use_graph);
+
+/* Replaces: the getters (get_chain/get_log_like/get_log_prob/get_betas/
+ * get_tsw/get_smd/accepted, emp/emp.py:1401-1429,1977-1990).
+ * Copies into host_out (capacity in bytes); shape_out[4] receives the dims
+ * (unused trailing dims = 1). */
+int emp_pt_get(EmpPT* pt, int32_t what, void* host_out, int64_t capacity_bytes, int64_t* shape_out);
+
+/* Overwrites positions (and re-evaluates lnL/lnp) and/or the ladder.
+ * p: [T, W, ndim] host or NULL; betas: [T] host or NULL. */
+int emp_pt_set_state(EmpPT* pt, const double* p, const double* betas);
+
+/* Replaces: sampler.select_adjustment('00') (support/endit_freeze1.scr:11),
+ * which freezes the ladder; mode -1 = frozen, 0 = Vousden et al. 2016. */
+int emp_pt_set_adaptation(EmpPT* pt, int32_t adapt_mode, double adapt_tau, double adapt_nu);
+
+/* No reference counterpart (the reference cannot resume a run): binary
+ * checkpoint of the full engine state, bitwise-resumable. */
+int emp_pt_checkpoint(EmpPT* pt, const char* path);
+int emp_pt_restore(EmpPT* pt, const char* path);
+
+/* Average device time of the last emp_pt_run's sweeps in milliseconds
+ * (HIP events on the engine stream), and of its half-step kernels. */
+int emp_pt_last_timing(EmpPT* pt, double* ms_per_sweep, double* ms_halfstep_kernel);
+
+/* No reference counterpart (measurement): launches n fused stretch half-step
+ * kernels one by one, bracketing each with HIP events on the engine stream,
+ * and returns their mean duration.  The launches advance the chain. */
+int emp_pt_time_halfstep(EmpPT* pt, int32_t n, double* ms_avg);
+
+/* ---------------------------------------------------------------------------
+ * Misc.
+ * ------------------------------------------------------------------------ */
+/* No reference counterpart (test hook, needs no device): the engine's
+ * Philox-4x32-10 block function, for known-answer tests of the RNG. */
+int emp_philox4x32_10(const uint32_t* ctr4, const uint32_t* key2, uint32_t* out4);
+
+const char* emp_last_error(void);
+int emp_abi_version(void);
+/* Device properties used by the roofline report: compute units, max clock (kHz), gcnArch name. */
+int emp_device_info(int32_t* n_cu, int32_t* clock_khz, char* arch_name, int32_t arch_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMP_ABI_H */

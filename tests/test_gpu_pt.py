@@ -1,0 +1,229 @@
+"""GPU parallel-tempering engine against the oracle's moves.
+
+* replay: the device's random numbers (Philox counters as documented in
+  emp_pt.hip) are regenerated on the host and fed to the oracle's stretch
+  and swap moves; positions, lnL, ln prior and acceptances must agree;
+* tempered Gaussian: offsets-only model with fixed jitter, whose posterior at
+  inverse temperature beta is N(m, v/beta) per instrument;
+* checkpoint/restore and graph/direct replay are bitwise identical.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import make_sim, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _philox(ctr, key):
+    from astroemperor_amd import _abi
+    c = np.array([v & 0xFFFFFFFF for v in ctr], dtype=np.uint32)
+    k = np.array([v & 0xFFFFFFFF for v in key], dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    _abi.check(_abi.lib().emp_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data))
+    return [int(v) for v in o]
+
+
+def _u01(hi, lo):
+    v = ((hi << 32) | lo) >> 11
+    return (v + 0.5) * 2.0 ** -53
+
+
+M32 = 0xFFFFFFFF
+
+
+def _feistel(i, n, k0, k1):
+    bits = 2
+    while (1 << bits) < n:
+        bits += 2
+    half = bits // 2
+    mask = (1 << half) - 1
+    x = i
+    while True:
+        L, R = x >> half, x & mask
+        for r in range(4):
+            h = ((R * 0x9E3779B1) & M32) ^ ((k0 + r * 0x85EBCA77) & M32)
+            h ^= h >> 15
+            h = (h * (0x2C1B3C6D ^ k1)) & M32
+            h ^= h >> 12
+            L, R = R, (L ^ h) & mask
+        x = (L << half) | R
+        if x < n:
+            return x
+
+
+class _Post:
+    def __init__(self, o):
+        self.o = o
+
+    def prior(self, th):
+        return self.o.my_prior(th)
+
+    def like(self, th):
+        return self.o.my_likelihood(th)
+
+
+def _replay_sweep(o, X, lnl, lnp, betas, sweep, seed, a=2.0):
+    from oracle import emp_oracle as orc
+    T, W, D = X.shape
+    W2 = W // 2
+    k0, k1 = seed & M32, (seed >> 32) & M32
+    post = _Post(o)
+    acc = np.zeros((T, W), dtype=int)
+    for half in (0, 1):
+        for t in range(T):
+            active = list(range(half * W2, (half + 1) * W2))
+            partner, uz, ua = [], [], []
+            for k in active:
+                wid = t * W + k
+                r1 = _philox([wid, sweep & M32, sweep >> 32, half], [k0, k1])
+                r2 = _philox([wid, sweep & M32, sweep >> 32, half | 0x80000000], [k0, k1])
+                uz.append(_u01(r1[0], r1[1]))
+                partner.append((1 - half) * W2 + int(_u01(r1[2], r1[3]) * W2))
+                ua.append(_u01(r2[0], r2[1]))
+            got = orc.stretch_halfstep(X[t], lnl[t], lnp[t], betas[t], active, partner, uz, ua, post, a)
+            acc[t, active] += got
+    for h in range(T - 1, 0, -1):
+        c = h - 1
+        perm = [_feistel(i, W, k0 ^ (sweep & M32) ^ 0x5bd1e995,
+                         k1 ^ ((h * 0x27d4eb2f) & M32) ^ (sweep >> 32)) for i in range(W)]
+        u = []
+        for i in range(W):
+            r = _philox([i, sweep & M32, sweep >> 32, 0x40000000 | h], [k0, k1])
+            u.append(_u01(r[0], r[1]))
+        orc.swap_pair(X[h], lnl[h], lnp[h], X[c], lnl[c], lnp[c], betas[h], betas[c], perm, u)
+    return acc
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_replay_matches_oracle(precision):
+    from astroemperor_amd import PTSampler
+    from astroemperor_amd.simulation import test_init
+    sim = make_sim("51Peg", 1, parameterisation=1,
+                   conditions=[("Period 1", "limits", [4.0, 4.5]), ("Amplitude 1", "limits", [40, 70])])
+    m = sim.model
+    T, W, seed = 3, 16, 0x1234_5678_9abc
+    betas = np.array([1.0, 0.3, 0.05])
+    p0 = test_init(m, T, W, np.random.default_rng(1))
+    s = PTSampler(W, m.ndim__, m.my_likelihood, m.my_prior, ntemps=T, betas=betas, adapt_mode=-1,
+                  seed=seed, precision=precision)
+    s.run_mcmc(p0, nsweeps=3, use_graph=False)
+    chain = s.get_chain()                       # [T, S, W, D]
+    o = oracle_for(m)
+    X = p0.copy()
+    lnl = np.array([[o.my_likelihood(x) for x in X[t]] for t in range(T)])
+    lnp = np.array([[o.my_prior(x) for x in X[t]] for t in range(T)])
+    acc_tot = np.zeros((T, W), dtype=int)
+    for sweep in range(3):
+        acc_tot += _replay_sweep(o, X, lnl, lnp, betas, sweep, seed)
+        assert np.allclose(chain[:, sweep], X, rtol=1e-12, atol=1e-12), sweep
+    # FP64 path: 1e-10 relative; mixed path: within 1e-6 absolute of the FP64 value
+    rtol, atol = (1e-10, 0.0) if precision == "fp64" else (0.0, 1e-6)
+    assert np.allclose(s.get_log_like()[:, -1], lnl, rtol=rtol, atol=atol)
+    assert np.array_equal(s.get_log_prior()[:, -1], lnp)
+    frac = s.acceptance_fraction
+    assert np.array_equal(np.round(frac * 3).astype(int), acc_tot)
+    assert acc_tot.sum() > 0
+
+
+def _gaussian_model(rng):
+    """Three instruments, offsets free, jitter fixed: lnL is Gaussian in the offsets."""
+    from astroemperor_amd import JitterBlock, OffsetBlock, RVData, ReddModel
+    n = [12, 20, 30]
+    X = [np.sort(rng.uniform(0, 500, k)) for k in n]
+    Y = [rng.normal(0, 3, k) + off for k, off in zip(n, [5.0, -2.0, 0.0])]
+    E = [rng.uniform(1, 3, k) for k in n]
+    data = RVData.from_arrays(X, Y, E)
+    ob, jb = OffsetBlock(3), JitterBlock(3)
+    for p in ob:
+        p.prior, p.limits = "Uniform", [-200.0, 200.0]
+    for p, v in zip(jb, [1.0, 2.0, 0.5]):
+        p.fixed = v
+    m = ReddModel([ob, jb], data)
+    err2 = data.YERR ** 2 + np.array([1.0, 2.0, 0.5])[data.FLAG - 1] ** 2
+    mean = np.array([np.sum(data.Y[data.FLAG == k] / err2[data.FLAG == k]) /
+                     np.sum(1 / err2[data.FLAG == k]) for k in (1, 2, 3)])
+    var = np.array([1 / np.sum(1 / err2[data.FLAG == k]) for k in (1, 2, 3)])
+    return m, mean, var
+
+
+def test_tempered_gaussian_moments():
+    from astroemperor_amd import PTSampler
+    rng = np.random.default_rng(2024)
+    m, mean, var = _gaussian_model(rng)
+    betas = np.array([1.0, 0.5, 0.2, 0.05])
+    T, W, S = len(betas), 64, 4000
+    p0 = mean + rng.normal(0, 1, size=(T, W, 3)) * np.sqrt(var) * 3
+    s = PTSampler(W, 3, m, ntemps=T, betas=betas, adapt_mode=-1, seed=99)
+    s.run_mcmc(p0, nsweeps=S)
+    chain = s.get_chain(discard=500)               # [T, S', W, 3]
+    ll = s.get_log_like(discard=500)
+    lmax = m.my_likelihood(mean)
+    for t, b in enumerate(betas):
+        x = chain[t].reshape(-1, 3)
+        # ~ (S-500) W / tau samples with tau ~ 10-20: 1e4+ effective
+        assert np.all(np.abs(x.mean(0) - mean) < 6 * np.sqrt(var / b / 5000)), (t, x.mean(0), mean)
+        assert np.all(np.abs(x.var(0) / (var / b) - 1) < 0.08), (t, x.var(0) / (var / b))
+        # E_beta[lnL] = lnL_max - D / (2 beta)
+        # std of the mean of lnL: sqrt(D/2)/beta/sqrt(N_eff), N_eff >= 2000
+        assert abs(ll[t].mean() - (lmax - 3 / (2 * b))) < 5 * np.sqrt(1.5) / b / np.sqrt(2000), (t, ll[t].mean())
+    tsw = s.get_tsw()
+    assert tsw.shape == (S, T - 1) and 0.05 < tsw.mean() < 0.95
+    af = s.acceptance_fraction
+    assert np.all((af > 0.2) & (af < 0.9))
+
+
+def test_checkpoint_restore_bitwise(tmp_path):
+    from astroemperor_amd import PTSampler
+    from astroemperor_amd.simulation import test_init
+    sim = make_sim("GJ876", 1, parameterisation=1, moav={"order": 1, "global": False}, semantics="intended")
+    m = sim.model
+    T, W = 4, 32
+    p0 = test_init(m, T, W, np.random.default_rng(3))
+    s = PTSampler(W, m.ndim__, m, ntemps=T, seed=7, adapt_mode=0)
+    s.run_mcmc(p0, nsweeps=10, capacity=40)
+    s.checkpoint(tmp_path / "ck.bin")
+    s.run_mcmc(None, nsweeps=10)
+    a = s.get_last_sample()
+    s.restore(tmp_path / "ck.bin")
+    s.run_mcmc(None, nsweeps=10)
+    b = s.get_last_sample()
+    assert np.array_equal(a.coords, b.coords)
+    assert np.array_equal(a.log_like, b.log_like)
+    assert np.array_equal(a.betas, b.betas)
+
+
+def test_graph_equals_direct():
+    from astroemperor_amd import PTSampler
+    from astroemperor_amd.simulation import test_init
+    sim = make_sim("51Peg", 2, parameterisation=1)
+    m = sim.model
+    p0 = test_init(m, 5, 20, np.random.default_rng(4))
+    out = []
+    for g in (True, False):
+        s = PTSampler(20, m.ndim__, m, ntemps=5, seed=11)
+        s.run_mcmc(p0, nsweeps=25, use_graph=g)
+        out.append((s.get_chain(), s.get_log_like(), s.get_betas(), s.get_tsw(), s.get_smd()))
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
+def test_ladder_adaptation_matches_oracle():
+    from astroemperor_amd import PTSampler
+    from astroemperor_amd.simulation import test_init
+    from oracle import emp_oracle as orc
+    sim = make_sim("51Peg", 1, parameterisation=1)
+    m = sim.model
+    T, W = 6, 32
+    betas = np.array([1.0, 0.6, 0.3, 0.1, 0.02, 0.0])
+    p0 = test_init(m, T, W, np.random.default_rng(5))
+    s = PTSampler(W, m.ndim__, m, ntemps=T, betas=betas, seed=3, adapt_tau=50, adapt_nu=2)
+    s.run_mcmc(p0, nsweeps=30)
+    bh = s.get_betas()                  # ladder after each sweep
+    tsw = s.get_tsw()
+    prev = betas
+    for n in range(30):
+        want = orc.ladder_update(prev, tsw[n], n, 50.0, 2.0)
+        assert np.allclose(bh[n], want, rtol=1e-12, atol=0), n
+        prev = bh[n]
+    assert bh[-1][0] == 1.0 and bh[-1][-1] == 0.0 and np.all(np.diff(bh[-1]) < 0)
