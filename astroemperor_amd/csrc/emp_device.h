@@ -144,10 +144,47 @@ __device__ __forceinline__ double planet_M(const PlanetC& pc, double t) {
 }
 
 // ----------------------------------------------------------------------------
-// FP64 Kepler solve (Markley 1995).  Input: M reduced to [-pi, pi].
-// Output: sin E, cos E.
+// FP64 sin/cos on [0, pi] from a table of (sin, cos) at k pi / SC_N held in
+// LDS: x = k h + d with |d| <= h/2 = pi/512, and
+//   sin x = s_k cos d + c_k sin d,  cos x = c_k cos d - s_k sin d,
+//   sin d = d - d^3/6 + d^5/120,    cos d = 1 - d^2/2 + d^4/24 - d^6/720
+// (series truncation below 1e-19).  The table is filled on the host in
+// extended precision.  Callers clamp the argument into [0, pi].
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, double& sE, double& cE) {
+constexpr int SC_N = 256;
+constexpr double SC_H = PI_D / SC_N;                        // exact: power-of-two scaling
+constexpr double SC_H_LO = 1.2246467991473532e-16 / SC_N;  // (pi - PI_D) / SC_N
+constexpr double SC_INV_H = SC_N / PI_D;
+
+struct SinCosTab { double2 v[SC_N + 1]; };
+
+__device__ __forceinline__ void sincos_tab(const SinCosTab& tb, double x, double& s, double& c) {
+    const int k = (int)__builtin_rint(x * SC_INV_H);
+    const double kd = (double)k;
+    const double d = __builtin_fma(-kd, SC_H_LO, __builtin_fma(-kd, SC_H, x));
+    const double2 sc = tb.v[k];
+    const double d2 = d * d;
+    const double sd = __builtin_fma(d * d2, __builtin_fma(d2, 1.0 / 120.0, -1.0 / 6.0), d);
+    const double cd = __builtin_fma(d2, __builtin_fma(d2, __builtin_fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+    s = __builtin_fma(sc.x, cd, sc.y * sd);
+    c = __builtin_fma(sc.y, cd, -sc.x * sd);
+}
+
+// x^(2/3) for x >= 0 in FP64: hardware FP32 log2/exp2 seed (~1e-7) and one
+// Newton step on w^3 = x^2, w <- (2 w + x^2 / w^2) / 3 (quadratic: ~1e-14).
+__device__ __forceinline__ double pow23_fp64(double x) {
+    const float xf = (float)x;
+    const double w0 = (double)__builtin_amdgcn_exp2f(0.6666667f * __builtin_amdgcn_logf(xf));
+    const double w = (2.0 * w0 + x * x * rcp64(w0 * w0)) * (1.0 / 3.0);
+    return xf > 0.0f ? w : 0.0;
+}
+
+// ----------------------------------------------------------------------------
+// FP64 Kepler solve (Markley 1995).  Input: M reduced to [-pi, pi].
+// Output: sin E, cos E (and E for M >= 0 through kepler_fp64_E).
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ void kepler_fp64_E(double M, const PlanetC& pc, const SinCosTab& tb, double& E,
+                                              double& sE, double& cE) {
     const double e = pc.e;
     const double Ma = __builtin_fabs(M);
     const double ome = 1.0 - e;
@@ -158,40 +195,78 @@ __device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, double&
     const double q = __builtin_fma(2.0 * ad, ome, -M2);
     const double r = __builtin_fma(3.0 * ad * (d - ome), Ma, M2 * Ma);
     const double q2 = q * q;
-    double w = cbrt(__builtin_fabs(r) + sqrt(__builtin_fma(q2, q, r * r)));
-    w = w * w;
-    const double E1 = (2.0 * r * w / (__builtin_fma(w, w + q, q2)) + Ma) / d;
+    const double w = pow23_fp64(__builtin_fabs(r) + sqrt(__builtin_fma(q2, q, r * r)));
+    double E1 = __builtin_fma(2.0 * r * w, rcp64(__builtin_fma(w, w + q, q2)), Ma) * rcp64(d);
+    E1 = __builtin_fmin(__builtin_fmax(E1, 0.0), PI_D);
     double s, c;
-    sincos(E1, &s, &c);
+    sincos_tab(tb, E1, s, c);
     // Fifth-order Householder correction (Markley eqs. 20-22).
     const double f0 = E1 - __builtin_fma(e, s, Ma);
     const double f1 = __builtin_fma(-e, c, 1.0);
     const double f2 = e * s;
     const double f3 = 1.0 - f1;
-    const double d3 = -f0 / (f1 - 0.5 * f0 * f2 / f1);
-    const double d4 = -f0 / (f1 + 0.5 * d3 * f2 + d3 * d3 * f3 * (1.0 / 6.0));
-    const double d5 = -f0 / (f1 + 0.5 * d4 * f2 + d4 * d4 * f3 * (1.0 / 6.0)
-                             - d4 * d4 * d4 * f2 * (1.0 / 24.0));
+    const double dn = -f0 * rcp64(f1);
+    const double d3 = -f0 * rcp64(__builtin_fma(0.5 * dn, f2, f1));
+    const double d4 = -f0 * rcp64(f1 + 0.5 * d3 * f2 + d3 * d3 * f3 * (1.0 / 6.0));
+    const double d5 = -f0 * rcp64(f1 + 0.5 * d4 * f2 + d4 * d4 * f3 * (1.0 / 6.0)
+                                  - d4 * d4 * d4 * f2 * (1.0 / 24.0));
     // Rotate (s, c) by d5; |d5| < 1e-3 so three Taylor terms are exact to 1e-19.
     const double dd = d5 * d5;
     const double sd = d5 * __builtin_fma(-dd, 1.0 / 6.0, 1.0);
     const double cd = __builtin_fma(dd, __builtin_fma(dd, 1.0 / 24.0, -0.5), 1.0);
     sE = __builtin_fma(s, cd, c * sd);
     cE = __builtin_fma(c, cd, -s * sd);
+    E = E1 + d5;
     if (M < 0.0) sE = -sE;
 }
 
+__device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, const SinCosTab& tb, double& sE,
+                                            double& cE) {
+    double E;
+    kepler_fp64_E(M, pc, tb, E, sE, cE);
+}
+
 // ----------------------------------------------------------------------------
-// Mixed-precision Kepler solve.  The FP32 part (Markley starter with
-// hardware log2/exp2/rcp/sqrt and a third-order FP32 correction with the
-// hardware sin/cos) buys the iteration count; one FP64 Halley step from the
-// FP32 root (cubic convergence from ~1e-6) restores FP64 accuracy, so the
-// likelihood stays within 1e-6 of the FP64 path.
+// Per-vector Kepler tables (mixed path).  For one parameter vector the
+// eccentricity of a planet is fixed, so E(M) on [0, pi] is tabulated once at
+// KT_N nodes M_j = j h (h = pi / (KT_N - 1)) as (E_j, h dE/dM_j) in FP32;
+// each epoch then takes a cubic Hermite interpolant (error < 2e-5 rad for
+// e <= KT_EMAX) and one FP64 Halley step.  Above KT_EMAX the FP32 Markley
+// starter is used instead.
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ void kepler_mixed(double M, const PlanetC& pc, double& sE, double& cE) {
+constexpr int KT_N = 64;
+constexpr double KT_H = PI_D / (KT_N - 1);
+constexpr double KT_EMAX = 0.7;
+
+// Node `lane` of planet pc's table (called by one wave: 64 lanes, 64 nodes).
+__device__ __forceinline__ float2 ktab_node(const PlanetC& pc, const SinCosTab& tb, int lane) {
+    double E, s, c;
+    kepler_fp64_E(lane * KT_H, pc, tb, E, s, c);
+    return make_float2((float)E, (float)(KT_H * rcp64(__builtin_fma(-pc.e, c, 1.0))));
+}
+
+__device__ __forceinline__ float ktab_start(const float2* kt, float Mf) {
+    const float u = Mf * (float)((KT_N - 1) / PI_D);
+    const int j = min((int)u, KT_N - 2);
+    const float x = u - (float)j;
+    const float2 a = kt[j], b = kt[j + 1];
+    const float x2 = x * x;
+    const float h01 = x2 * __builtin_fmaf(-2.0f, x, 3.0f);      // 3x^2 - 2x^3
+    const float h11 = x2 * (x - 1.0f);                          // x^3 - x^2
+    const float h10 = __builtin_fmaf(x, (x - 1.0f) * (x - 1.0f), 0.0f);   // x (1 - x)^2
+    return __builtin_fmaf(b.x - a.x, h01, a.x) + __builtin_fmaf(a.y, h10, b.y * h11);
+}
+
+// ----------------------------------------------------------------------------
+// Mixed-precision Kepler solve.  FP32 buys the iteration count: the starter
+// comes from the per-vector FP32 table (e <= KT_EMAX) or, above it, from
+// Markley's starter in FP32 (hardware log2/exp2/rcp/sqrt) plus one FP32
+// Halley step with the hardware sin/cos.  One FP64 Halley step from that
+// root (cubic convergence from <= 2e-5) and the FP64 projection restore FP64
+// accuracy, so the likelihood stays within 1e-6 of the FP64 path.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) {
     const float ef = (float)pc.e;
-    const double Ma = __builtin_fabs(M);
-    const float Mf = (float)Ma;
     const float omef = 1.0f - ef;
     const float alpha = __builtin_fmaf((float)pc.mk_a1, (float)PI_D - Mf, (float)MK_A0);
     const float d = __builtin_fmaf(alpha, ef, (float)pc.mk_d0);
@@ -213,10 +288,15 @@ __device__ __forceinline__ void kepler_mixed(double M, const PlanetC& pc, double
         const float dn = -f0 * __builtin_amdgcn_rcpf(f1);
         E += -f0 * __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f * dn, ef * s, f1));
     }
-    // FP64 Halley step from the FP32 root.
-    const double E0 = (double)E;
+    return E;
+}
+
+// One FP64 Halley step from E0 (Ma = |M| in [0, pi]); returns sin E, cos E.
+__device__ __forceinline__ void kepler_fp64_halley(double Ma, float Ef, const PlanetC& pc, const SinCosTab& tb,
+                                                   double& sE, double& cE) {
+    const double E0 = __builtin_fmin(__builtin_fmax((double)Ef, 0.0), PI_D);
     double s, c;
-    sincos(E0, &s, &c);
+    sincos_tab(tb, E0, s, c);
     const double e = pc.e;
     const double f0 = E0 - __builtin_fma(e, s, Ma);
     const double f1 = __builtin_fma(-e, c, 1.0);
@@ -228,15 +308,24 @@ __device__ __forceinline__ void kepler_mixed(double M, const PlanetC& pc, double
     const double cd = __builtin_fma(dd, __builtin_fma(dd, 1.0 / 24.0, -0.5), 1.0);
     sE = __builtin_fma(s, cd, c * sd);
     cE = __builtin_fma(c, cd, -s * sd);
-    if (M < 0.0) sE = -sE;
 }
 
+// Planet term at time t.  PREC 0: FP64 Markley; PREC 1: mixed, with `kt` the
+// planet's per-vector table (read only when pc.e <= KT_EMAX, a per-vector,
+// hence wave-uniform, branch).
 template <int PREC>
-__device__ __forceinline__ double planet_mu(const PlanetC& pc, double t) {
+__device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t) {
     const double M = reduce_2pi(planet_M(pc, t));
     double sE, cE;
-    if (PREC == 0) kepler_fp64(M, pc, sE, cE);
-    else kepler_mixed(M, pc, sE, cE);
+    if (PREC == 0) {
+        kepler_fp64(M, pc, tb, sE, cE);
+    } else {
+        const double Ma = __builtin_fabs(M);
+        const float Mf = (float)Ma;
+        const float Ef = pc.e <= KT_EMAX ? ktab_start(kt, Mf) : kepler_fp32_start(Mf, pc);
+        kepler_fp64_halley(Ma, Ef, pc, tb, sE, cE);
+        if (M < 0.0) sE = -sE;
+    }
     const double den = __builtin_fma(-pc.e, cE, 1.0);
     return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64(den);
 }

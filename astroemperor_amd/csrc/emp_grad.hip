@@ -42,11 +42,9 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     if (live) expand_theta(md, sl, theta + b * ld, g, G);
     for (int j = g; j < MAXD; j += G) gl.g[j] = 0.0;
     for (int j = g; j < KMAX * 5; j += G) (&gl.nat[0][0])[j] = 0.0;
+    tab_load(md, s_tile);
     __syncthreads();
-    if (live && wsub == 0) {
-        slot_setup(md, sl, lane);
-    }
-    __syncthreads();
+    slot_prepare<0, WPT>(md, sl, s_tile, live, wsub, lane);
 
     double acc_nat[KMAX][5];
 #pragma unroll
@@ -84,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     const PlanetC& pc = sl.planet[p];
                     const double M = reduce_2pi(planet_M(pc, t));
                     double sE, cE;
-                    kepler_fp64(M, pc, sE, cE);
+                    kepler_fp64(M, pc, s_tile.tab, sE, cE);
                     const double e = pc.e;
                     const double rD = rcp64(__builtin_fma(-e, cE, 1.0));
                     const double bq = sqrt((1.0 - e) * (1.0 + e));

@@ -15,7 +15,7 @@ constexpr int IMAX = 16;         // instruments per model
 constexpr int AMAX = 4;          // acceleration order
 constexpr int BLOCK = 512;       // threads per workgroup (8 waves)
 constexpr int NWAVES = BLOCK / 64;
-constexpr int TILE = 1024;       // epochs staged in LDS per tile
+constexpr int TILE = 512;        // epochs staged in LDS per tile
 
 // Everything a kernel needs to know about a model, passed by value.
 struct ModelDev {
@@ -34,6 +34,7 @@ struct ModelDev {
     const EmpPriorTerm* prior;        // [Dfull]
     const int* block_start;           // [nblocks + 1]
     const EmpPriorTerm* ecc_prior;    // [K]
+    const double2* sctab;             // [SC_N + 1] (sin, cos) of k pi / SC_N
     double lnl_const;                 // -0.5 ln(2 pi) N
     double t0;                        // X_[0] (acceleration origin)
 };
@@ -44,6 +45,7 @@ struct SlotLDS {
     PlanetC planet[KMAX];
     double gam[IMAX], jit2[IMAX], ma_c[IMAX], ma_tau[IMAX], jit[IMAX];
     double acc[AMAX];
+    float2 ktab[KMAX][KT_N];   // mixed path: (E_j, h dE/dM_j) per planet
 };
 
 }  // namespace emp

@@ -79,7 +79,13 @@ __device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
 struct TileLDS {
     double t[TILE], y[TILE], s2[TILE], dt[TILE];
     int ins[TILE];
+    SinCosTab tab;
 };
+
+// The sin/cos table is read by every planet-epoch; load it once per workgroup.
+__device__ __forceinline__ void tab_load(const ModelDev& md, TileLDS& tl) {
+    for (int j = threadIdx.x; j <= SC_N; j += BLOCK) tl.tab.v[j] = md.sctab[j];
+}
 
 __device__ __forceinline__ void tile_load(const ModelDev& md, TileLDS& tl, int tb, int tn) {
     for (int j = threadIdx.x; j < tn; j += BLOCK) {
@@ -91,9 +97,24 @@ __device__ __forceinline__ void tile_load(const ModelDev& md, TileLDS& tl, int t
     }
 }
 
+// Per-slot constants, then (mixed path) the per-vector Kepler tables, one
+// planet per wave of the slot.  The sin/cos table must already be in LDS.
+// All threads of the workgroup call this (it holds two barriers).
+template <int PREC, int WPT>
+__device__ __forceinline__ void slot_prepare(const ModelDev& md, SlotLDS& sl, const TileLDS& tl, bool live,
+                                             int wsub, int lane) {
+    if (live && wsub == 0) slot_setup(md, sl, lane);
+    __syncthreads();
+    if (PREC == 1 && live) {
+        for (int p = wsub; p < md.K; p += WPT)
+            if (sl.planet[p].e <= KT_EMAX) sl.ktab[p][lane] = ktab_node(sl.planet[p], tl.tab, lane);
+    }
+    __syncthreads();
+}
+
 // Model value (planets + offset + acceleration) of one epoch.
 template <int PREC>
-__device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl, double t, int k) {
+__device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t, int k) {
     double mu = sl.gam[k];
     if (md.acc) {
         const double x = t - md.t0;
@@ -101,7 +122,7 @@ __device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl
         for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
         mu += v * x;
     }
-    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], t);
+    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t);
     return mu;
 }
 
@@ -126,7 +147,8 @@ struct ScanLDS {
 
 // lnL of the slot's parameter vector.  All threads of the workgroup must
 // call this together (it contains workgroup barriers); `live` == false makes
-// a slot skip the arithmetic while still taking part in the barriers.
+// a slot skip the arithmetic while still taking part in the barriers.  The
+// slot must be prepared (slot_prepare) and the sin/cos table loaded.
 // With WRITE the per-epoch model and err2 are stored (device order).
 template <int PREC, int WPT, bool WRITE>
 __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
@@ -151,7 +173,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
             int k = 0;
             if (valid) {
                 k = tl.ins[j];
-                mu = epoch_mu<PREC>(md, sl, tl.t[j], k);
+                mu = epoch_mu<PREC>(md, sl, tl.tab, tl.t[j], k);
                 err2 = tl.s2[j] + sl.jit2[k];
                 r = tl.y[j] - mu;
             }

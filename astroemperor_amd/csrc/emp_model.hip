@@ -91,9 +91,9 @@ __global__ __launch_bounds__(BLOCK) void k_lnl(ModelDev md, const double* __rest
     const bool live = b < B;
     SlotLDS& sl = s_slot[slot];
     if (live) expand_theta(md, sl, theta + b * ld, wsub * 64 + lane, 64 * WPT);
+    tab_load(md, s_tile);
     __syncthreads();
-    if (live && wsub == 0) slot_setup(md, sl, lane);
-    __syncthreads();
+    slot_prepare<PREC, WPT>(md, sl, s_tile, live, wsub, lane);
     double v = block_lnl<PREC, WPT, false>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
     if (live && wsub == 0 && lane == 0) {
         if (!(v == v) || v == __builtin_inf()) {
@@ -125,9 +125,9 @@ __global__ __launch_bounds__(BLOCK) void k_model_eval(ModelDev md, const double*
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     SlotLDS& sl = s_slot[0];
     expand_theta(md, sl, theta, threadIdx.x, BLOCK);
+    tab_load(md, s_tile);
     __syncthreads();
-    if (wave == 0) slot_setup(md, sl, lane);
-    __syncthreads();
+    slot_prepare<0, NWAVES>(md, sl, s_tile, true, wave, lane);
     block_lnl<0, NWAVES, true>(md, sl, s_tile, s_scan, true, wave, out_model, out_err2);
 }
 
@@ -320,6 +320,15 @@ extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
     EMP_UP(d->prior, (size_t)d->ndim_full, md.prior);
     EMP_UP(bstart.data(), bstart.size(), md.block_start);
     EMP_UP(ecc.data(), ecc.size(), md.ecc_prior);
+    {   // (sin, cos) of k pi / SC_N in extended precision, rounded to double
+        std::vector<double2> tab(SC_N + 1);
+        const long double pi_l = 3.141592653589793238462643383279502884L;
+        for (int k = 0; k <= SC_N; ++k) {
+            const long double x = pi_l * (long double)k / (long double)SC_N;
+            tab[k] = make_double2((double)sinl(x), (double)cosl(x));
+        }
+        EMP_UP(tab.data(), tab.size(), md.sctab);
+    }
 #undef EMP_UP
     void* nf = nullptr;
     if (hipMalloc(&nf, sizeof(unsigned long long)) != hipSuccess) {
