@@ -14,3 +14,4 @@ from .data import DataWrapper, RVData  # noqa: F401
 from .model import ReddModel  # noqa: F401
 from .sampler import PTSampler  # noqa: F401
 from .simulation import Simulation  # noqa: F401
+from .systems import SystemsPT  # noqa: F401

@@ -54,6 +54,19 @@ class EmpPTDesc(C.Structure):
                 ("adapt_nu", C.c_double), ("max_sweeps", C.c_int64)]
 
 
+class EmpSystemsDesc(C.Structure):
+    _fields_ = [("n_systems", C.c_int32), ("ntemps", C.c_int32), ("nwalkers", C.c_int32),
+                ("nsteps", C.c_int32), ("precision", C.c_int32), ("adapt_mode", C.c_int32),
+                ("a", C.c_double), ("seed", C.c_uint64), ("adapt_tau", C.c_double),
+                ("adapt_nu", C.c_double), ("record_every", C.c_int32), ("record_temps", C.c_int32),
+                ("max_records", C.c_int64)]
+
+
+class EmpSystemsTuning(C.Structure):
+    _fields_ = [("waves_per_wg", C.c_int32), ("waves_per_theta", C.c_int32), ("wgs_per_cu", C.c_int32),
+                ("order", C.c_int32), ("ms_per_sweep", C.c_double)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "emp_model_create": (C.c_int, [C.POINTER(EmpModelDesc), C.POINTER(_P)]),
@@ -75,6 +88,14 @@ _SIGS = {
     "emp_pt_restore": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "emp_pt_time_halfstep": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "emp_systems_create": (C.c_int, [_P, C.POINTER(EmpSystemsDesc), _P, _P, C.POINTER(_P)]),
+    "emp_systems_destroy": (None, [_P]),
+    "emp_systems_run": (C.c_int, [_P, C.c_int64, C.c_int32]),
+    "emp_systems_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(EmpSystemsTuning)]),
+    "emp_systems_set_tuning": (C.c_int, [_P, C.POINTER(EmpSystemsTuning)]),
+    "emp_systems_get_tuning": (C.c_int, [_P, C.POINTER(EmpSystemsTuning)]),
+    "emp_systems_get": (C.c_int, [_P, C.c_int32, C.c_int32, _P, C.c_int64, C.POINTER(C.c_int64)]),
+    "emp_systems_last_timing": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "emp_philox4x32_10": (C.c_int, [_P, _P, _P]),
     "emp_last_error": (C.c_char_p, []),
     "emp_abi_version": (C.c_int, []),

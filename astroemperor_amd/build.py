@@ -12,6 +12,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemp_mi355x.so")
 SOURCES = ["emp_model.hip", "emp_pt.hip", "emp_grad.hip", "emp_systems.hip"]
+# Per-source extra flags.  The batched-systems sweep runs the likelihood
+# inside loops over systems and walkers; machine LICM would hoist the 64-bit
+# constants of every inlined log/exp out of those loops and keep them live in
+# ~60 VGPRs through the hot planet loop (240+ VGPRs, or scratch spills at
+# 1024 threads).  Left in place they cost a scalar move where they are used.
+EXTRA = {"emp_systems.hip": ["-mllvm", "-disable-machine-licm"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          # HIP's default (fast) ignores `#pragma clang fp contract(off)`, which
@@ -22,19 +28,40 @@ FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Wno-unused-command-line-argument"]
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
+def _clean(stderr: str) -> str:
     noise = "'-packed-fp32-ops' is not a recognized feature"
-    err = "\n".join(l for l in (r.stderr or "").splitlines() if noise not in l)
+    return "\n".join(l for l in (stderr or "").splitlines() if noise not in l)
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every source to an object (in parallel), then links the .so."""
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = []
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", path, "-o", obj]
+        cmd.remove("-shared")
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+    objs, errors = [], []
+    for src, obj, proc in jobs:
+        _, err = proc.communicate()
+        err = _clean(err)
+        if proc.returncode != 0:
+            errors.append(f"{src}: hipcc failed ({proc.returncode}):\n{err}")
+        elif verbose and err.strip():
+            print(err)
+        objs.append(obj)
+    if errors:
+        raise RuntimeError("\n".join(errors))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *FLAGS, *objs, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{err}")
-    if verbose and err.strip():
-        print(err)
+        raise RuntimeError(f"hipcc link failed ({r.returncode}):\n{_clean(r.stderr)}")
     os.replace(tmp, LIB)
     return LIB
 

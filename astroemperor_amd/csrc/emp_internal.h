@@ -15,6 +15,7 @@ constexpr int IMAX = 16;         // instruments per model
 constexpr int AMAX = 4;          // acceleration order
 constexpr int BLOCK = 512;       // threads per workgroup (8 waves)
 constexpr int NWAVES = BLOCK / 64;
+constexpr int MAX_WAVES = 16;    // largest workgroup any kernel launches (1024 threads)
 constexpr int TILE = 512;        // epochs staged in LDS per tile
 
 // Everything a kernel needs to know about a model, passed by value.
@@ -63,6 +64,21 @@ struct EmpModel {
     int device = 0;
     int n_cu = 256;
 };
+
+// Host-side preparation of one model: validated, epochs in device order
+// (grouped by instrument when the per-instrument moving average is active),
+// MA time steps, free/fixed maps, prior block starts.  dev holds the scalars;
+// its pointers are left null for the caller to place the arrays.
+struct ModelHost {
+    emp::ModelDev dev{};
+    std::vector<int> order;   // device position -> caller position
+    std::vector<double> t, y, s2, dt, fixed;
+    std::vector<int> ins, inv, free_idx, bstart;
+    std::vector<EmpPriorTerm> prior, ecc;
+};
+int emp_model_host(const EmpModelDesc* d, ModelHost* h);
+// (sin, cos) of k pi / SC_N, k = 0..SC_N, rounded from extended precision.
+void emp_sincos_table(std::vector<double2>& tab);
 
 // Error handling.
 void emp_set_error(const char* fmt, ...);

@@ -84,11 +84,11 @@ struct TileLDS {
 
 // The sin/cos table is read by every planet-epoch; load it once per workgroup.
 __device__ __forceinline__ void tab_load(const ModelDev& md, TileLDS& tl) {
-    for (int j = threadIdx.x; j <= SC_N; j += BLOCK) tl.tab.v[j] = md.sctab[j];
+    for (int j = threadIdx.x; j <= SC_N; j += blockDim.x) tl.tab.v[j] = md.sctab[j];
 }
 
 __device__ __forceinline__ void tile_load(const ModelDev& md, TileLDS& tl, int tb, int tn) {
-    for (int j = threadIdx.x; j < tn; j += BLOCK) {
+    for (int j = threadIdx.x; j < tn; j += blockDim.x) {
         tl.t[j] = md.t[tb + j];
         tl.y[j] = md.y[tb + j];
         tl.s2[j] = md.s2[tb + j];
@@ -141,8 +141,8 @@ __device__ __forceinline__ void wave_affine_scan(double& a, double& b, int lane)
 }
 
 struct ScanLDS {
-    double tot[2][NWAVES][2];
-    double red[NWAVES][2];
+    double tot[2][MAX_WAVES][2];
+    double red[MAX_WAVES][2];
 };
 
 // lnL of the slot's parameter vector.  All threads of the workgroup must
@@ -150,9 +150,10 @@ struct ScanLDS {
 // a slot skip the arithmetic while still taking part in the barriers.  The
 // slot must be prepared (slot_prepare) and the sin/cos table loaded.
 // With WRITE the per-epoch model and err2 are stored (device order).
+// `resident`: the caller has already staged all epochs (md.N <= TILE) in tl.
 template <int PREC, int WPT, bool WRITE>
 __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
-                            bool live, int wsub, double* out_model, double* out_err2) {
+                            bool live, int wsub, double* out_model, double* out_err2, bool resident = false) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int G = 64 * WPT;
@@ -163,9 +164,11 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
     int parity = 0;
     for (int tb = 0; tb < md.N; tb += TILE) {
         const int tn = min(TILE, md.N - tb);
-        __syncthreads();
-        tile_load(md, tl, tb, tn);
-        __syncthreads();
+        if (!(resident && md.N <= TILE)) {
+            __syncthreads();
+            tile_load(md, tl, tb, tn);
+            __syncthreads();
+        }
         for (int cb = 0; cb < tn; cb += G) {
             const int j = cb + g;
             const bool valid = live && (j < tn);

@@ -242,9 +242,7 @@ static int validate(const EmpModelDesc* d) {
     return EMP_OK;
 }
 
-extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
-    if (!out) { emp_set_error("null output handle"); return EMP_EINVAL; }
-    *out = nullptr;
+int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
     int rc = validate(d);
     if (rc) return rc;
     const bool ma_active = d->ma_order > 0 && (d->ma_global || d->semantics == EMP_SEM_INTENDED);
@@ -252,50 +250,44 @@ extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
         emp_set_error("moving-average order %d: the kernels implement order 1", d->ma_order);
         return EMP_EUNSUP;
     }
-    EmpModel* m = new EmpModel();
-    m->desc = *d;
-    EMP_HIP(hipGetDevice(&m->device));
-    {
-        hipDeviceProp_t p;
-        EMP_HIP(hipGetDeviceProperties(&p, m->device));
-        m->n_cu = p.multiProcessorCount;
-    }
     const int N = d->n_epochs;
     // Device order: per-instrument MA needs each instrument's time series
     // contiguous; otherwise keep the caller's (time) order.
     const bool by_ins = ma_active && !d->ma_global;
-    m->order.resize(N);
-    for (int i = 0; i < N; ++i) m->order[i] = i;
-    if (by_ins) std::stable_sort(m->order.begin(), m->order.end(), [&](int a, int b) { return d->ins[a] < d->ins[b]; });
-    std::vector<double> t(N), y(N), s2(N), dt(N);
-    std::vector<int> ins(N);
+    h->order.resize(N);
+    for (int i = 0; i < N; ++i) h->order[i] = i;
+    if (by_ins) std::stable_sort(h->order.begin(), h->order.end(), [&](int a, int b) { return d->ins[a] < d->ins[b]; });
+    h->t.resize(N); h->y.resize(N); h->s2.resize(N); h->dt.resize(N); h->ins.resize(N);
     for (int i = 0; i < N; ++i) {
-        const int o = m->order[i];
-        t[i] = d->t[o];
-        y[i] = d->y[o];
-        s2[i] = d->yerr[o] * d->yerr[o];
-        ins[i] = d->ins[o];
+        const int o = h->order[i];
+        h->t[i] = d->t[o];
+        h->y[i] = d->y[o];
+        h->s2[i] = d->yerr[o] * d->yerr[o];
+        h->ins[i] = d->ins[o];
     }
     for (int i = 0; i < N; ++i) {
-        const bool start = i == 0 || (by_ins && ins[i] != ins[i - 1]);
-        dt[i] = start ? -1.0 : std::fabs(t[i] - t[i - 1]);
+        const bool start = i == 0 || (by_ins && h->ins[i] != h->ins[i - 1]);
+        h->dt[i] = start ? -1.0 : std::fabs(h->t[i] - h->t[i - 1]);
     }
-    std::vector<int> inv(d->ndim_full, -1);
-    for (int k = 0; k < d->ndim; ++k) inv[d->free_index[k]] = k;
-    std::vector<int> bstart(d->n_blocks + 1, d->ndim_full);
+    h->inv.assign(d->ndim_full, -1);
+    for (int k = 0; k < d->ndim; ++k) h->inv[d->free_index[k]] = k;
+    h->free_idx.assign(d->free_index, d->free_index + d->ndim);
+    h->fixed.assign(d->fixed_value, d->fixed_value + d->ndim_full);
+    h->prior.assign(d->prior, d->prior + d->ndim_full);
+    h->bstart.assign(d->n_blocks + 1, d->ndim_full);
     for (int b = d->n_blocks - 1; b >= 0; --b) {
-        int s = bstart[b + 1];
+        int s = h->bstart[b + 1];
         for (int j = 0; j < d->ndim_full; ++j) if (d->prior[j].block == b) { s = j; break; }
-        bstart[b] = s;
+        h->bstart[b] = s;
     }
-    bstart[0] = 0;
-    std::vector<EmpPriorTerm> ecc(std::max(d->n_planets, 1));
-    for (int k = 0; k < d->n_planets; ++k) {
-        if (d->ecc_prior) ecc[k] = d->ecc_prior[k];
-        else { ecc[k] = EmpPriorTerm{}; ecc[k].kind = EMP_PRIOR_NONE; }
+    h->bstart[0] = 0;
+    h->ecc.resize(std::max(d->n_planets, 1));
+    for (int k = 0; k < (int)h->ecc.size(); ++k) {
+        if (d->ecc_prior && k < d->n_planets) h->ecc[k] = d->ecc_prior[k];
+        else { h->ecc[k] = EmpPriorTerm{}; h->ecc[k].kind = EMP_PRIOR_NONE; }
     }
-
-    ModelDev& md = m->dev;
+    ModelDev& md = h->dev;
+    md = ModelDev{};
     md.N = N; md.I = d->n_ins; md.K = d->n_planets; md.param = d->parameterisation; md.use_c = d->use_c;
     md.acc = d->acc_order; md.has_jit = d->has_jitter;
     md.ma_q = d->ma_order; md.ma_global = d->ma_global; md.ma_active = ma_active ? 1 : 0;
@@ -304,30 +296,55 @@ extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
     md.off_jit = d->off_jitter; md.off_ma = d->off_ma;
     md.lnl_const = -0.5 * std::log(2.0 * M_PI) * (double)N;
     md.t0 = d->t[0];
-#define EMP_UP(host, n, field)                                   \
+    return EMP_OK;
+}
+
+void emp_sincos_table(std::vector<double2>& tab) {
+    tab.resize(SC_N + 1);
+    const long double pi_l = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k <= SC_N; ++k) {
+        const long double x = pi_l * (long double)k / (long double)SC_N;
+        tab[k] = make_double2((double)sinl(x), (double)cosl(x));
+    }
+}
+
+extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
+    if (!out) { emp_set_error("null output handle"); return EMP_EINVAL; }
+    *out = nullptr;
+    ModelHost h;
+    int rc = emp_model_host(d, &h);
+    if (rc) return rc;
+    EmpModel* m = new EmpModel();
+    m->desc = *d;
+    EMP_HIP(hipGetDevice(&m->device));
+    {
+        hipDeviceProp_t p;
+        EMP_HIP(hipGetDeviceProperties(&p, m->device));
+        m->n_cu = p.multiProcessorCount;
+    }
+    m->order = h.order;
+    ModelDev& md = m->dev;
+    md = h.dev;
+#define EMP_UP(vec, field)                                       \
     do {                                                         \
-        int r_ = upload(m, host, n, &field);                     \
+        int r_ = upload(m, vec.data(), vec.size(), &field);      \
         if (r_) { emp_model_destroy(m); return r_; }             \
     } while (0)
-    EMP_UP(t.data(), (size_t)N, md.t);
-    EMP_UP(y.data(), (size_t)N, md.y);
-    EMP_UP(s2.data(), (size_t)N, md.s2);
-    EMP_UP(dt.data(), (size_t)N, md.dt);
-    EMP_UP(ins.data(), (size_t)N, md.ins);
-    EMP_UP(inv.data(), inv.size(), md.inv_free);
-    EMP_UP(d->free_index, (size_t)d->ndim, md.free_idx);
-    EMP_UP(d->fixed_value, (size_t)d->ndim_full, md.fixed_val);
-    EMP_UP(d->prior, (size_t)d->ndim_full, md.prior);
-    EMP_UP(bstart.data(), bstart.size(), md.block_start);
-    EMP_UP(ecc.data(), ecc.size(), md.ecc_prior);
-    {   // (sin, cos) of k pi / SC_N in extended precision, rounded to double
-        std::vector<double2> tab(SC_N + 1);
-        const long double pi_l = 3.141592653589793238462643383279502884L;
-        for (int k = 0; k <= SC_N; ++k) {
-            const long double x = pi_l * (long double)k / (long double)SC_N;
-            tab[k] = make_double2((double)sinl(x), (double)cosl(x));
-        }
-        EMP_UP(tab.data(), tab.size(), md.sctab);
+    EMP_UP(h.t, md.t);
+    EMP_UP(h.y, md.y);
+    EMP_UP(h.s2, md.s2);
+    EMP_UP(h.dt, md.dt);
+    EMP_UP(h.ins, md.ins);
+    EMP_UP(h.inv, md.inv_free);
+    EMP_UP(h.free_idx, md.free_idx);
+    EMP_UP(h.fixed, md.fixed_val);
+    EMP_UP(h.prior, md.prior);
+    EMP_UP(h.bstart, md.block_start);
+    EMP_UP(h.ecc, md.ecc_prior);
+    {
+        std::vector<double2> tab;
+        emp_sincos_table(tab);
+        EMP_UP(tab, md.sctab);
     }
 #undef EMP_UP
     void* nf = nullptr;

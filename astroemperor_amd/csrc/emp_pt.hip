@@ -19,7 +19,7 @@
 #include <cstring>
 #include <vector>
 
-#include "emp_kernels.h"
+#include "emp_pt_common.h"
 
 using namespace emp;
 
@@ -60,8 +60,14 @@ struct EmpPT {
     double last_ms_per_sweep = -1.0, last_ms_half = -1.0;
 };
 
+static __host__ __device__ inline PTView view_of(const PTDev& pd) {
+    return pt_view(pd.T, pd.W, pd.D, pd.a, pd.key0, pd.key1, 0u, pd.adapt_mode, pd.adapt_tau, pd.adapt_nu, pd.X,
+                   pd.lnl, pd.lnp, pd.map, pd.betas, pd.accepted);
+}
+
 // ---------------------------------------------------------------------------
-// Fused stretch half-step: propose, prior, likelihood, accept.
+// Fused stretch half-step: propose, prior, likelihood, accept; one slot of
+// the workgroup per walker of the moving half.
 // ---------------------------------------------------------------------------
 template <int PREC, int WPT>
 __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int half, int step) {
@@ -71,158 +77,46 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
     __shared__ ScanLDS s_scan;
     __shared__ double s_lp[SLOTS];
     __shared__ int s_acc[SLOTS];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int slot = wave / WPT, wsub = wave % WPT;
-    const int g = wsub * 64 + lane, G = 64 * WPT;
     const int64_t b = (int64_t)blockIdx.x * SLOTS + slot;
-    const bool task = b < (int64_t)pd.T * pd.W2;
-    SlotLDS& sl = s_slot[slot];
-    const long long sweep = *pd.sweep;
-    int t = 0, k = 0, sk = 0;
-    double z = 1.0, uacc = 0.5;
-    if (task) {
-        t = (int)(b / pd.W2);
-        k = half * pd.W2 + (int)(b % pd.W2);
-        const uint32_t wid = (uint32_t)(t * pd.W + k);
-        U4 c1{wid, (uint32_t)sweep, (uint32_t)((unsigned long long)sweep >> 32), (uint32_t)(step * 2 + half)};
-        U4 r1 = philox4x32_10(c1, pd.key0, pd.key1);
-        U4 c2 = c1;
-        c2.w |= 0x80000000u;
-        U4 r2 = philox4x32_10(c2, pd.key0, pd.key1);
-        const double uz = u01(r1.x, r1.y), uj = u01(r1.z, r1.w);
-        uacc = u01(r2.x, r2.y);
-        const double zz = (pd.a - 1.0) * uz + 1.0;
-        z = zz * zz / pd.a;                                  // Goodman & Weare (2010) eq. 9
-        const int j = (1 - half) * pd.W2 + (int)(uj * (double)pd.W2);
-        sk = pd.map[t * pd.W + k];
-        const int sj = pd.map[t * pd.W + j];
-        const double* Xk = pd.X + (size_t)sk * pd.D;
-        const double* Xj = pd.X + (size_t)sj * pd.D;
-        for (int d = g; d < md.D; d += G) sl.theta[md.free_idx[d]] = Xj[d] + z * (Xk[d] - Xj[d]);
-        for (int q = g; q < md.Dfull; q += G)
-            if (md.inv_free[q] < 0) sl.theta[q] = md.fixed_val[q];
-    }
-    tab_load(md, s_tile);
-    __syncthreads();
-    if (task && wsub == 0) {
-        if (lane == 0) s_lp[slot] = prior_seq(md, sl.theta);
-        slot_setup(md, sl, lane);
-    }
-    __syncthreads();
-    const double lp = task ? s_lp[slot] : -__builtin_inf();
-    const bool live = task && lp == lp && lp != -__builtin_inf();
-    if (PREC == 1 && live) {       // per-vector Kepler tables, one planet per wave
-        for (int p = wsub; p < md.K; p += WPT)
-            if (sl.planet[p].e <= KT_EMAX) sl.ktab[p][lane] = ktab_node(sl.planet[p], s_tile.tab, lane);
-    }
-    __syncthreads();
-    double ll = block_lnl<PREC, WPT, false>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
-    if (task && wsub == 0 && lane == 0) {
-        int acc = 0;
-        if (live) {
-            atomicAdd((unsigned long long*)(pd.sweep + 1), 1ull);
-            if (!(ll == ll) || ll == __builtin_inf()) ll = -__builtin_inf();
-            const double beta = pd.betas[t];
-            const double tn = beta == 0.0 ? 0.0 : beta * ll;
-            const double to = beta == 0.0 ? 0.0 : beta * pd.lnl[sk];
-            const double lnalpha = (double)(md.D - 1) * log(z) + (tn - to) + (lp - pd.lnp[sk]);
-            if (log(uacc) < lnalpha) {
-                pd.lnl[sk] = ll;
-                pd.lnp[sk] = lp;
-                pd.accepted[t * pd.W + k] += 1;
-                acc = 1;
-            }
-        }
-        s_acc[slot] = acc;
-    }
-    __syncthreads();
-    if (task && s_acc[slot]) {
-        double* Xk = pd.X + (size_t)sk * pd.D;
-        for (int d = g; d < md.D; d += G) Xk[d] = sl.theta[md.free_idx[d]];
-    }
+    const PTView pv = view_of(pd);
+    tab_load(md, s_tile);      // the task's first barrier publishes it
+    pt_half_task<PREC, WPT>(md, pv, *pd.sweep, half, step, b, s_slot[slot], s_tile, s_scan, s_lp, s_acc, slot, wsub,
+                            false, (unsigned long long*)(pd.sweep + 1));
 }
 
 // ---------------------------------------------------------------------------
 // Adjacent-temperature swaps (hottest pair first), swap statistics, ladder
-// adaptation and the sweep counter.  One workgroup.  The map and the
-// per-position likelihoods are staged in LDS when they fit (USE_LDS), else
-// worked on in global memory.
+// adaptation and the sweep counter.  One workgroup.  With USE_LDS the map,
+// the per-position likelihoods and the pre-drawn pairings/uniforms are
+// staged in LDS; otherwise the round works on the map in global memory and
+// draws its randoms inline.
 // ---------------------------------------------------------------------------
 constexpr int SWAP_BLOCK = 1024;
-constexpr int SWAP_WAVES = SWAP_BLOCK / 64;
 
-__device__ __forceinline__ uint32_t swap_partner(const PTDev& pd, long long sweep, int h, int i) {
-    return feistel_perm((uint32_t)i, (uint32_t)pd.W, pd.key0 ^ (uint32_t)sweep ^ 0x5bd1e995u,
-                        pd.key1 ^ ((uint32_t)h * 0x27d4eb2fu) ^ (uint32_t)((unsigned long long)sweep >> 32));
-}
-
-__device__ __forceinline__ double swap_logu(const PTDev& pd, long long sweep, int h, int i) {
-    U4 cu{(uint32_t)i, (uint32_t)sweep, (uint32_t)((unsigned long long)sweep >> 32), 0x40000000u | (uint32_t)h};
-    U4 r = philox4x32_10(cu, pd.key0, pd.key1);
-    return log(u01(r.x, r.y));
-}
-
-// USE_LDS: the map, the per-position likelihoods and the pairing/uniforms of
-// every pair (computed in parallel up front; they do not depend on the state)
-// live in LDS, so the sequential hottest-first loop is compare-and-exchange.
 template <bool USE_LDS>
 __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     extern __shared__ double s_dyn[];
     __shared__ double s_red[64][2];
     __shared__ double s_rate[64];
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x;
     const long long sweep = *pd.sweep;
+    const PTView pv = view_of(pd);
     const int T = pd.T, W = pd.W;
     const int n = T * W, np = (T - 1) * W;
     double* L = USE_LDS ? s_dyn : pd.swap_l;
     double* LU = s_dyn + n;                          // [T-1][W] log uniforms (LDS path)
     int* Mp = USE_LDS ? (int*)(s_dyn + n + np) : pd.map;
     int* JP = (int*)(s_dyn + n + np) + n;            // [T-1][W] partners (LDS path)
-    for (int e = tid; e < 128; e += SWAP_BLOCK) (&s_red[0][0])[e] = 0.0;
-    for (int e = tid; e < n; e += SWAP_BLOCK) {
-        const int m = pd.map[e];
-        if (USE_LDS) Mp[e] = m;
-        L[e] = pd.lnl[m];
-    }
     if (USE_LDS) {
-        for (int e = tid; e < np; e += SWAP_BLOCK) {
-            const int h = e / W + 1, i = e - (h - 1) * W;
-            JP[e] = (int)swap_partner(pd, sweep, h, i);
-            LU[e] = swap_logu(pd, sweep, h, i);
-        }
+        pt_swap_stage(pv, sweep, L, Mp, LU, JP, SWAP_BLOCK);
+    } else {
+        for (int e = tid; e < n; e += SWAP_BLOCK) L[e] = pd.lnl[pd.map[e]];
+        __threadfence();
     }
     __syncthreads();
-    for (int h = T - 1; h >= 1; --h) {
-        const int c = h - 1;
-        const double dbeta = pd.betas[c] - pd.betas[h];
-        double nacc = 0.0, dist = 0.0;
-        for (int i = tid; i < W; i += SWAP_BLOCK) {
-            const int e = (h - 1) * W + i;
-            const int jp = USE_LDS ? JP[e] : (int)swap_partner(pd, sweep, h, i);
-            const double lu = USE_LDS ? LU[e] : swap_logu(pd, sweep, h, i);
-            const int ih = h * W + i, jc = c * W + jp;
-            const double lh = L[ih], lc = L[jc];
-            double lr = dbeta * (lh - lc);
-            if (!(lr == lr)) lr = -__builtin_inf();
-            const bool acc = lu < lr;
-            if (acc) {
-                L[ih] = lc;
-                L[jc] = lh;
-                const int mh = Mp[ih];
-                Mp[ih] = Mp[jc];
-                Mp[jc] = mh;
-            }
-            nacc += acc ? 1.0 : 0.0;
-            if (lr > -__builtin_inf() && lr < __builtin_inf()) dist += fabs(lr);
-        }
-        if (tid - lane < W) {          // whole waves only; waves without walkers add nothing
-            nacc = wave_sum(nacc);
-            dist = wave_sum(dist);
-            if (lane == 0) { atomicAdd(&s_red[c][0], nacc); atomicAdd(&s_red[c][1], dist); }
-        }
-        if (!USE_LDS) __threadfence();
-        __syncthreads();
-    }
+    pt_swap_round(pv, sweep, L, Mp, LU, JP, USE_LDS, s_red, SWAP_BLOCK);
     if (tid < T - 1) {
         const double na = s_red[tid][0], di = s_red[tid][1];
         s_rate[tid] = na / W;
@@ -235,21 +129,7 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
         for (int e = tid; e < n; e += SWAP_BLOCK) pd.map[e] = Mp[e];
     __syncthreads();
     if (tid == 0) {
-        // Vousden, Farr & Mandel (2016): S_i = ln(T_i - T_{i-1}) moves by
-        // kappa (A_{i-1} - A_i), kappa = tau / (n + tau) / nu; the coldest and
-        // hottest temperatures are fixed.
-        if (pd.adapt_mode == 0 && T >= 3) {
-            const double kappa = pd.adapt_tau / ((double)sweep + pd.adapt_tau) / pd.adapt_nu;
-            double Tprev_old = 1.0 / pd.betas[0];
-            double acc_T = Tprev_old;
-            for (int i = 1; i < T - 1; ++i) {
-                const double Ti_old = 1.0 / pd.betas[i];
-                const double dS = kappa * (s_rate[i - 1] - s_rate[i]);
-                acc_T += (Ti_old - Tprev_old) * exp(dS);
-                Tprev_old = Ti_old;
-                pd.betas[i] = 1.0 / acc_T;
-            }
-        }
+        pt_ladder_update(pv, sweep, s_rate);
         if (sweep < pd.max_sweeps)
             for (int i = 0; i < T; ++i) pd.h_betas[(size_t)sweep * T + i] = pd.betas[i];
         *pd.sweep = sweep + 1;
@@ -414,6 +294,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     EMP_TRY(hipMemcpy(v.betas, betas, (size_t)v.T * sizeof(double), hipMemcpyHostToDevice));
     EMP_TRY(hipMemset(v.accepted, 0, TW * sizeof(long long)));
     EMP_TRY(hipMemset(v.sweep, 0, 2 * sizeof(long long)));
+    EMP_TRY(hipDeviceSynchronize());     // the engine stream does not wait for the null stream
 #undef EMP_TRY
     p->wpt = emp_pick_wpt(m, (int64_t)v.T * v.W2, d.precision);
     rc = eval_state(p);

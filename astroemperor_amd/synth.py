@@ -17,6 +17,12 @@ setup ``(ntemps, nwalkers, nsweeps, nsteps)``.
 * ``c4``: 5 Keplerians, N = 2000, 2 instruments, parameterisation 1.
 * ``c5``: 7 Keplerians, N = 10000 over 5 instruments with MA(1);
   16 x 1024 x 5000.
+* ``c3``: batched uneven systems: S = 4096 independent systems, N_s =
+  clip(round(LogNormal(ln 200, 0.6)), 50, 2000) epochs over 3000 d, k_s ~
+  U{1..4} Keplerians (ln P ~ U(ln 2, ln 1000), K ~ U(1, 50), e ~ Beta(0.867,
+  3.03)), I_s ~ U{1..3} instruments sharing the epochs by a Dirichlet(1)
+  draw (at least 5 each); parameterisation 1 with the uniform priors of
+  ``c1``; each system 8 x 64 walkers.
 """
 from __future__ import annotations
 
@@ -158,6 +164,61 @@ def c5():
     return _multi(5, 7, [1000, 1500, 2000, 2500, 3000], 6000.0, 6, (16, 1024, 5000, 1))
 
 
+def _system(rng, span=3000.0):
+    N = int(np.clip(np.round(rng.lognormal(np.log(200.0), 0.6)), 50, 2000))
+    k = int(rng.integers(1, 5))
+    I = int(rng.integers(1, 4))
+    share = rng.dirichlet(np.ones(I))
+    counts = 5 + rng.multinomial(N - 5 * I, share)
+    bjd, sig = _instruments(rng, list(counts), span)
+    t0 = min(b.min() for b in bjd)
+    planets = []
+    for _ in range(k):
+        planets.append((float(np.exp(rng.uniform(np.log(2), np.log(1000)))), float(rng.uniform(1, 50)),
+                        float(rng.beta(0.867, 3.03)), float(rng.uniform(0, 2 * np.pi)),
+                        float(rng.uniform(0, 2 * np.pi))))
+    gam = rng.uniform(-10, 10, I)
+    jit = rng.uniform(0, 5, I)
+    rvs = []
+    for i in range(I):
+        mu = sum(kepler_rv(bjd[i], *pl, t0) for pl in planets) + gam[i]
+        rvs.append(mu + rng.normal(0, np.sqrt(sig[i] ** 2 + jit[i] ** 2)))
+    data = RVData.from_arrays(bjd, rvs, sig, labels=[f"ins{i + 1}" for i in range(I)])
+    blocks = [("kep", {"param": 1, "n": n + 1}) for n in range(k)] + [("off", None), ("jit", None)]
+    conds = [(f"Jitter {i + 1}", "prior", "Uniform") for i in range(I)]
+    model = _build(blocks, data, conds, "reference")
+    truth = []
+    for (P, K, e, w, M0) in planets:
+        truth += [P, K, M0, np.sqrt(e) * np.sin(w), np.sqrt(e) * np.cos(w)]
+    truth += list(gam - np.array([np.mean(r) for r in rvs])) + list(jit)
+    return model, np.array(truth)
+
+
+def c3(n_systems=4096):
+    rng = np.random.default_rng(BASE_SEED + 3)
+    models, truths = [], []
+    for _ in range(n_systems):
+        m, t = _system(rng)
+        models.append(m)
+        truths.append(t)
+    return dict(models=models, truths=truths, setup=(8, 64, 1000, 1))
+
+
+def initial_positions(model, ntemps, nwalkers, rng):
+    """[T, W, D] uniform in the prior box, with every (S, C) pair of a
+    parameterisation 1/2/4/7 Keplerian inside the unit disk (e < 1)."""
+    lim = model.limits_free()
+    D = len(lim)
+    x = lim[:, 0] + (lim[:, 1] - lim[:, 0]) * rng.uniform(size=(ntemps, nwalkers, D))
+    names = model.names_free()
+    for j, nm in enumerate(names):
+        if nm.startswith("Ecc_sin") and j + 1 < D and names[j + 1].startswith("Ecc_cos"):
+            r = np.sqrt(rng.uniform(size=(ntemps, nwalkers)))
+            th = rng.uniform(0, 2 * np.pi, size=(ntemps, nwalkers))
+            x[..., j], x[..., j + 1] = 0.99 * r * np.sin(th), 0.99 * r * np.cos(th)
+    return x
+
+
 def prior_draws(model, n, rng):
     """Draws inside the prior box of every free parameter (uniform over the
     limits), re-drawing those with a -inf prior (S^2 + C^2 > 1 etc.)."""
@@ -166,4 +227,4 @@ def prior_draws(model, n, rng):
     return out
 
 
-CONFIGS = {"c1": c1, "c2": c2, "c4": c4, "c5": c5}
+CONFIGS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}

@@ -96,6 +96,77 @@ def dlnl_report(work, samplers):
                 mismatched_finiteness=int((np.isfinite(a) != np.isfinite(b)).sum()))
 
 
+def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), trial_sweeps=2):
+    """Batched uneven systems (synth.c3): every system's own 8 x 64 PT
+    ensemble advanced by one persistent kernel launch per sweep, replayed
+    from a hipGraph, launch shape from the autotuner."""
+    from astroemperor_amd import SystemsPT, synth
+    t0 = time.perf_counter()
+    work = synth.c3(n_systems)
+    models = work["models"]
+    T, W, _, nsteps = work["setup"]
+    rng = np.random.default_rng(seed)
+    p0 = [synth.initial_positions(m, T, W, rng) for m in models]
+    build_s = time.perf_counter() - t0
+    N = np.array([m.data.ndat for m in models])
+    K = np.array([m.kplan__ for m in models])
+    flops_sys = np.array([flops_per_theta(m, "mixed") for m in models])
+    out = {"workload": f"{len(models)} independent systems, epochs lognormal(median 200) clipped to "
+                       "[50, 2000], 1-4 Keplerians, 1-3 instruments, each 8 temps x 64 walkers",
+           "n_systems": len(models), "epochs_total": int(N.sum()), "epochs_max": int(N.max()),
+           "planet_epochs_total": int((N * K).sum()), "ntemps": T, "nwalkers": W, "host_build_s": round(build_s, 1)}
+    states = {}
+    for prec in precisions:
+        t1 = time.perf_counter()
+        s = SystemsPT(models, T, W, p0, seed=seed, precision=prec, adapt_tau=1000, adapt_nu=1)
+        create_s = time.perf_counter() - t1
+        s.run(1)
+        tune = s.autotune(trial_sweeps=trial_sweeps)
+        s.run(max(warmup, 1))
+        n0 = s.n_lnl_evaluated
+        ta = time.perf_counter()
+        s.run(steps, use_graph=True)
+        tb = time.perf_counter()
+        n1 = s.n_lnl_evaluated
+        wall = tb - ta
+        ev_ms = s.last_ms_per_sweep()
+        props = steps * T * W * nsteps * len(models)
+        frac_eval = (n1 - n0) / props
+        # algorithmic FP64-equivalent flops of one sweep: every evaluated lnL
+        # at its system's cost (evaluation fraction taken as uniform)
+        flops = float(flops_sys.sum()) * T * W * nsteps * frac_eval
+        achieved = flops / (ev_ms * 1e-3) / 1e12
+        out[prec] = {"sweeps_per_s": round(steps / wall, 3), "ms_per_sweep": round(1e3 * wall / steps, 4),
+                     "ms_per_sweep_events": round(ev_ms, 4),
+                     "lnl_evals_per_s": round(props / wall, 1), "lnl_evaluated_per_s": round((n1 - n0) / wall, 1),
+                     "tuning": tune, "create_s": round(create_s, 2),
+                     "roofline": {"bound": "valu", "kernel": f"k_sys_sweep<{prec}>", "achieved": round(achieved, 3),
+                                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": round(achieved / PEAK_FP64_TFLOPS, 4)}}
+        print(f"[bench] systems {prec}: {steps / wall:.2f} sweeps/s ({props / wall / 1e6:.1f} M lnL/s), "
+              f"tuning {tune}", file=sys.stderr, flush=True)
+        states[prec] = s
+    if "mixed" in states and "fp64" in states:
+        out["mixed_over_fp64"] = round(out["mixed"]["sweeps_per_s"] / out["fp64"]["sweeps_per_s"], 3)
+        # |dlnL| on the states the mixed run visited, a spread of systems
+        s = states["mixed"]
+        dmax, n = 0.0, 0
+        for i in np.linspace(0, len(models) - 1, 24).astype(int):
+            x, _, _ = s.state(int(i))
+            th = x.reshape(-1, x.shape[-1])
+            a = models[i].lnl_batch(th, "fp64")
+            b = models[i].lnl_batch(th, "mixed")
+            fin = np.isfinite(a) & np.isfinite(b)
+            if fin.any():
+                dmax = max(dmax, float(np.abs(a[fin] - b[fin]).max()))
+                n += int(fin.sum())
+            models[i]._device = None
+        out["dlnl_mixed_vs_fp64"] = {"n": n, "max": dmax}
+    for s in states.values():
+        s.close()
+    return out
+
+
 def run_cpu(work, sweeps, cores, seed):
     from oracle import cpu_reference
     from astroemperor_amd.simulation import test_init
@@ -120,6 +191,9 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--half-launches", type=int, default=40)
+    ap.add_argument("--systems", type=int, default=4096, help="batched-systems leg (0 = skip)")
+    ap.add_argument("--systems-steps", type=int, default=30)
+    ap.add_argument("--systems-warmup", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -137,6 +211,9 @@ def main():
         print(f"[bench] {prec}: {res[prec]['sweeps_per_s']:.1f} sweeps/s, "
               f"half-step kernel {res[prec]['ms_halfstep']:.4f} ms", file=sys.stderr, flush=True)
     dl = dlnl_report(work, [res["fp64"]["sampler"], res["mixed"]["sampler"]])
+    for r in res.values():
+        r["sampler"].close()
+    systems = run_systems(args.systems, args.systems_steps, args.systems_warmup, args.seed) if args.systems else None
 
     mixed, fp64 = res["mixed"], res["fp64"]
     flops = flops_per_theta(m, "mixed") * mixed["evals_per_halfstep"]
@@ -181,6 +258,7 @@ def main():
         "dlnl_mixed_vs_fp64": dl,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched_systems": systems,
         "device": info,
     }
     if cpu:

@@ -212,6 +212,73 @@ int emp_pt_last_timing(EmpPT* pt, double* ms_per_sweep, double* ms_halfstep_kern
 int emp_pt_time_halfstep(EmpPT* pt, int32_t n, double* ms_avg);
 
 /* ---------------------------------------------------------------------------
+ * Batched independent systems: S separate RV models, each with its own PT
+ * ensemble of T x W walkers, advanced together one sweep per launch.
+ * Replaces S separate PTSampler(...).run_mcmc(...) runs (emp/emp.py:576-593,
+ * support/endit_reddemcee.scr:3), one per star.  The ensembles share T, W,
+ * nsteps, a and the precision; epochs, instruments, planets and ndim differ.
+ * ------------------------------------------------------------------------ */
+typedef struct EmpSystems EmpSystems;
+
+typedef struct EmpSystemsDesc {
+    int32_t n_systems;          /* S */
+    int32_t ntemps;             /* T per system */
+    int32_t nwalkers;           /* W per system (even) */
+    int32_t nsteps;             /* stretch steps per sweep */
+    int32_t precision;          /* EMP_FP64 / EMP_MIXED */
+    int32_t adapt_mode;         /* -1 frozen, 0 Vousden et al. 2016 */
+    double a;                   /* stretch scale */
+    uint64_t seed;              /* Philox key; system s draws with s mixed into the counters */
+    double adapt_tau, adapt_nu;
+    int32_t record_every;       /* record the coldest temperatures every k sweeps (0: never) */
+    int32_t record_temps;       /* how many of the coldest temperatures are recorded (1..T) */
+    int64_t max_records;        /* record capacity */
+} EmpSystemsDesc;
+
+/* Launch configuration of the batched sweep kernel, chosen by the autotuner
+ * or set by the caller. */
+typedef struct EmpSystemsTuning {
+    int32_t waves_per_wg;       /* 8 or 16 */
+    int32_t waves_per_theta;    /* 1 or 2 */
+    int32_t wgs_per_cu;         /* persistent workgroups per compute unit, 1..4 */
+    int32_t order;              /* 0: longest system first (LPT), 1: input order */
+    double ms_per_sweep;        /* measured by the autotuner (output) */
+} EmpSystemsTuning;
+
+/* models: [S] descriptors (the same rules as emp_model_create).
+ * betas: [S, T] host.  p0: host, system after system, each [T, W, ndim_s].
+ * Evaluates the initial lnL / lnp of every walker. */
+int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDesc* desc, const double* betas,
+                       const double* p0, EmpSystems** out);
+void emp_systems_destroy(EmpSystems* sys);
+
+/* nsweeps sweeps of every system; one kernel launch per sweep (captured once
+ * into a hipGraph when use_graph != 0).  Returns after completion. */
+int emp_systems_run(EmpSystems* sys, int64_t nsweeps, int32_t use_graph);
+
+/* Times every launch configuration that fits for trial_sweeps sweeps each
+ * (restoring the state after each trial, so the chains are unchanged), keeps
+ * the fastest and reports it in *best (may be NULL). */
+int emp_systems_autotune(EmpSystems* sys, int32_t trial_sweeps, EmpSystemsTuning* best);
+int emp_systems_set_tuning(EmpSystems* sys, const EmpSystemsTuning* tuning);
+int emp_systems_get_tuning(EmpSystems* sys, EmpSystemsTuning* tuning);
+
+/* Getters.  system = s selects one system; system = -1 returns all of them
+ * (system after system for the ragged EMP_PT_STATE_X and EMP_PT_CHAIN).
+ *   EMP_PT_STATE_X      [T, W, D_s]
+ *   EMP_PT_STATE_LNL/LNP, EMP_PT_ACCEPTED (int64)   [S?, T, W]
+ *   EMP_PT_STATE_BETAS  [S?, T]
+ *   EMP_PT_TSW, EMP_PT_SMD  [S?, T-1] means over the sweeps run so far
+ *   EMP_PT_CHAIN        [R, record_temps, W, D_s] (system >= 0 only)
+ *   EMP_PT_LNL/LNP      [R, S?, record_temps, W]
+ *   EMP_PT_NSWEEPS      [2] sweeps done, likelihoods evaluated (int64) */
+int emp_systems_get(EmpSystems* sys, int32_t what, int32_t system, void* host_out, int64_t capacity_bytes,
+                    int64_t* shape_out);
+
+/* Average device time per sweep of the last emp_systems_run (ms). */
+int emp_systems_last_timing(EmpSystems* sys, double* ms_per_sweep);
+
+/* ---------------------------------------------------------------------------
  * Misc.
  * ------------------------------------------------------------------------ */
 /* No reference counterpart (test hook, needs no device): the engine's
