@@ -88,6 +88,7 @@ _SIGS = {
     "emp_pt_restore": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "emp_pt_time_halfstep": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "emp_pt_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "emp_systems_create": (C.c_int, [_P, C.POINTER(EmpSystemsDesc), _P, _P, C.POINTER(_P)]),
     "emp_systems_destroy": (None, [_P]),
     "emp_systems_run": (C.c_int, [_P, C.c_int64, C.c_int32]),

@@ -12,12 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemp_mi355x.so")
 SOURCES = ["emp_model.hip", "emp_pt.hip", "emp_grad.hip", "emp_systems.hip"]
-# Per-source extra flags.  The batched-systems sweep runs the likelihood
-# inside loops over systems and walkers; machine LICM would hoist the 64-bit
-# constants of every inlined log/exp out of those loops and keep them live in
-# ~60 VGPRs through the hot planet loop (240+ VGPRs, or scratch spills at
-# 1024 threads).  Left in place they cost a scalar move where they are used.
-EXTRA = {"emp_systems.hip": ["-mllvm", "-disable-machine-licm"]}
+# Per-source extra flags (none at present).
+EXTRA: dict = {}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          # HIP's default (fast) ignores `#pragma clang fp contract(off)`, which
@@ -25,6 +21,14 @@ FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-ffp-contract=fast-honor-pragmas",
          # no packed-FP32 instructions in device code
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # The likelihood runs inside loops over tiles, walkers and systems;
+         # machine LICM hoists the 64-bit constants of every inlined log/exp
+         # out of them and keeps them live in ~30-60 VGPRs through the hot
+         # planet loop (the batched-systems sweep reached 240+ VGPRs, or
+         # scratch spills at 1024 threads).  Left in place they cost a move
+         # where they are used; measured on the half-step kernel: -17% time
+         # (mixed) and -10% (FP64) at 3 planets x 1530 epochs.
+         "-mllvm", "-disable-machine-licm",
          "-Wno-unused-command-line-argument"]
 
 

@@ -56,11 +56,24 @@ __device__ __forceinline__ double rcp64(double x) {
     return __builtin_fma(r, e, r);
 }
 
+// One refinement only (mixed path): the hardware estimate's error squared,
+// below 1e-13 relative.
+__device__ __forceinline__ double rcp64_1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
 // Reduce an FP64 angle to [-pi, pi] with a two-term (Cody-Waite) 2 pi.
 __device__ __forceinline__ double reduce_2pi(double M) {
     double n = __builtin_rint(M * INV_TWO_PI);
     double r = __builtin_fma(-n, TWO_PI_HI, M);
     return __builtin_fma(-n, TWO_PI_LO, r);
+}
+
+// One-term reduction (mixed path): the error n |2 pi - TWO_PI_HI| stays below
+// 1e-12 rad for |M| < 2.5e4 rad (6000 d of a 1.5 d period).
+__device__ __forceinline__ double reduce_2pi_1(double M) {
+    return __builtin_fma(-__builtin_rint(M * INV_TWO_PI), TWO_PI_HI, M);
 }
 
 // ----------------------------------------------------------------------------
@@ -79,6 +92,7 @@ struct PlanetC {
     double cw, sw;   // cos w, sin w (for gradients)
     int tp_mode;
     int bad;         // NaN-producing parameter set (reference would give NaN)
+    int ktab;        // mixed path: this planet's per-vector Kepler table is built and used
 };
 
 // Builds PlanetC from the 5 block parameters, following the reference's
@@ -94,6 +108,7 @@ __device__ inline PlanetC planet_setup(const double* th, int param, int use_c) {
     pc.phase = 0.0;
     pc.tp = 0.0;
     pc.bad = 0;
+    pc.ktab = 0;
     bool sc = (param == 1 || param == 2 || param == 4 || param == 7);
     if (param == 2 || param == 6 || param == 7) per = exp(th[0]); else per = th[0];
     if (param == 2) {
@@ -170,6 +185,21 @@ __device__ __forceinline__ void sincos_tab(const SinCosTab& tb, double x, double
     c = __builtin_fma(sc.y, cd, -sc.x * sd);
 }
 
+// Mixed path: sin/cos of E = (double)Ef for an FP32 estimate Ef in [0, pi].
+// The table index comes from the FP32 value; d = E - k h in one FMA (the
+// double h = pi / SC_N is exact to 1e-19, k <= SC_N); sin d to d^3
+// (d^5 / 120 < 7e-14 dropped).
+__device__ __forceinline__ void sincos_tab_f(const SinCosTab& tb, float Ef, double E, double& s, double& c) {
+    const int k = (int)__builtin_rintf(Ef * (float)SC_INV_H);
+    const double d = __builtin_fma(-(double)k, SC_H, E);
+    const double2 sc = tb.v[k];
+    const double d2 = d * d;
+    const double sd = __builtin_fma(d * d2, -1.0 / 6.0, d);
+    const double cd = __builtin_fma(d2, __builtin_fma(d2, 1.0 / 24.0, -0.5), 1.0);
+    s = __builtin_fma(sc.x, cd, sc.y * sd);
+    c = __builtin_fma(sc.y, cd, -sc.x * sd);
+}
+
 // x^(2/3) for x >= 0 in FP64: hardware FP32 log2/exp2 seed (~1e-7) and one
 // Newton step on w^3 = x^2, w <- (2 w + x^2 / w^2) / 3 (quadratic: ~1e-14).
 __device__ __forceinline__ double pow23_fp64(double x) {
@@ -205,10 +235,12 @@ __device__ __forceinline__ void kepler_fp64_E(double M, const PlanetC& pc, const
     const double f1 = __builtin_fma(-e, c, 1.0);
     const double f2 = e * s;
     const double f3 = 1.0 - f1;
-    const double dn = -f0 * rcp64(f1);
-    const double d3 = -f0 * rcp64(__builtin_fma(0.5 * dn, f2, f1));
-    const double d4 = -f0 * rcp64(f1 + 0.5 * d3 * f2 + d3 * d3 * f3 * (1.0 / 6.0));
-    const double d5 = -f0 * rcp64(f1 + 0.5 * d4 * f2 + d4 * d4 * f3 * (1.0 / 6.0)
+    // The corrections are below 4e-4 rad: reciprocals to 2e-15 (one
+    // refinement) leave them exact to 1e-18.
+    const double dn = -f0 * rcp64_1(f1);
+    const double d3 = -f0 * rcp64_1(__builtin_fma(0.5 * dn, f2, f1));
+    const double d4 = -f0 * rcp64_1(f1 + 0.5 * d3 * f2 + d3 * d3 * f3 * (1.0 / 6.0));
+    const double d5 = -f0 * rcp64_1(f1 + 0.5 * d4 * f2 + d4 * d4 * f3 * (1.0 / 6.0)
                                   - d4 * d4 * d4 * f2 * (1.0 / 24.0));
     // Rotate (s, c) by d5; |d5| < 1e-3 so three Taylor terms are exact to 1e-19.
     const double dd = d5 * d5;
@@ -230,13 +262,18 @@ __device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, const S
 // Per-vector Kepler tables (mixed path).  For one parameter vector the
 // eccentricity of a planet is fixed, so E(M) on [0, pi] is tabulated once at
 // KT_N nodes M_j = j h (h = pi / (KT_N - 1)) as (E_j, h dE/dM_j) in FP32;
-// each epoch then takes a cubic Hermite interpolant (error < 2e-5 rad for
-// e <= KT_EMAX) and one FP64 Halley step.  Above KT_EMAX the FP32 Markley
-// starter is used instead.
+// each epoch then takes a cubic Hermite interpolant (error <= 6e-7 rad for
+// e <= KT_EMAX, 1.4e-5 at e = 0.7) and one FP64 Newton step.  Above KT_EMAX
+// the FP32 Markley starter (error <= 5e-7 rad to e = 0.99) and one FP64
+// Halley step are used instead.
 // ----------------------------------------------------------------------------
 constexpr int KT_N = 64;
 constexpr double KT_H = PI_D / (KT_N - 1);
-constexpr double KT_EMAX = 0.7;
+constexpr double KT_EMAX = 0.5;
+// Building a table costs about as much as 64 FP64 planet-epochs, and saves
+// the FP32 Markley starter's ~40 extra operations per epoch: it pays from
+// about 256 epochs on.
+constexpr int KT_MIN_EPOCHS = 256;
 
 // Node `lane` of planet pc's table (called by one wave: 64 lanes, 64 nodes).
 __device__ __forceinline__ float2 ktab_node(const PlanetC& pc, const SinCosTab& tb, int lane) {
@@ -261,9 +298,8 @@ __device__ __forceinline__ float ktab_start(const float2* kt, float Mf) {
 // Mixed-precision Kepler solve.  FP32 buys the iteration count: the starter
 // comes from the per-vector FP32 table (e <= KT_EMAX) or, above it, from
 // Markley's starter in FP32 (hardware log2/exp2/rcp/sqrt) plus one FP32
-// Halley step with the hardware sin/cos.  One FP64 Halley step from that
-// root (cubic convergence from <= 2e-5) and the FP64 projection restore FP64
-// accuracy, so the likelihood stays within 1e-6 of the FP64 path.
+// Halley step with the hardware sin/cos.  One FP64 step from that root and
+// the FP64 projection restore the accuracy the likelihood needs.
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) {
     const float ef = (float)pc.e;
@@ -291,43 +327,46 @@ __device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) 
     return E;
 }
 
-// One FP64 Halley step from E0 (Ma = |M| in [0, pi]); returns sin E, cos E.
-__device__ __forceinline__ void kepler_fp64_halley(double Ma, float Ef, const PlanetC& pc, const SinCosTab& tb,
-                                                   double& sE, double& cE) {
-    const double E0 = __builtin_fmin(__builtin_fmax((double)Ef, 0.0), PI_D);
+// One FP64 step from the FP32 root Ef (Ma = |M| in [0, pi]): Newton from the
+// table start (<= 6e-7 rad), Halley from the FP32 Markley start.  The step is
+// below 1e-6 rad, so sin E, cos E follow from a first-order rotation
+// (neglected d^2 / 2 < 1e-12).  Planet term error <= 1.5e-12 K for e < 0.99.
+template <bool HALLEY>
+__device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, const SinCosTab& tb, double& sE,
+                                                 double& cE) {
+    Ef = __builtin_amdgcn_fmed3f(Ef, 0.0f, (float)PI_D);
+    const double E0 = (double)Ef;
     double s, c;
-    sincos_tab(tb, E0, s, c);
-    const double e = pc.e;
+    sincos_tab_f(tb, Ef, E0, s, c);
     const double f0 = E0 - __builtin_fma(e, s, Ma);
     const double f1 = __builtin_fma(-e, c, 1.0);
-    const double r1 = rcp64(f1);
-    const double dn = -f0 * r1;
-    const double dl = dn * __builtin_fma(-0.5 * dn * e * s, r1, 1.0);
-    const double dd = dl * dl;
-    const double sd = dl * __builtin_fma(-dd, 1.0 / 6.0, 1.0);
-    const double cd = __builtin_fma(dd, __builtin_fma(dd, 1.0 / 24.0, -0.5), 1.0);
-    sE = __builtin_fma(s, cd, c * sd);
-    cE = __builtin_fma(c, cd, -s * sd);
+    const double r1 = rcp64_1(f1);
+    double dl = -f0 * r1;
+    if (HALLEY) dl *= __builtin_fma(-0.5 * dl * e * s, r1, 1.0);
+    sE = __builtin_fma(c, dl, s);
+    cE = __builtin_fma(-s, dl, c);
 }
 
 // Planet term at time t.  PREC 0: FP64 Markley; PREC 1: mixed, with `kt` the
-// planet's per-vector table (read only when pc.e <= KT_EMAX, a per-vector,
+// planet's per-vector table (read only when pc.ktab is set, a per-vector,
 // hence wave-uniform, branch).
 template <int PREC>
 __device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t) {
-    const double M = reduce_2pi(planet_M(pc, t));
     double sE, cE;
     if (PREC == 0) {
+        const double M = reduce_2pi(planet_M(pc, t));
         kepler_fp64(M, pc, tb, sE, cE);
-    } else {
-        const double Ma = __builtin_fabs(M);
-        const float Mf = (float)Ma;
-        const float Ef = pc.e <= KT_EMAX ? ktab_start(kt, Mf) : kepler_fp32_start(Mf, pc);
-        kepler_fp64_halley(Ma, Ef, pc, tb, sE, cE);
-        if (M < 0.0) sE = -sE;
+        const double den = __builtin_fma(-pc.e, cE, 1.0);
+        return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64(den);
     }
+    const double M = reduce_2pi_1(planet_M(pc, t));
+    const double Ma = __builtin_fabs(M);
+    const float Mf = (float)Ma;
+    if (pc.ktab) kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
+    else kepler_fp64_step<true>(Ma, kepler_fp32_start(Mf, pc), pc.e, tb, sE, cE);
+    if (M < 0.0) sE = -sE;
     const double den = __builtin_fma(-pc.e, cE, 1.0);
-    return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64(den);
+    return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64_1(den);
 }
 
 // ----------------------------------------------------------------------------

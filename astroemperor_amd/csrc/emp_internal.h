@@ -22,12 +22,14 @@ constexpr int TILE = 512;        // epochs staged in LDS per tile
 struct ModelDev {
     int N, I, K, param, use_c, acc, has_jit;
     int ma_q, ma_global, ma_active;
+    int ma_wfixed;        // every MA timescale is fixed: dt holds the weights exp(-dt / tau)
     int Dfull, D, nblocks;
     int off_kep, off_acc, off_off, off_jit, off_ma;
     const double* t;      // [N] device order
     const double* y;      // [N]
     const double* s2;     // [N] yerr^2
-    const double* dt;     // [N] t_i - t_{i-1} within an MA segment, -1 at a segment start
+    const double* dt;     // [N] t_i - t_{i-1} within an MA segment, -1 at a segment start;
+                          //     with ma_wfixed exp(-dt / tau), 0 at a segment start
     const int* ins;       // [N]
     const int* inv_free;  // [Dfull] free index of each full parameter, -1 if fixed
     const int* free_idx;  // [D]

@@ -61,7 +61,11 @@ __device__ inline double prior_seq(const ModelDev& md, const double* th) {
 // Per-slot constants (planets, instruments, acceleration) into LDS.
 // Called by the slot's first wave; the caller synchronises afterwards.
 __device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
-    if (lane < md.K) sl.planet[lane] = planet_setup(sl.theta + md.off_kep + 5 * lane, md.param, md.use_c);
+    if (lane < md.K) {
+        PlanetC pc = planet_setup(sl.theta + md.off_kep + 5 * lane, md.param, md.use_c);
+        pc.ktab = md.N >= KT_MIN_EPOCHS && pc.e <= KT_EMAX;
+        sl.planet[lane] = pc;
+    }
     if (lane < md.I) {
         sl.gam[lane] = sl.theta[md.off_off + lane];
         double j = md.has_jit ? sl.theta[md.off_jit + lane] : 0.0;
@@ -107,7 +111,7 @@ __device__ __forceinline__ void slot_prepare(const ModelDev& md, SlotLDS& sl, co
     __syncthreads();
     if (PREC == 1 && live) {
         for (int p = wsub; p < md.K; p += WPT)
-            if (sl.planet[p].e <= KT_EMAX) sl.ktab[p][lane] = ktab_node(sl.planet[p], tl.tab, lane);
+            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], tl.tab, lane);
     }
     __syncthreads();
 }
@@ -185,7 +189,8 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
                 double a = 1.0, b = 0.0;
                 if (valid) {
                     const double dtj = tl.dt[j];
-                    a = dtj < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-dtj / sl.ma_tau[k]));
+                    if (md.ma_wfixed) a = -(sl.ma_c[k] * dtj);
+                    else a = dtj < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-dtj / sl.ma_tau[k]));
                     b = r;
                 }
                 wave_affine_scan(a, b, lane);
@@ -210,7 +215,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
                 }
             }
             if (valid) {
-                chi = __builtin_fma(eps * eps, rcp64(err2), chi);
+                chi = __builtin_fma(eps * eps, PREC == 1 ? rcp64_1(err2) : rcp64(err2), chi);
                 prod *= err2;
                 pexp += __builtin_amdgcn_frexp_exp(prod);
                 prod = __builtin_amdgcn_frexp_mant(prod);

@@ -296,6 +296,23 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
     md.off_jit = d->off_jitter; md.off_ma = d->off_ma;
     md.lnl_const = -0.5 * std::log(2.0 * M_PI) * (double)N;
     md.t0 = d->t[0];
+    // Fixed MA timescales (the usual setup): the weight exp(-dt / tau) of
+    // every epoch is the same for all parameter vectors; compute it once.
+    md.ma_wfixed = 0;
+    if (ma_active) {
+        bool fixed = true;
+        for (int k = 0; k < d->n_ins && fixed; ++k) {
+            const int j = d->off_ma + 2 * d->ma_order * (d->ma_global ? 0 : k) + 1;
+            fixed = h->inv[j] < 0;
+        }
+        if (fixed) {
+            md.ma_wfixed = 1;
+            for (int i = 0; i < N; ++i) {
+                const int j = d->off_ma + 2 * d->ma_order * (d->ma_global ? 0 : h->ins[i]) + 1;
+                h->dt[i] = h->dt[i] < 0.0 ? 0.0 : std::exp(-h->dt[i] / d->fixed_value[j]);
+            }
+        }
+    }
     return EMP_OK;
 }
 

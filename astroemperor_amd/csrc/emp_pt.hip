@@ -39,6 +39,9 @@ struct PTDev {
     double* betas;      // [T]
     long long* accepted;  // [T][W] (logical positions)
     long long* sweep;     // [2] sweeps completed, likelihoods evaluated
+    double* prop_lp;      // [T W/2] prior of each task's proposal (current half-step)
+    int* list;            // [T W/2] tasks whose proposal has a finite prior
+    int* nlive;           // [2] list lengths of the two halves
     double* h_chain;    // [S][T][W][D]
     double* h_lnl;      // [S][T][W]
     double* h_lnp;      // [S][T][W]
@@ -66,8 +69,49 @@ static __host__ __device__ inline PTView view_of(const PTDev& pd) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused stretch half-step: propose, prior, likelihood, accept; one slot of
-// the workgroup per walker of the moving half.
+// Half-step, part 1: the prior of every proposal (one thread per walker of
+// the moving half) and the list of those inside the prior support.  Each
+// workgroup reserves its part of the list with one atomic (the list order
+// varies from run to run; what each task computes does not).  The other
+// half's counter is cleared here: nothing reads it until that half's own
+// part 1 has run again.
+// ---------------------------------------------------------------------------
+constexpr int PRIOR_BLOCK = 256;
+
+__global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd, int half, int step) {
+    __shared__ int s_wcnt[PRIOR_BLOCK / 64];
+    __shared__ int s_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x * PRIOR_BLOCK + threadIdx.x;
+    const PTView pv = view_of(pd);
+    bool f = false;
+    if (b < pd.T * pd.W2) {
+        const Proposal pr = pt_propose(pv, *pd.sweep, half, step, b);
+        const double lp = prior_proposal(md, pd.X + (size_t)pr.sk * pd.D, pd.X + (size_t)pr.sj * pd.D, pr.z);
+        pd.prop_lp[b] = lp;
+        f = lp > -__builtin_inf();
+    }
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) s_wcnt[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < PRIOR_BLOCK / 64; ++w) tot += s_wcnt[w];
+        s_base = tot ? atomicAdd(&pd.nlive[half], tot) : 0;
+        if (blockIdx.x == 0) pd.nlive[1 - half] = 0;
+    }
+    __syncthreads();
+    if (f) {
+        int off = s_base + __popcll(m & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wave; ++w) off += s_wcnt[w];
+        pd.list[off] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Half-step, part 2: likelihood and acceptance of the listed tasks, one slot
+// of the workgroup per task.  The grid covers every task; workgroups past
+// the list's end leave at once.
 // ---------------------------------------------------------------------------
 template <int PREC, int WPT>
 __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int half, int step) {
@@ -75,15 +119,18 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
     __shared__ SlotLDS s_slot[SLOTS];
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
-    __shared__ double s_lp[SLOTS];
     __shared__ int s_acc[SLOTS];
+    const int n = pd.nlive[half];
+    if ((int)blockIdx.x * SLOTS >= n) return;
     const int wave = threadIdx.x >> 6;
     const int slot = wave / WPT, wsub = wave % WPT;
-    const int64_t b = (int64_t)blockIdx.x * SLOTS + slot;
+    const int q = blockIdx.x * SLOTS + slot;
+    const int b = q < n ? pd.list[q] : -1;
     const PTView pv = view_of(pd);
     tab_load(md, s_tile);      // the task's first barrier publishes it
-    pt_half_task<PREC, WPT>(md, pv, *pd.sweep, half, step, b, s_slot[slot], s_tile, s_scan, s_lp, s_acc, slot, wsub,
-                            false, (unsigned long long*)(pd.sweep + 1));
+    pt_half_task<PREC, WPT>(md, pv, *pd.sweep, half, step, b, b >= 0 ? pd.prop_lp[b] : -__builtin_inf(),
+                            s_slot[slot], s_tile, s_scan, s_acc, slot, wsub, false,
+                            (unsigned long long*)(pd.sweep + 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -175,6 +222,12 @@ static void launch_half_t(const ModelDev& md, const PTDev& pd, int half, int ste
 static int launch_half(EmpPT* p, int half, int step) {
     const ModelDev& md = p->model->dev;
     const int prec = p->desc.precision, w = p->wpt;
+    {
+        const int64_t tasks = (int64_t)p->dev.T * p->dev.W2;
+        hipLaunchKernelGGL(k_pt_prior, dim3((unsigned)((tasks + PRIOR_BLOCK - 1) / PRIOR_BLOCK)), dim3(PRIOR_BLOCK), 0,
+                           p->stream, md, p->dev, half, step);
+        EMP_HIP(hipGetLastError());
+    }
 #define EMP_CASE(P, W) if (prec == P && w == W) { launch_half_t<P, W>(md, p->dev, half, step, p->stream); EMP_HIP(hipGetLastError()); return EMP_OK; }
     EMP_CASE(0, 1) EMP_CASE(0, 2) EMP_CASE(0, 4) EMP_CASE(0, 8)
     EMP_CASE(1, 1) EMP_CASE(1, 2) EMP_CASE(1, 4) EMP_CASE(1, 8)
@@ -225,6 +278,7 @@ static int eval_state(EmpPT* p) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_mask_lnl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, d.lnl, d.lnp, n);
     EMP_HIP(hipGetLastError());
+    EMP_HIP(hipMemsetAsync(d.nlive, 0, 2 * sizeof(int), p->stream));
     EMP_HIP(hipStreamSynchronize(p->stream));
     return EMP_OK;
 }
@@ -272,7 +326,8 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
         (rc = alloc(p, &v.map, TW)) || (rc = alloc(p, &v.swap_l, TW)) ||
         (rc = alloc(p, &v.betas, (size_t)v.T)) || (rc = alloc(p, &v.accepted, TW)) || (rc = alloc(p, &v.sweep, 2)) ||
         (rc = alloc(p, &v.h_betas, S * v.T)) || (rc = alloc(p, &v.h_tsw, S * std::max(v.T - 1, 1))) ||
-        (rc = alloc(p, &v.h_smd, S * std::max(v.T - 1, 1)))) {
+        (rc = alloc(p, &v.h_smd, S * std::max(v.T - 1, 1))) || (rc = alloc(p, &v.prop_lp, TW / 2)) ||
+        (rc = alloc(p, &v.list, TW / 2)) || (rc = alloc(p, &v.nlive, 2))) {
         emp_pt_destroy(p);
         return rc;
     }
@@ -294,6 +349,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     EMP_TRY(hipMemcpy(v.betas, betas, (size_t)v.T * sizeof(double), hipMemcpyHostToDevice));
     EMP_TRY(hipMemset(v.accepted, 0, TW * sizeof(long long)));
     EMP_TRY(hipMemset(v.sweep, 0, 2 * sizeof(long long)));
+    EMP_TRY(hipMemset(v.nlive, 0, 2 * sizeof(int)));
     EMP_TRY(hipDeviceSynchronize());     // the engine stream does not wait for the null stream
 #undef EMP_TRY
     p->wpt = emp_pick_wpt(m, (int64_t)v.T * v.W2, d.precision);
@@ -351,12 +407,15 @@ extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     if (!p || n < 1 || !ms_avg) { emp_set_error("bad argument"); return EMP_EINVAL; }
     std::vector<hipEvent_t> ev(2 * (size_t)n);
     for (auto& e : ev) EMP_HIP(hipEventCreate(&e));
+    EMP_HIP(hipMemsetAsync(p->dev.nlive, 0, 2 * sizeof(int), p->stream));
     for (int i = 0; i < n; ++i) {
         EMP_HIP(hipEventRecord(ev[2 * i], p->stream));
         int rc = launch_half(p, i & 1, 0);
         if (rc) return rc;
         EMP_HIP(hipEventRecord(ev[2 * i + 1], p->stream));
     }
+    // leave both list counters clear for the next sweep's first half
+    EMP_HIP(hipMemsetAsync(p->dev.nlive, 0, 2 * sizeof(int), p->stream));
     EMP_HIP(hipStreamSynchronize(p->stream));
     double tot = 0.0;
     for (int i = 0; i < n; ++i) {
@@ -367,6 +426,52 @@ extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     for (auto& e : ev) (void)hipEventDestroy(e);
     *ms_avg = tot / n;
     p->last_ms_half = *ms_avg;
+    return EMP_OK;
+}
+
+// Times the half-step (both kernels) at 1, 2, 4 and 8 waves per walker on a
+// copy of the state, restores the state and keeps the fastest shape.
+extern "C" int emp_pt_autotune(EmpPT* p, int32_t trials, int32_t* best_wpt, double* best_ms) {
+    if (!p || trials < 1) { emp_set_error("bad argument"); return EMP_EINVAL; }
+    PTDev& v = p->dev;
+    const size_t TW = (size_t)v.T * v.W;
+    const std::pair<void*, size_t> items[] = {{v.X, TW * v.D * 8}, {v.lnl, TW * 8}, {v.lnp, TW * 8},
+                                              {v.accepted, TW * 8}, {v.sweep, 16}, {v.nlive, 8}};
+    std::vector<void*> copies;
+    auto release = [&]() { for (void* q : copies) (void)hipFree(q); };
+    for (const auto& it : items) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, it.second);
+        if (e != hipSuccess) { release(); return emp_hip_fail(e, "hipMalloc (autotune snapshot)", __FILE__, __LINE__); }
+        copies.push_back(q);
+        if ((e = hipMemcpyAsync(q, it.first, it.second, hipMemcpyDeviceToDevice, p->stream)) != hipSuccess) {
+            release();
+            return emp_hip_fail(e, "snapshot", __FILE__, __LINE__);
+        }
+    }
+    const int keep = p->wpt;
+    int best = keep;
+    double best_t = -1.0;
+    int rc = EMP_OK;
+    for (int w = 1; w <= 8 && !rc; w *= 2) {
+        p->wpt = w;
+        double ms = 0.0;
+        rc = emp_pt_time_halfstep(p, 2 * trials, &ms);
+        for (size_t i = 0; i < copies.size() && !rc; ++i)
+            if (hipMemcpyAsync(items[i].first, copies[i], items[i].second, hipMemcpyDeviceToDevice, p->stream) != hipSuccess)
+                rc = emp_hip_fail(hipGetLastError(), "restore", __FILE__, __LINE__);
+        if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "restore", __FILE__, __LINE__);
+        if (!rc && (best_t < 0 || ms < best_t)) { best_t = ms; best = w; }
+    }
+    release();
+    p->wpt = rc ? keep : best;
+    if (rc) return rc;
+    if (p->graph_exec) {
+        EMP_HIP(hipGraphExecDestroy(p->graph_exec));
+        p->graph_exec = nullptr;
+    }
+    if (best_wpt) *best_wpt = best;
+    if (best_ms) *best_ms = best_t;
     return EMP_OK;
 }
 
@@ -532,6 +637,7 @@ extern "C" int emp_pt_restore(EmpPT* p, const char* path) {
     const int64_t n = (int64_t)TW;
     hipLaunchKernelGGL(k_identity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, v.map, n);
     EMP_HIP(hipGetLastError());
+    EMP_HIP(hipMemsetAsync(v.nlive, 0, 2 * sizeof(int), p->stream));
     EMP_HIP(hipStreamSynchronize(p->stream));
     p->host_sweeps = sw;
     return EMP_OK;

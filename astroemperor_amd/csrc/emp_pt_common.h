@@ -39,81 +39,143 @@ __host__ __device__ inline PTView pt_view(int T, int W, int D, double a, uint32_
     return v;
 }
 
-// One slot's task of a fused stretch half-step: walker k = half W/2 + (b mod
-// W/2) of temperature t = b / (W/2).  Proposal (Goodman & Weare 2010 eq. 9,
-// partner from the complementary half), prior, likelihood when the prior is
-// finite, Metropolis acceptance of
+// Stretch proposal of task b of a half-step: walker k = half W/2 + (b mod
+// W/2) of temperature t = b / (W/2) moves towards/away from walker j of the
+// complementary half, Y = X_j + z (X_k - X_j) with z = ((a - 1) u + 1)^2 / a
+// (Goodman & Weare 2010, eq. 9).
+struct Proposal {
+    int t, k, sk, sj;     // temperature, walker, slots of k and j
+    double z, uacc;       // stretch factor, acceptance uniform
+};
+
+__device__ __forceinline__ Proposal pt_propose(const PTView& pv, long long sweep, int half, int step, int b) {
+    Proposal p;
+    p.t = b / pv.W2;
+    p.k = half * pv.W2 + b % pv.W2;
+    const uint32_t shi = (uint32_t)((unsigned long long)sweep >> 32) ^ pv.sysmix;
+    U4 c1{(uint32_t)(p.t * pv.W + p.k), (uint32_t)sweep, shi, (uint32_t)(step * 2 + half)};
+    const U4 r1 = philox4x32_10(c1, pv.key0, pv.key1);
+    U4 c2 = c1;
+    c2.w |= 0x80000000u;
+    const U4 r2 = philox4x32_10(c2, pv.key0, pv.key1);
+    const double uz = u01(r1.x, r1.y), uj = u01(r1.z, r1.w);
+    p.uacc = u01(r2.x, r2.y);
+    const double zz = (pv.a - 1.0) * uz + 1.0;
+    p.z = zz * zz / pv.a;
+    const int j = (1 - half) * pv.W2 + (int)(uj * (double)pv.W2);
+    p.sk = pv.map[p.t * pv.W + p.k];
+    p.sj = pv.map[p.t * pv.W + j];
+    return p;
+}
+
+// Free parameter f of the proposal (an explicit FMA, so that every caller
+// forms the same value).
+__device__ __forceinline__ double pt_prop_value(const double* Xk, const double* Xj, double z, int f) {
+    return __builtin_fma(z, Xk[f] - Xj[f], Xj[f]);
+}
+
+// Prior of a proposal, by one thread, in the order of prior_seq (and with
+// the same result bit for bit), reading the parameters straight from the
+// two walkers' rows.
+__device__ inline double prior_proposal(const ModelDev& md, const double* Xk, const double* Xj, double z) {
+#pragma clang fp contract(off)
+    double lp = 0.0;
+    for (int b = 0; b < md.nblocks; ++b) {
+        for (int j = md.block_start[b]; j < md.block_start[b + 1]; ++j) {
+            const int f = md.inv_free[j];
+            if (f < 0) continue;
+            lp += prior_term(md.prior[j], pt_prop_value(Xk, Xj, z, f));
+        }
+        if (lp == -__builtin_inf()) return lp;
+        if (b < md.K && md.ecc_prior[b].kind != EMP_PRIOR_NONE) {
+            const int o = md.off_kep + 5 * b + 3;
+            const int f3 = md.inv_free[o], f4 = md.inv_free[o + 1];
+            const double s3 = f3 >= 0 ? pt_prop_value(Xk, Xj, z, f3) : md.fixed_val[o];
+            const double s4 = f4 >= 0 ? pt_prop_value(Xk, Xj, z, f4) : md.fixed_val[o + 1];
+            const double x = s3 * s3 + s4 * s4;
+            lp += prior_term(md.ecc_prior[b], x);
+        }
+    }
+    return lp;
+}
+
+// Indices b < n with lp[b] > -inf, in increasing order, into list[]; every
+// thread of the workgroup (nthr threads, whole waves) gets the count.
+// s_wcnt: LDS scratch of one int per wave.
+__device__ inline int compact_live(const double* lp, int n, int* list, int* s_wcnt, int nthr) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = nthr >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < n; c0 += nthr) {
+        const int b = c0 + (int)threadIdx.x;
+        const bool f = b < n && lp[b] > -__builtin_inf();
+        const unsigned long long m = __ballot(f);
+        const int within = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wcnt[wave] = __popcll(m);
+        __syncthreads();
+        int off = base, tot = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int c = s_wcnt[w];
+            if (w < wave) off += c;
+            tot += c;
+        }
+        if (f) list[off + within] = b;
+        __syncthreads();
+        base += tot;
+    }
+    return base;
+}
+
+// One slot's task of a fused stretch half-step, for a task whose proposal
+// prior lp is already known (b < 0: no task for this slot in this pass):
+// proposal into LDS, likelihood, Metropolis acceptance of
 //   (D - 1) ln z + beta_t (lnL' - lnL) + ln pi' - ln pi.
 // Every evaluated likelihood adds one to *n_lnl (global or LDS).  The
 // sin/cos table must be in `tile`; with `resident` all epochs are too.
 // All threads of the workgroup call this together (it contains barriers).
 template <int PREC, int WPT>
-__device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long sweep, int half, int step,
-                             int64_t b, SlotLDS& sl, TileLDS& tile, ScanLDS& scan, double* s_lp, int* s_acc,
-                             int slot, int wsub, bool resident, unsigned long long* n_lnl) {
+__device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long sweep, int half, int step, int b,
+                             double lp, SlotLDS& sl, TileLDS& tile, ScanLDS& scan, int* s_acc, int slot, int wsub,
+                             bool resident, unsigned long long* n_lnl) {
     const int lane = threadIdx.x & 63;
     const int g = wsub * 64 + lane, G = 64 * WPT;
-    const bool task = b < (int64_t)pv.T * pv.W2;
-    int t = 0, k = 0, sk = 0;
-    double z = 1.0, uacc = 0.5;
-    const uint32_t shi = (uint32_t)((unsigned long long)sweep >> 32) ^ pv.sysmix;
-    if (task) {
-        t = (int)(b / pv.W2);
-        k = half * pv.W2 + (int)(b % pv.W2);
-        const uint32_t wid = (uint32_t)(t * pv.W + k);
-        U4 c1{wid, (uint32_t)sweep, shi, (uint32_t)(step * 2 + half)};
-        U4 r1 = philox4x32_10(c1, pv.key0, pv.key1);
-        U4 c2 = c1;
-        c2.w |= 0x80000000u;
-        U4 r2 = philox4x32_10(c2, pv.key0, pv.key1);
-        const double uz = u01(r1.x, r1.y), uj = u01(r1.z, r1.w);
-        uacc = u01(r2.x, r2.y);
-        const double zz = (pv.a - 1.0) * uz + 1.0;
-        z = zz * zz / pv.a;
-        const int j = (1 - half) * pv.W2 + (int)(uj * (double)pv.W2);
-        sk = pv.map[t * pv.W + k];
-        const int sj = pv.map[t * pv.W + j];
-        const double* Xk = pv.X + (size_t)sk * pv.D;
-        const double* Xj = pv.X + (size_t)sj * pv.D;
-        for (int d = g; d < md.D; d += G) sl.theta[md.free_idx[d]] = Xj[d] + z * (Xk[d] - Xj[d]);
+    const bool live = b >= 0 && lp > -__builtin_inf();
+    Proposal pr{0, 0, 0, 0, 1.0, 0.5};
+    if (live) {
+        pr = pt_propose(pv, sweep, half, step, b);
+        const double* Xk = pv.X + (size_t)pr.sk * pv.D;
+        const double* Xj = pv.X + (size_t)pr.sj * pv.D;
+        for (int d = g; d < md.D; d += G) sl.theta[md.free_idx[d]] = pt_prop_value(Xk, Xj, pr.z, d);
         for (int q = g; q < md.Dfull; q += G)
             if (md.inv_free[q] < 0) sl.theta[q] = md.fixed_val[q];
     }
     __syncthreads();
-    if (task && wsub == 0) {
-        if (lane == 0) s_lp[slot] = prior_seq(md, sl.theta);
-        slot_setup(md, sl, lane);
-    }
+    if (live && wsub == 0) slot_setup(md, sl, lane);
     __syncthreads();
-    const double lp = task ? s_lp[slot] : -__builtin_inf();
-    const bool live = task && lp == lp && lp != -__builtin_inf();
     if (PREC == 1 && live) {       // per-vector Kepler tables, one planet per wave
         for (int p = wsub; p < md.K; p += WPT)
-            if (sl.planet[p].e <= KT_EMAX) sl.ktab[p][lane] = ktab_node(sl.planet[p], tile.tab, lane);
+            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], tile.tab, lane);
     }
     __syncthreads();
     double ll = block_lnl<PREC, WPT, false>(md, sl, tile, scan, live, wsub, nullptr, nullptr, resident);
-    if (task && wsub == 0 && lane == 0) {
+    if (live && wsub == 0 && lane == 0) {
         int acc = 0;
-        if (live) {
-            atomicAdd(n_lnl, 1ull);
-            if (!(ll == ll) || ll == __builtin_inf()) ll = -__builtin_inf();
-            const double beta = pv.betas[t];
-            const double tn = beta == 0.0 ? 0.0 : beta * ll;
-            const double to = beta == 0.0 ? 0.0 : beta * pv.lnl[sk];
-            const double lnalpha = (double)(md.D - 1) * log(z) + (tn - to) + (lp - pv.lnp[sk]);
-            if (log(uacc) < lnalpha) {
-                pv.lnl[sk] = ll;
-                pv.lnp[sk] = lp;
-                pv.accepted[t * pv.W + k] += 1;
-                acc = 1;
-            }
+        atomicAdd(n_lnl, 1ull);
+        if (!(ll == ll) || ll == __builtin_inf()) ll = -__builtin_inf();
+        const double beta = pv.betas[pr.t];
+        const double tn = beta == 0.0 ? 0.0 : beta * ll;
+        const double to = beta == 0.0 ? 0.0 : beta * pv.lnl[pr.sk];
+        const double lnalpha = (double)(md.D - 1) * log(pr.z) + (tn - to) + (lp - pv.lnp[pr.sk]);
+        if (log(pr.uacc) < lnalpha) {
+            pv.lnl[pr.sk] = ll;
+            pv.lnp[pr.sk] = lp;
+            pv.accepted[pr.t * pv.W + pr.k] += 1;
+            acc = 1;
         }
         s_acc[slot] = acc;
     }
     __syncthreads();
-    if (task && s_acc[slot]) {
-        double* Xk = pv.X + (size_t)sk * pv.D;
+    if (live && s_acc[slot]) {
+        double* Xk = pv.X + (size_t)pr.sk * pv.D;
         for (int d = g; d < md.D; d += G) Xk[d] = sl.theta[md.free_idx[d]];
     }
 }

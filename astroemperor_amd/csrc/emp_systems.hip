@@ -24,6 +24,8 @@
 
 using namespace emp;
 
+constexpr int SYS_MAX_TASKS = 512;   // T W / 2 per system (per-task prior and list in LDS)
+
 struct SysDev {
     int S, T, W, nsteps;
     double a;
@@ -75,10 +77,12 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
     __shared__ SlotLDS s_slot[SLOTS];
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
-    __shared__ double s_lp[SLOTS];
     __shared__ int s_acc[SLOTS];
     __shared__ double s_red[64][2];
     __shared__ double s_rate[64];
+    __shared__ double s_tlp[SYS_MAX_TASKS];    // prior of each task's proposal
+    __shared__ int s_list[SYS_MAX_TASKS];      // tasks with a finite prior, in order
+    __shared__ int s_wcnt[MAX_WAVES];
     __shared__ long long s_item;
     __shared__ unsigned long long s_cnt;
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -104,9 +108,20 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
         if (res) tile_load(md, s_tile, 0, md.N);   // published by the first task's barrier
         for (int st = 0; st < sd.nsteps; ++st)
             for (int half = 0; half < 2; ++half) {
-                for (int b0 = 0; b0 < ntask; b0 += SLOTS)
-                    pt_half_task<PREC, WPT>(md, pv, sweep, half, st, b0 + slot, s_slot[slot], s_tile, s_scan, s_lp,
-                                            s_acc, slot, wsub, res, &s_cnt);
+                // Priors of all proposals, one thread each; then only the
+                // proposals inside the prior support take likelihood slots.
+                for (int b = tid; b < ntask; b += NT) {
+                    const Proposal pr = pt_propose(pv, sweep, half, st, b);
+                    s_tlp[b] = prior_proposal(md, pv.X + (size_t)pr.sk * md.D, pv.X + (size_t)pr.sj * md.D, pr.z);
+                }
+                __syncthreads();
+                const int nlive = compact_live(s_tlp, ntask, s_list, s_wcnt, NT);
+                for (int q0 = 0; q0 < nlive; q0 += SLOTS) {
+                    const int q = q0 + slot;
+                    const int b = q < nlive ? s_list[q] : -1;
+                    pt_half_task<PREC, WPT>(md, pv, sweep, half, st, b, b >= 0 ? s_tlp[b] : -__builtin_inf(),
+                                            s_slot[slot], s_tile, s_scan, s_acc, slot, wsub, res, &s_cnt);
+                }
                 __syncthreads();     // this half's writes before the other half reads them
             }
         // Swap round on LDS copies that alias the (now idle) slot scratch.
@@ -306,6 +321,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
     if (S < 1) { emp_set_error("n_systems must be >= 1"); return EMP_EINVAL; }
     if (T < 1 || T > 64) { emp_set_error("ntemps must be in [1, 64]"); return EMP_EINVAL; }
     if (W < 2 || W % 2 || W > 1024) { emp_set_error("nwalkers must be even and in [2, 1024]"); return EMP_EINVAL; }
+    if (T * (W / 2) > SYS_MAX_TASKS) { emp_set_error("ntemps x nwalkers / 2 must be <= %d", SYS_MAX_TASKS); return EMP_EUNSUP; }
     if (d.nsteps < 1) { emp_set_error("nsteps must be >= 1"); return EMP_EINVAL; }
     if (d.precision != EMP_FP64 && d.precision != EMP_MIXED) { emp_set_error("bad precision"); return EMP_EINVAL; }
     if (!(d.a > 1.0)) { emp_set_error("stretch scale a must be > 1"); return EMP_EINVAL; }

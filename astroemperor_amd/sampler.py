@@ -273,6 +273,13 @@ class PTSampler:
         _abi.check(_abi.lib().emp_pt_last_timing(self._pt, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def autotune(self, trials=10):
+        """Pick the fastest waves per walker for the half-step (state left
+        unchanged); returns (waves_per_walker, ms per half-step)."""
+        w, t = C.c_int32(), C.c_double()
+        _abi.check(_abi.lib().emp_pt_autotune(self._pt, int(trials), C.byref(w), C.byref(t)))
+        return w.value, t.value
+
     def time_halfstep(self, n=20):
         v = C.c_double()
         _abi.check(_abi.lib().emp_pt_time_halfstep(self._pt, int(n), C.byref(v)))

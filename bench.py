@@ -60,6 +60,7 @@ def run_gpu(work, precision, steps, warmup, seed, half_launches):
     s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=precision, adapt_tau=1000, adapt_nu=1)
     cap = warmup + steps + 4
     s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=cap)
+    wpt, _ = s.autotune(10)
     n0 = s.n_lnl_evaluated
     t0 = time.perf_counter()
     s.run_mcmc(None, nsweeps=steps)
@@ -75,7 +76,7 @@ def run_gpu(work, precision, steps, warmup, seed, half_launches):
                 lnl_evaluated_per_s=(n1 - n0) / wall,
                 accept_in_support=(n1 - n0) / (steps * T * W),
                 ms_halfstep=ms_half, evals_per_halfstep=(n2 - n1) / half_launches,
-                T=T, W=W)
+                T=T, W=W, waves_per_walker=wpt)
 
 
 def dlnl_report(work, samplers):
@@ -252,8 +253,10 @@ def main():
                    "ndim": int(m.ndim__), "ntemps": T, "nwalkers": W},
         "lnl_evals_per_s": round(mixed["proposals_per_s"], 1),
         "lnl_evaluated_per_s": round(mixed["lnl_evaluated_per_s"], 1),
+        "waves_per_walker": mixed["waves_per_walker"],
         "fp64": {"value": round(fp64["sweeps_per_s"], 2), "lnl_evals_per_s": round(fp64["proposals_per_s"], 1),
-                 "ms_per_step": round(fp64["ms_per_sweep"], 5), "halfstep_ms": round(fp64["ms_halfstep"], 5)},
+                 "ms_per_step": round(fp64["ms_per_sweep"], 5), "halfstep_ms": round(fp64["ms_halfstep"], 5),
+                 "waves_per_walker": fp64["waves_per_walker"]},
         "mixed_over_fp64": round(mixed["sweeps_per_s"] / fp64["sweeps_per_s"], 3),
         "dlnl_mixed_vs_fp64": dl,
         "roofline": roof,
