@@ -70,12 +70,6 @@ __device__ __forceinline__ double reduce_2pi(double M) {
     return __builtin_fma(-n, TWO_PI_LO, r);
 }
 
-// One-term reduction (mixed path): the error n |2 pi - TWO_PI_HI| stays below
-// 1e-12 rad for |M| < 2.5e4 rad (6000 d of a 1.5 d period).
-__device__ __forceinline__ double reduce_2pi_1(double M) {
-    return __builtin_fma(-__builtin_rint(M * INV_TWO_PI), TWO_PI_HI, M);
-}
-
 // ----------------------------------------------------------------------------
 // Per-planet constants, computed once per theta (per planet) and kept in LDS.
 // ----------------------------------------------------------------------------
@@ -328,23 +322,41 @@ __device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) 
 }
 
 // One FP64 step from the FP32 root Ef (Ma = |M| in [0, pi]): Newton from the
-// table start (<= 6e-7 rad), Halley from the FP32 Markley start.  The step is
-// below 1e-6 rad, so sin E, cos E follow from a first-order rotation
-// (neglected d^2 / 2 < 1e-12).  Planet term error <= 1.5e-12 K for e < 0.99.
+// table start (<= 6e-7 rad, so sin E, cos E follow from a first-order
+// rotation, neglected d^2 / 2 < 2e-13), Halley with a third-order rotation
+// from the FP32 Markley start.
 template <bool HALLEY>
 __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, const SinCosTab& tb, double& sE,
                                                  double& cE) {
     Ef = __builtin_amdgcn_fmed3f(Ef, 0.0f, (float)PI_D);
-    const double E0 = (double)Ef;
+    double E0 = (double)Ef;
     double s, c;
-    sincos_tab_f(tb, Ef, E0, s, c);
+    // Near periastron at high e the step amplifies an error in sin E0 by
+    // 1 / (1 - e cos E0): there the full-accuracy table evaluation is used
+    // (on [0, pi]: (float)pi lies above pi).
+    if (HALLEY) {
+        E0 = __builtin_fmin(E0, PI_D);
+        sincos_tab(tb, E0, s, c);
+    } else {
+        sincos_tab_f(tb, Ef, E0, s, c);
+    }
     const double f0 = E0 - __builtin_fma(e, s, Ma);
     const double f1 = __builtin_fma(-e, c, 1.0);
     const double r1 = rcp64_1(f1);
     double dl = -f0 * r1;
-    if (HALLEY) dl *= __builtin_fma(-0.5 * dl * e * s, r1, 1.0);
-    sE = __builtin_fma(c, dl, s);
-    cE = __builtin_fma(-s, dl, c);
+    if (HALLEY) {
+        // The FP32 Markley start (hardware sin/cos) is off by up to ~1e-4 rad
+        // near periastron at e -> 1: rotate to third order (d^4 / 24 < 5e-18).
+        dl *= __builtin_fma(-0.5 * dl * e * s, r1, 1.0);
+        const double dd = dl * dl;
+        const double sd = dl * __builtin_fma(dd, -1.0 / 6.0, 1.0);
+        const double cd = __builtin_fma(dd, -0.5, 1.0);
+        sE = __builtin_fma(c, sd, s * cd);
+        cE = __builtin_fma(-s, sd, c * cd);
+    } else {
+        sE = __builtin_fma(c, dl, s);
+        cE = __builtin_fma(-s, dl, c);
+    }
 }
 
 // Planet term at time t.  PREC 0: FP64 Markley; PREC 1: mixed, with `kt` the
@@ -359,7 +371,7 @@ __device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& 
         const double den = __builtin_fma(-pc.e, cE, 1.0);
         return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64(den);
     }
-    const double M = reduce_2pi_1(planet_M(pc, t));
+    const double M = reduce_2pi(planet_M(pc, t));
     const double Ma = __builtin_fabs(M);
     const float Mf = (float)Ma;
     if (pc.ktab) kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);

@@ -193,3 +193,22 @@ def test_large_config_mixed_vs_fp64():
     idx = rng.choice(np.where(fin)[0], 16, replace=False)
     ref = np.array([o.my_likelihood(t) for t in th[idx]])
     assert np.max(np.abs(a[idx] - ref) / np.abs(ref)) <= REL
+
+
+def test_high_eccentricity_mixed_vs_fp64():
+    """Every planet at e in [0.9, 0.99] (the FP32 Markley start and the FP64
+    Halley step of the mixed path), on draws that fit the data badly, which
+    magnifies every model error: still within 1e-6 of the FP64 path."""
+    from astroemperor_amd import synth
+    m = synth.c2()["model"]
+    rng = np.random.default_rng(17)
+    th = _draws(m, 4096, rng)
+    ecol = [i for i, n in enumerate(m.names_free()) if n.startswith("Eccentricity")]
+    th[:, ecol] = rng.uniform(0.9, 0.99, size=(len(th), len(ecol)))
+    a = m.lnl_batch(th, "fp64")
+    b = m.lnl_batch(th, "mixed")
+    fin = np.isfinite(a)
+    assert fin.sum() > 1000 and np.array_equal(fin, np.isfinite(b))
+    dm = np.abs(a[fin] - b[fin])
+    print(f"high-e mixed vs fp64: max {dm.max():.2e} p99 {np.percentile(dm, 99):.2e}", flush=True)
+    assert dm.max() < 1e-6
