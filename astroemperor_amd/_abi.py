@@ -106,12 +106,30 @@ _SIGS = {
 _lib = None
 
 
+def _preload_torch_hip():
+    """PyTorch-ROCm ships its own HIP runtime under the same soname
+    (libamdhip64.so.7).  Load that copy first, so that this library binds to
+    it: a process must hold one HIP runtime, and tensors and streams pass
+    between torch and this library."""
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
 def lib():
     """Load the HIP library (raises EmpError when it is not built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise EmpError(-4, f"{LIB_PATH} is not built; run astroemperor_amd.build.build()")
+        _preload_torch_hip()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             if hasattr(L, name):

@@ -17,6 +17,11 @@ constexpr int BLOCK = 512;       // threads per workgroup (8 waves)
 constexpr int NWAVES = BLOCK / 64;
 constexpr int MAX_WAVES = 16;    // largest workgroup any kernel launches (1024 threads)
 constexpr int TILE = 512;        // epochs staged in LDS per tile
+// Prior sequence entries: j >= 0 the prior of (free) full parameter j;
+// PSEQ_ECC - b the derived eccentricity prior of Keplerian block b;
+// PSEQ_CHECK: return -inf if the sum is -inf (the end of a block's own terms).
+constexpr int PSEQ_CHECK = -1;
+constexpr int PSEQ_ECC = -2;
 
 // Everything a kernel needs to know about a model, passed by value.
 struct ModelDev {
@@ -37,6 +42,8 @@ struct ModelDev {
     const EmpPriorTerm* prior;        // [Dfull]
     const int* block_start;           // [nblocks + 1]
     const EmpPriorTerm* ecc_prior;    // [K]
+    const int* pseq;                  // [npseq] prior terms in summation order (see PSEQ_*)
+    int npseq;
     const double2* sctab;             // [SC_N + 1] (sin, cos) of k pi / SC_N
     double lnl_const;                 // -0.5 ln(2 pi) N
     double t0;                        // X_[0] (acceleration origin)
@@ -75,7 +82,7 @@ struct ModelHost {
     emp::ModelDev dev{};
     std::vector<int> order;   // device position -> caller position
     std::vector<double> t, y, s2, dt, fixed;
-    std::vector<int> ins, inv, free_idx, bstart;
+    std::vector<int> ins, inv, free_idx, bstart, pseq;
     std::vector<EmpPriorTerm> prior, ecc;
 };
 int emp_model_host(const EmpModelDesc* d, ModelHost* h);

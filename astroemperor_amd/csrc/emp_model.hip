@@ -286,6 +286,15 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
         if (d->ecc_prior && k < d->n_planets) h->ecc[k] = d->ecc_prior[k];
         else { h->ecc[k] = EmpPriorTerm{}; h->ecc[k].kind = EMP_PRIOR_NONE; }
     }
+    // The prior's summation order (prior_seq): each block's free parameters,
+    // the -inf check, then the block's derived eccentricity prior.
+    h->pseq.clear();
+    for (int b = 0; b < d->n_blocks; ++b) {
+        for (int j = h->bstart[b]; j < h->bstart[b + 1]; ++j)
+            if (h->inv[j] >= 0) h->pseq.push_back(j);
+        h->pseq.push_back(PSEQ_CHECK);
+        if (b < d->n_planets && h->ecc[b].kind != EMP_PRIOR_NONE) h->pseq.push_back(PSEQ_ECC - b);
+    }
     ModelDev& md = h->dev;
     md = ModelDev{};
     md.N = N; md.I = d->n_ins; md.K = d->n_planets; md.param = d->parameterisation; md.use_c = d->use_c;
@@ -295,6 +304,7 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
     md.off_kep = d->off_kep; md.off_acc = d->off_acc; md.off_off = d->off_offset;
     md.off_jit = d->off_jitter; md.off_ma = d->off_ma;
     md.lnl_const = -0.5 * std::log(2.0 * M_PI) * (double)N;
+    md.npseq = (int)h->pseq.size();
     md.t0 = d->t[0];
     // Fixed MA timescales (the usual setup): the weight exp(-dt / tau) of
     // every epoch is the same for all parameter vectors; compute it once.
@@ -358,6 +368,7 @@ extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
     EMP_UP(h.prior, md.prior);
     EMP_UP(h.bstart, md.block_start);
     EMP_UP(h.ecc, md.ecc_prior);
+    EMP_UP(h.pseq, md.pseq);
     {
         std::vector<double2> tab;
         emp_sincos_table(tab);

@@ -69,41 +69,42 @@ static __host__ __device__ inline PTView view_of(const PTDev& pd) {
 }
 
 // ---------------------------------------------------------------------------
-// Half-step, part 1: the prior of every proposal (one thread per walker of
-// the moving half) and the list of those inside the prior support.  Each
-// workgroup reserves its part of the list with one atomic (the list order
-// varies from run to run; what each task computes does not).  The other
-// half's counter is cleared here: nothing reads it until that half's own
-// part 1 has run again.
+// Half-step, part 1: the prior of every proposal (one wave per walker of the
+// moving half, its lanes on the prior's terms) and the list of those inside
+// the prior support.  Each workgroup reserves its part of the list with one
+// atomic (the list order varies from run to run; what each task computes
+// does not).  The other half's counter is cleared here: nothing reads it
+// until that half's own part 1 has run again.
 // ---------------------------------------------------------------------------
 constexpr int PRIOR_BLOCK = 256;
+constexpr int PRIOR_WAVES = PRIOR_BLOCK / 64;
 
 __global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd, int half, int step) {
-    __shared__ int s_wcnt[PRIOR_BLOCK / 64];
+    __shared__ int s_f[PRIOR_WAVES];
     __shared__ int s_base;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = blockIdx.x * PRIOR_BLOCK + threadIdx.x;
+    const int b = blockIdx.x * PRIOR_WAVES + wave;
     const PTView pv = view_of(pd);
-    bool f = false;
+    int f = 0;
     if (b < pd.T * pd.W2) {
         const Proposal pr = pt_propose(pv, *pd.sweep, half, step, b);
-        const double lp = prior_proposal(md, pd.X + (size_t)pr.sk * pd.D, pd.X + (size_t)pr.sj * pd.D, pr.z);
-        pd.prop_lp[b] = lp;
+        const double lp = prior_proposal_group<64>(md, pd.X + (size_t)pr.sk * pd.D, pd.X + (size_t)pr.sj * pd.D,
+                                                   pr.z, lane, 0);
+        if (lane == 0) pd.prop_lp[b] = lp;
         f = lp > -__builtin_inf();
     }
-    const unsigned long long m = __ballot(f);
-    if (lane == 0) s_wcnt[wave] = __popcll(m);
+    if (lane == 0) s_f[wave] = f;
     __syncthreads();
     if (threadIdx.x == 0) {
         int tot = 0;
-        for (int w = 0; w < PRIOR_BLOCK / 64; ++w) tot += s_wcnt[w];
+        for (int w = 0; w < PRIOR_WAVES; ++w) tot += s_f[w];
         s_base = tot ? atomicAdd(&pd.nlive[half], tot) : 0;
         if (blockIdx.x == 0) pd.nlive[1 - half] = 0;
     }
     __syncthreads();
-    if (f) {
-        int off = s_base + __popcll(m & ((1ull << lane) - 1ull));
-        for (int w = 0; w < wave; ++w) off += s_wcnt[w];
+    if (lane == 0 && f) {
+        int off = s_base;
+        for (int w = 0; w < wave; ++w) off += s_f[w];
         pd.list[off] = b;
     }
 }
@@ -224,7 +225,7 @@ static int launch_half(EmpPT* p, int half, int step) {
     const int prec = p->desc.precision, w = p->wpt;
     {
         const int64_t tasks = (int64_t)p->dev.T * p->dev.W2;
-        hipLaunchKernelGGL(k_pt_prior, dim3((unsigned)((tasks + PRIOR_BLOCK - 1) / PRIOR_BLOCK)), dim3(PRIOR_BLOCK), 0,
+        hipLaunchKernelGGL(k_pt_prior, dim3((unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES)), dim3(PRIOR_BLOCK), 0,
                            p->stream, md, p->dev, half, step);
         EMP_HIP(hipGetLastError());
     }

@@ -74,26 +74,42 @@ __device__ __forceinline__ double pt_prop_value(const double* Xk, const double* 
     return __builtin_fma(z, Xk[f] - Xj[f], Xj[f]);
 }
 
-// Prior of a proposal, by one thread, in the order of prior_seq (and with
-// the same result bit for bit), reading the parameters straight from the
-// two walkers' rows.
-__device__ inline double prior_proposal(const ModelDev& md, const double* Xk, const double* Xj, double z) {
+// Prior of a proposal, in the order of prior_seq, read straight from the two
+// walkers' rows, by a group of G lanes (a power of two dividing 64; gl =
+// lane within the group, g0 = the group's first lane of the wave): the lanes
+// evaluate the terms of the summation sequence md.pseq in parallel, with
+// their loads in flight together, then every lane adds them up in sequence
+// order with prior_seq's -inf return, so the sum is bit-identical.
+template <int G>
+__device__ inline double prior_proposal_group(const ModelDev& md, const double* Xk, const double* Xj, double z,
+                                              int gl, int g0) {
 #pragma clang fp contract(off)
     double lp = 0.0;
-    for (int b = 0; b < md.nblocks; ++b) {
-        for (int j = md.block_start[b]; j < md.block_start[b + 1]; ++j) {
-            const int f = md.inv_free[j];
-            if (f < 0) continue;
-            lp += prior_term(md.prior[j], pt_prop_value(Xk, Xj, z, f));
+    for (int i0 = 0; i0 < md.npseq; i0 += G) {
+        const int i = i0 + gl;
+        double v = 0.0;
+        if (i < md.npseq) {
+            const int q = md.pseq[i];
+            if (q >= 0) {
+                v = prior_term(md.prior[q], pt_prop_value(Xk, Xj, z, md.inv_free[q]));
+            } else if (q <= PSEQ_ECC) {
+                const int b = PSEQ_ECC - q;
+                const int o = md.off_kep + 5 * b + 3;
+                const int f3 = md.inv_free[o], f4 = md.inv_free[o + 1];
+                const double s3 = f3 >= 0 ? pt_prop_value(Xk, Xj, z, f3) : md.fixed_val[o];
+                const double s4 = f4 >= 0 ? pt_prop_value(Xk, Xj, z, f4) : md.fixed_val[o + 1];
+                const double x = s3 * s3 + s4 * s4;
+                v = prior_term(md.ecc_prior[b], x);
+            }
         }
-        if (lp == -__builtin_inf()) return lp;
-        if (b < md.K && md.ecc_prior[b].kind != EMP_PRIOR_NONE) {
-            const int o = md.off_kep + 5 * b + 3;
-            const int f3 = md.inv_free[o], f4 = md.inv_free[o + 1];
-            const double s3 = f3 >= 0 ? pt_prop_value(Xk, Xj, z, f3) : md.fixed_val[o];
-            const double s4 = f4 >= 0 ? pt_prop_value(Xk, Xj, z, f4) : md.fixed_val[o + 1];
-            const double x = s3 * s3 + s4 * s4;
-            lp += prior_term(md.ecc_prior[b], x);
+        const int n = min(G, md.npseq - i0);
+        for (int k = 0; k < n; ++k) {
+            const double vk = __shfl(v, g0 + k, 64);
+            if (md.pseq[i0 + k] == PSEQ_CHECK) {
+                if (lp == -__builtin_inf()) return lp;
+            } else {
+                lp += vk;
+            }
         }
     }
     return lp;

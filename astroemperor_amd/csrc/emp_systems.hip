@@ -70,6 +70,68 @@ __device__ __forceinline__ PTView sys_view(const SysDev& sd, int s, int D) {
                    sd.accepted + s * TW);
 }
 
+// Prior of every proposal of one half-step of one system into lp[ntask], in
+// three workgroup-wide steps over `scratch` (bytes of LDS not otherwise in
+// use): the proposals (one thread per task), every (task, term) pair of the
+// prior's summation sequence md.pseq (all loads in flight together), then
+// the sum of each task's terms in sequence order with prior_seq's -inf
+// return (one thread per task), bit-identical to prior_seq.  Ends with a
+// barrier.
+__device__ inline void sys_priors(const ModelDev& md, const PTView& pv, long long sweep, int half, int step,
+                                  int ntask, char* scratch, size_t bytes, double* lp, int nthr) {
+#pragma clang fp contract(off)
+    double* z = (double*)scratch;                      // [ntask]
+    int* sk = (int*)(z + ntask);                       // [ntask]
+    int* sj = sk + ntask;                              // [ntask]
+    double* terms = (double*)(sj + ntask);             // 8-byte aligned: z, sk, sj take 16 ntask bytes
+    const int L = md.npseq;
+    const int chunk = (int)min((size_t)ntask, (bytes - (size_t)((char*)terms - scratch)) / (sizeof(double) * L));
+    for (int b = threadIdx.x; b < ntask; b += nthr) {
+        const Proposal pr = pt_propose(pv, sweep, half, step, b);
+        z[b] = pr.z;
+        sk[b] = pr.sk;
+        sj[b] = pr.sj;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < ntask; c0 += chunk) {
+        const int cn = min(chunk, ntask - c0);
+        for (int e = threadIdx.x; e < cn * L; e += nthr) {
+            const int bl = e / L, i = e - bl * L, b = c0 + bl;
+            const int q = md.pseq[i];
+            if (q == PSEQ_CHECK) continue;
+            const double* Xk = pv.X + (size_t)sk[b] * pv.D;
+            const double* Xj = pv.X + (size_t)sj[b] * pv.D;
+            double v;
+            if (q >= 0) {
+                v = prior_term(md.prior[q], pt_prop_value(Xk, Xj, z[b], md.inv_free[q]));
+            } else {
+                const int pb = PSEQ_ECC - q;
+                const int o = md.off_kep + 5 * pb + 3;
+                const int f3 = md.inv_free[o], f4 = md.inv_free[o + 1];
+                const double s3 = f3 >= 0 ? pt_prop_value(Xk, Xj, z[b], f3) : md.fixed_val[o];
+                const double s4 = f4 >= 0 ? pt_prop_value(Xk, Xj, z[b], f4) : md.fixed_val[o + 1];
+                const double x = s3 * s3 + s4 * s4;
+                v = prior_term(md.ecc_prior[pb], x);
+            }
+            terms[e] = v;
+        }
+        __syncthreads();
+        for (int bl = threadIdx.x; bl < cn; bl += nthr) {
+            const double* tv = terms + (size_t)bl * L;
+            double s = 0.0;
+            for (int i = 0; i < L; ++i) {
+                if (md.pseq[i] == PSEQ_CHECK) {
+                    if (s == -__builtin_inf()) break;
+                } else {
+                    s += tv[i];
+                }
+            }
+            lp[c0 + bl] = s;
+        }
+        __syncthreads();
+    }
+}
+
 // One sweep of every system.  NW waves per workgroup, WPT waves per walker.
 template <int PREC, int WPT, int NW>
 __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
@@ -108,12 +170,9 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
         if (res) tile_load(md, s_tile, 0, md.N);   // published by the first task's barrier
         for (int st = 0; st < sd.nsteps; ++st)
             for (int half = 0; half < 2; ++half) {
-                // Priors of all proposals, one thread each; then only the
-                // proposals inside the prior support take likelihood slots.
-                for (int b = tid; b < ntask; b += NT) {
-                    const Proposal pr = pt_propose(pv, sweep, half, st, b);
-                    s_tlp[b] = prior_proposal(md, pv.X + (size_t)pr.sk * md.D, pv.X + (size_t)pr.sj * md.D, pr.z);
-                }
+                // Priors of all proposals (in the idle slot memory), then only
+                // the proposals inside the prior support take likelihood slots.
+                sys_priors(md, pv, sweep, half, st, ntask, (char*)s_slot, sizeof(s_slot), s_tlp, NT);
                 __syncthreads();
                 const int nlive = compact_live(s_tlp, ntask, s_list, s_wcnt, NT);
                 for (int q0 = 0; q0 < nlive; q0 += SLOTS) {
@@ -334,7 +393,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
 
     // Host preparation and packing of every model.
     std::vector<ModelHost> hosts(S);
-    size_t nep = 0, npar = 0, nfree = 0, nblk = 0, necc = 0;
+    size_t nep = 0, npar = 0, nfree = 0, nblk = 0, necc = 0, nseq = 0;
     for (int s = 0; s < S; ++s) {
         int rc = emp_model_host(&models[s], &hosts[s]);
         if (rc) {
@@ -349,6 +408,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
         nfree += hosts[s].free_idx.size();
         nblk += hosts[s].bstart.size();
         necc += hosts[s].ecc.size();
+        nseq += hosts[s].pseq.size();
     }
     EmpSystems* p = new EmpSystems();
     p->desc = d;
@@ -363,19 +423,19 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
         p->n_cu = prop.multiProcessorCount;
     }
     std::vector<double> t(nep), y(nep), s2(nep), dt(nep), fixed(npar);
-    std::vector<int> ins(nep), inv(npar), fidx(nfree), bst(nblk);
+    std::vector<int> ins(nep), inv(npar), fidx(nfree), bst(nblk), pseq(nseq);
     std::vector<EmpPriorTerm> pri(npar), ecc(necc);
     std::vector<ModelDev> mds(S);
-    std::vector<size_t> o_ep(S), o_par(S), o_free(S), o_blk(S), o_ecc(S);
+    std::vector<size_t> o_ep(S), o_par(S), o_free(S), o_blk(S), o_ecc(S), o_seq(S);
     p->D.resize(S);
     p->xoff.resize(S);
     p->rxoff.resize(S);
     {
-        size_t e = 0, q = 0, f = 0, b = 0, c = 0;
+        size_t e = 0, q = 0, f = 0, b = 0, c = 0, g = 0;
         int64_t xo = 0, ro = 0;
         for (int s = 0; s < S; ++s) {
             const ModelHost& h = hosts[s];
-            o_ep[s] = e; o_par[s] = q; o_free[s] = f; o_blk[s] = b; o_ecc[s] = c;
+            o_ep[s] = e; o_par[s] = q; o_free[s] = f; o_blk[s] = b; o_ecc[s] = c; o_seq[s] = g;
             std::copy(h.t.begin(), h.t.end(), t.begin() + e);
             std::copy(h.y.begin(), h.y.end(), y.begin() + e);
             std::copy(h.s2.begin(), h.s2.end(), s2.begin() + e);
@@ -387,7 +447,9 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
             std::copy(h.free_idx.begin(), h.free_idx.end(), fidx.begin() + f);
             std::copy(h.bstart.begin(), h.bstart.end(), bst.begin() + b);
             std::copy(h.ecc.begin(), h.ecc.end(), ecc.begin() + c);
+            std::copy(h.pseq.begin(), h.pseq.end(), pseq.begin() + g);
             e += h.t.size(); q += h.inv.size(); f += h.free_idx.size(); b += h.bstart.size(); c += h.ecc.size();
+            g += h.pseq.size();
             p->D[s] = h.dev.D;
             p->xoff[s] = xo;
             p->rxoff[s] = ro;
@@ -406,7 +468,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
     const size_t STW = (size_t)S * T * W;
     int rc = 0;
     double *dt_, *dy, *ds2, *ddt, *dfix;
-    int *dins, *dinv, *dfidx, *dbst;
+    int *dins, *dinv, *dfidx, *dbst, *dseq;
     EmpPriorTerm *dpri, *decc;
     double2* dtab;
     std::vector<double2> tab;
@@ -414,7 +476,8 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
     if ((rc = upload(p, t, &dt_)) || (rc = upload(p, y, &dy)) || (rc = upload(p, s2, &ds2)) ||
         (rc = upload(p, dt, &ddt)) || (rc = upload(p, ins, &dins)) || (rc = upload(p, inv, &dinv)) ||
         (rc = upload(p, fixed, &dfix)) || (rc = upload(p, pri, &dpri)) || (rc = upload(p, fidx, &dfidx)) ||
-        (rc = upload(p, bst, &dbst)) || (rc = upload(p, ecc, &decc)) || (rc = upload(p, tab, &dtab))) {
+        (rc = upload(p, bst, &dbst)) || (rc = upload(p, ecc, &decc)) || (rc = upload(p, tab, &dtab)) ||
+        (rc = upload(p, pseq, &dseq))) {
         emp_systems_destroy(p);
         return rc;
     }
@@ -424,6 +487,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
         m.inv_free = dinv + o_par[s]; m.fixed_val = dfix + o_par[s]; m.prior = dpri + o_par[s];
         m.free_idx = dfidx + o_free[s]; m.block_start = dbst + o_blk[s]; m.ecc_prior = decc + o_ecc[s];
         m.sctab = dtab;
+        m.pseq = dseq + o_seq[s];
         mds[s] = m;
     }
     // Queue orders: longest first (stable), and the input order.

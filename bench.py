@@ -40,6 +40,10 @@ OPS_KEP = {"fp64": 24 + 3 + 20 * 5, "mixed": 24 * 0.5 + 3 + 4 * 4 + 20 * 1}
 OPS_EPOCH = 6 + 20 * 2
 OPS_MA = 3 + 20 * 1
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
+# HBM bytes per half-step launch pair (k_pt_prior + k_pt_half, mixed) on the
+# default workload, from rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by the same
+# counters on a copy of known size (profiles/); None until measured.
+TRAFFIC_BYTES_PER_LAUNCH = None
 
 
 def flops_per_theta(model, precision):
@@ -168,6 +172,149 @@ def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), tr
     return out
 
 
+def _state_dlnl(model, coords):
+    """|lnL_mixed - lnL_fp64| over a set of positions [..., D]."""
+    th = coords.reshape(-1, coords.shape[-1])
+    a = model.lnl_batch(th, "fp64")
+    b = model.lnl_batch(th, "mixed")
+    fin = np.isfinite(a) & np.isfinite(b)
+    d = np.abs(a[fin] - b[fin])
+    return {"n": int(fin.sum()), "max": float(d.max()) if d.size else None,
+            "p99": float(np.percentile(d, 99)) if d.size else None}
+
+
+def run_c1(seed, cpu_cores, with_cpu):
+    """1 Keplerian, 1 instrument, 256 epochs; 5 temps x 100 walkers x 500
+    sweeps, the full run on the GPU (FP64 and mixed) and on the host cores."""
+    from astroemperor_amd import PTSampler, synth
+    from astroemperor_amd.simulation import test_init
+    work = synth.c1()
+    m = work["model"]
+    T, W, S, _ = work["setup"]
+    p0 = test_init(m, T, W, np.random.default_rng(seed))
+    out = {"workload": f"1 Keplerian, 1 instrument, {m.data.ndat} epochs, {T} temps x {W} walkers x {S} sweeps"}
+    last = None
+    for prec in ("fp64", "mixed"):
+        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec)
+        s.run_mcmc(p0, nsweeps=1, capacity=S + 1)
+        t0 = time.perf_counter()
+        s.run_mcmc(None, nsweeps=S - 1)
+        wall = time.perf_counter() - t0
+        out[prec] = {"sweeps_per_s": round((S - 1) / wall, 1), "lnl_evals_per_s": round((S - 1) * T * W / wall, 1),
+                     "seconds": round(wall, 3)}
+        last = s.get_last_sample().coords
+        s.close()
+    out["dlnl_mixed_vs_fp64"] = _state_dlnl(m, last)
+    if with_cpu:
+        from oracle import cpu_reference
+        from astroemperor_amd.sampler import default_ladder
+        d = m.data
+        r = cpu_reference.run(m.spec(), d.X, d.Y, d.YERR, d.FLAG, p0, default_ladder(T, m.ndim__), S,
+                              ncores=cpu_cores, seed=seed)
+        out["cpu"] = {"sweeps_per_s": round(r["sweeps_per_s"], 2), "lnl_evals_per_s": round(r["lnl_evals_per_s"], 1),
+                      "cores": r["cores"], "seconds": round(r["seconds"], 2), "kind": "port",
+                      "sample": f"the full run ({S} sweeps)"}
+        out["fp64_over_cpu"] = round(out["fp64"]["sweeps_per_s"] / r["sweeps_per_s"], 1)
+    return out
+
+
+def run_c4(seed, batch=65536, reps=5, map_starts=1024):
+    """Adjoint gradients: 5 Keplerians, N = 2000, 2 instruments, a batch of
+    parameter vectors drawn from the prior; throughput of lnL + gradient,
+    a central finite-difference check and the batched L-BFGS MAP search."""
+    import torch
+    from astroemperor_amd import synth
+    from astroemperor_amd.autograd import map_search
+    work = synth.c4()
+    m = work["model"]
+    rng = np.random.default_rng(seed)
+    th = synth.initial_positions(m, 1, batch, rng)[0]
+    tt = torch.tensor(th, device="cuda")
+    m.lnl_grad_batch(tt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.lnl_grad_batch(tt)
+    torch.cuda.synchronize()
+    tg = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.lnl_batch(tt, "fp64")
+    torch.cuda.synchronize()
+    tl = (time.perf_counter() - t0) / reps
+    # FD check (FP64, h = 1e-6 max(|theta|, 1)) on 8 vectors
+    sub = th[:8]
+    _, g = m.lnl_grad_batch(sub)
+    errs = []
+    for i in range(len(sub)):
+        for j in range(sub.shape[1]):
+            h = 1e-6 * max(abs(sub[i, j]), 1.0)
+            xp, xm = sub[i].copy(), sub[i].copy()
+            xp[j] += h
+            xm[j] -= h
+            fd = (m.lnl_batch(xp[None])[0] - m.lnl_batch(xm[None])[0]) / (2 * h)
+            errs.append(abs(g[i, j] - fd) / max(abs(fd), 1.0))
+    t0 = time.perf_counter()
+    _, fbest = map_search(m, th[:map_starts], iters=50)
+    tmap = time.perf_counter() - t0
+    return {"workload": f"5 Keplerians, 2 instruments, {m.data.ndat} epochs, batch {batch} from the prior",
+            "lnl_grad_evals_per_s": round(batch / tg, 1), "lnl_evals_per_s": round(batch / tl, 1),
+            "grad_over_lnl_time": round(tg / tl, 2),
+            "fd_check": {"entries": len(errs), "max_rel": float(np.max(errs)), "median_rel": float(np.median(errs))},
+            "map_search": {"starts": map_starts, "iters": 50, "seconds": round(tmap, 2),
+                           "best_lnpost": float(np.max(fbest))}}
+
+
+def run_c5(seed, steps, warmup):
+    """Large-scale mixed: 7 Keplerians, N = 10000 over 5 instruments with
+    MA(1), 16 temps x 1024 walkers; a shortened timed run."""
+    from astroemperor_amd import PTSampler, synth
+    from astroemperor_amd.simulation import test_init
+    work = synth.c5()
+    m = work["model"]
+    T, W, _, _ = work["setup"]
+    p0 = test_init(m, T, W, np.random.default_rng(seed))
+    out = {"workload": f"7 Keplerians, 5 instruments, {m.data.ndat} epochs, MA(1), {T} temps x {W} walkers, "
+                       f"{steps} timed sweeps (stretch moves only)"}
+    last = None
+    for prec in ("mixed", "fp64"):
+        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec, record=False)
+        s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=warmup + steps + 1)
+        wpt, _ = s.autotune(3)
+        n0 = s.n_lnl_evaluated
+        t0 = time.perf_counter()
+        s.run_mcmc(None, nsweeps=steps)
+        wall = time.perf_counter() - t0
+        n1 = s.n_lnl_evaluated
+        h = s.time_halfstep(6)
+        n2 = s.n_lnl_evaluated
+        flops = flops_per_theta(m, prec) * (n2 - n1) / 6
+        ach = flops / (h * 1e-3) / 1e12
+        out[prec] = {"sweeps_per_s": round(steps / wall, 2), "lnl_evals_per_s": round(steps * T * W / wall, 1),
+                     "lnl_evaluated_per_s": round((n1 - n0) / wall, 1), "waves_per_walker": wpt,
+                     "roofline": {"bound": "valu", "kernel": f"k_pt_half<{prec}>", "achieved": round(ach, 3),
+                                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4)}}
+        if prec == "mixed":
+            last = s.get_last_sample().coords
+        s.close()
+    out["mixed_over_fp64"] = round(out["mixed"]["sweeps_per_s"] / out["fp64"]["sweeps_per_s"], 3)
+    out["dlnl_mixed_vs_fp64"] = _state_dlnl(m, last)
+    return out
+
+
+def run_copy(mib, reps):
+    """A plain device copy of known size (torch's copy kernel), the unit the
+    HBM counters (FETCH_SIZE, WRITE_SIZE) are calibrated against."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"copy_bytes_each_way": n * 8, "reps": reps}
+
+
 def run_cpu(work, sweeps, cores, seed):
     from oracle import cpu_reference
     from astroemperor_amd.simulation import test_init
@@ -195,17 +342,34 @@ def main():
     ap.add_argument("--systems", type=int, default=4096, help="batched-systems leg (0 = skip)")
     ap.add_argument("--systems-steps", type=int, default=30)
     ap.add_argument("--systems-warmup", type=int, default=5)
+    ap.add_argument("--no-configs", action="store_true", help="skip the other workloads (c1, c4, c5)")
+    ap.add_argument("--c5-steps", type=int, default=20)
+    ap.add_argument("--only", choices=["all", "c2", "systems", "configs", "copy"], default="all",
+                    help="run one leg only (profiling); 'copy' runs the HBM-counter calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the path shards by independent systems or temperature groups; see DESIGN.md)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by independent systems or by groups of temperatures)",
               file=sys.stderr)
         sys.exit(2)
 
     from astroemperor_amd import synth, device_info
+    info = device_info()
+    if args.only == "copy":
+        print(json.dumps({"calibration": run_copy(256, 10), "device": info}), flush=True)
+        return
+    if args.only == "systems":
+        print(json.dumps({"batched_systems": run_systems(args.systems, args.systems_steps, args.systems_warmup,
+                                                         args.seed), "device": info}), flush=True)
+        return
+    cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
+    if args.only == "configs":
+        print(json.dumps({"c1": run_c1(args.seed, cores, not args.no_cpu), "c4": run_c4(args.seed),
+                          "c5": run_c5(args.seed, args.c5_steps, 3), "device": info}), flush=True)
+        return
+
     work = synth.c2()
     m = work["model"]
-    info = device_info()
     res = {}
     for prec in ("fp64", "mixed"):
         res[prec] = run_gpu(work, prec, args.steps, args.warmup, args.seed, args.half_launches)
@@ -214,14 +378,24 @@ def main():
     dl = dlnl_report(work, [res["fp64"]["sampler"], res["mixed"]["sampler"]])
     for r in res.values():
         r["sampler"].close()
-    systems = run_systems(args.systems, args.systems_steps, args.systems_warmup, args.seed) if args.systems else None
+    full = args.only == "all"
+    systems = run_systems(args.systems, args.systems_steps, args.systems_warmup, args.seed) \
+        if full and args.systems else None
+    configs = None
+    if full and not args.no_configs:
+        configs = {"c1": run_c1(args.seed, cores, not args.no_cpu)}
+        print(f"[bench] c1: {configs['c1']}", file=sys.stderr, flush=True)
+        configs["c4"] = run_c4(args.seed)
+        print(f"[bench] c4: {configs['c4']}", file=sys.stderr, flush=True)
+        configs["c5"] = run_c5(args.seed, args.c5_steps, 3)
+        print(f"[bench] c5: {configs['c5']}", file=sys.stderr, flush=True)
 
     mixed, fp64 = res["mixed"], res["fp64"]
     flops = flops_per_theta(m, "mixed") * mixed["evals_per_halfstep"]
     achieved = flops / (mixed["ms_halfstep"] * 1e-3) / 1e12
     roof = {"bound": "valu", "kernel": "k_pt_half<mixed>", "achieved": round(achieved, 3),
             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
             "convention": "FP64-equivalent flops per lnL eval = 2 x (planet-epochs x ops_kep + epochs x 46"
                           " + MA epochs x 23); ops_kep = 127 (fp64) / 51 (mixed)",
             "flops_per_launch": flops, "launch_ms": mixed["ms_halfstep"],
@@ -231,7 +405,6 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
         r = run_cpu(work, args.cpu_sweeps, cores, args.seed)
         cpu = {"value": round(r["sweeps_per_s"], 4), "unit": "PT sweeps/s", "cores": r["cores"],
                "kind": "port",
@@ -262,6 +435,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched_systems": systems,
+        "configs": configs,
         "device": info,
     }
     if cpu:
