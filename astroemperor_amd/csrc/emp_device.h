@@ -269,13 +269,6 @@ constexpr double KT_EMAX = 0.5;
 // about 256 epochs on.
 constexpr int KT_MIN_EPOCHS = 256;
 
-// Node `lane` of planet pc's table (called by one wave: 64 lanes, 64 nodes).
-__device__ __forceinline__ float2 ktab_node(const PlanetC& pc, const SinCosTab& tb, int lane) {
-    double E, s, c;
-    kepler_fp64_E(lane * KT_H, pc, tb, E, s, c);
-    return make_float2((float)E, (float)(KT_H * rcp64(__builtin_fma(-pc.e, c, 1.0))));
-}
-
 __device__ __forceinline__ float ktab_start(const float2* kt, float Mf) {
     const float u = Mf * (float)((KT_N - 1) / PI_D);
     const int j = min((int)u, KT_N - 2);
@@ -319,6 +312,15 @@ __device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) 
         E += -f0 * __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f * dn, ef * s, f1));
     }
     return E;
+}
+
+// Node `lane` of planet pc's table (called by one wave: 64 lanes, 64 nodes),
+// solved in FP32: the nodes are stored in FP32, and FP32 roots leave the
+// interpolant's error where exact nodes put it (<= 6.2e-7 rad for e <= 0.5).
+__device__ __forceinline__ float2 ktab_node(const PlanetC& pc, int lane) {
+    const float E = kepler_fp32_start((float)(lane * KT_H), pc);
+    const float c = __builtin_amdgcn_cosf(E * 0.15915494f);
+    return make_float2(E, (float)KT_H * __builtin_amdgcn_rcpf(__builtin_fmaf(-(float)pc.e, c, 1.0f)));
 }
 
 // One FP64 step from the FP32 root Ef (Ma = |M| in [0, pi]): Newton from the

@@ -111,7 +111,7 @@ __device__ __forceinline__ void slot_prepare(const ModelDev& md, SlotLDS& sl, co
     __syncthreads();
     if (PREC == 1 && live) {
         for (int p = wsub; p < md.K; p += WPT)
-            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], tl.tab, lane);
+            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
     }
     __syncthreads();
 }

@@ -104,7 +104,15 @@ __device__ inline double prior_proposal_group(const ModelDev& md, const double* 
         }
         const int n = min(G, md.npseq - i0);
         for (int k = 0; k < n; ++k) {
-            const double vk = __shfl(v, g0 + k, 64);
+            double vk;
+            if (G == 64) {   // k is uniform: a scalar read of lane k
+                const long long bits = __double_as_longlong(v);
+                const int lo = __builtin_amdgcn_readlane((int)bits, k);
+                const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
+                vk = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+            } else {
+                vk = __shfl(v, g0 + k, 64);
+            }
             if (md.pseq[i0 + k] == PSEQ_CHECK) {
                 if (lp == -__builtin_inf()) return lp;
             } else {
@@ -169,7 +177,7 @@ __device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long swe
     __syncthreads();
     if (PREC == 1 && live) {       // per-vector Kepler tables, one planet per wave
         for (int p = wsub; p < md.K; p += WPT)
-            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], tile.tab, lane);
+            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
     }
     __syncthreads();
     double ll = block_lnl<PREC, WPT, false>(md, sl, tile, scan, live, wsub, nullptr, nullptr, resident);
