@@ -1,0 +1,51 @@
+"""Text summaries of rocprofv3 result databases (rocpd SQLite).
+
+    python -m astroemperor_amd.profiling RESULTS.db [--counters] [--top N]
+
+Without --counters: one row per kernel (dispatches, total and mean device
+time, VGPRs, LDS) from a ``--kernel-trace`` run.  With --counters: the mean
+value per dispatch of every collected counter, per kernel, from a ``--pmc``
+run (FETCH_SIZE / WRITE_SIZE are in KiB).
+"""
+from __future__ import annotations
+
+import argparse
+import sqlite3
+
+
+def kernel_table(db, top=12):
+    c = sqlite3.connect(db)
+    rows = c.execute(
+        "select name, count(*), sum(duration), avg(duration), avg(vgpr_count), avg(lds_size), avg(grid_x), "
+        "avg(workgroup_x) from kernels group by name order by sum(duration) desc limit ?", (top,)).fetchall()
+    total = c.execute("select sum(duration) from kernels").fetchone()[0] or 1
+    out = ["| kernel | dispatches | total ms | mean us | share | VGPRs | LDS B | grid | WG |",
+           "|---|---|---|---|---|---|---|---|---|"]
+    for name, n, tot, avg, vg, lds, grid, wg in rows:
+        out.append(f"| `{name[:80]}` | {n} | {tot / 1e6:.3f} | {avg / 1e3:.2f} | {100 * tot / total:.1f}% | "
+                   f"{vg:.0f} | {lds:.0f} | {grid:.0f} | {wg:.0f} |")
+    return "\n".join(out)
+
+
+def counter_table(db, top=12):
+    c = sqlite3.connect(db)
+    rows = c.execute(
+        "select kernel_name, counter_name, count(*), avg(value), min(value), max(value) from counters_collection "
+        "group by kernel_name, counter_name order by avg(value) desc limit ?", (top,)).fetchall()
+    out = ["| kernel | counter | dispatches | mean | min | max |", "|---|---|---|---|---|---|"]
+    for name, cn, n, avg, mn, mx in rows:
+        out.append(f"| `{name[:80]}` | {cn} | {n} | {avg:.1f} | {mn:.1f} | {mx:.1f} |")
+    return "\n".join(out)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("db")
+    ap.add_argument("--counters", action="store_true")
+    ap.add_argument("--top", type=int, default=12)
+    a = ap.parse_args()
+    print(counter_table(a.db, a.top) if a.counters else kernel_table(a.db, a.top))
+
+
+if __name__ == "__main__":
+    main()
