@@ -7,8 +7,22 @@
 // the planet term mu = A (cos(f + w) + e cos w) has
 //   dmu/dM = -A sin(f + w) b / D^2
 //   dmu/de = -A sin(f + w) (b sin E / D^2 + sin E/(b D)) + A cos w
-//   dmu/dw = -A (sin(f + w) + e sin w),   dmu/dA = cos(f + w) + e cos w,
-// and dlnL/dmu_i = r_i / err2_i,  dlnL/dJ_k = J_k sum_{i in k} (r_i^2/err2_i^2 - 1/err2_i).
+//   dmu/dw = -A (sin(f + w) + e sin w),   dmu/dA = cos(f + w) + e cos w.
+// Without the moving average dlnL/dmu_i = r_i / err2_i and
+// dlnL/dJ_k = J_k sum_{i in k} (r_i^2/err2_i^2 - 1/err2_i).
+//
+// MA(1) (moav01.model:7-15; per instrument for the intended semantics):
+// eps_i = r_i + a_i eps_{i-1}, a_i = -c exp(-dt_i / tau), and lnL sums
+// -(eps_i^2 / err2_i + ln err2_i) / 2.  Its adjoint lambda_i = dlnL/deps_i
+// obeys the reverse recurrence lambda_i = -eps_i / err2_i + a_{i+1}
+// lambda_{i+1}; then dlnL/dmu_i = -lambda_i, dlnL/dc = sum lambda_i
+// eps_{i-1} da_i/dc and dlnL/dtau likewise, with eps in place of r in the
+// jitter term.  The kernel makes two passes: the forward likelihood, which
+// stores eps_i (scratch, one row per vector), then the epochs in reverse
+// (tiles, then chunks), where the planet terms and their derivatives are
+// recomputed and lambda comes from a suffix scan with the carry running
+// backwards.
+//
 // The natural-parameter gradients are then mapped to the block's
 // parameterisation (block_repo.py:23-31) once per vector.
 #include <cmath>
@@ -22,11 +36,81 @@ struct GradLDS {
     double nat[KMAX][5];       // per planet: (M*tau, M, e, w, A) accumulators
 };
 
+// Model value of one epoch and its planet derivatives (natural parameters).
+struct EpochDeriv {
+    double mu;
+    double dM[KMAX], de[KMAX], dw[KMAX], dA[KMAX], tau[KMAX];
+};
+
+__device__ __forceinline__ void epoch_deriv(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
+                                            int k, EpochDeriv& d) {
+    d.mu = sl.gam[k];
+    const double x = t - md.t0;
+    if (md.acc) {
+        double v = 0.0;
+        for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
+        d.mu += v * x;
+    }
+#pragma unroll
+    for (int p = 0; p < KMAX; ++p) {
+        d.dM[p] = d.de[p] = d.dw[p] = d.dA[p] = d.tau[p] = 0.0;
+        if (p < md.K) {
+            const PlanetC& pc = sl.planet[p];
+            const double M = reduce_2pi(planet_M(pc, t));
+            double sE, cE;
+            kepler_fp64(M, pc, tab, sE, cE);
+            const double e = pc.e;
+            const double rD = rcp64(__builtin_fma(-e, cE, 1.0));
+            const double bq = sqrt((1.0 - e) * (1.0 + e));
+            const double cosf = (cE - e) * rD, sinf = bq * sE * rD;
+            const double cfw = cosf * pc.cw - sinf * pc.sw;
+            const double sfw = sinf * pc.cw + cosf * pc.sw;
+            const double A = pc.K;
+            d.mu += A * (cfw + e * pc.cw);
+            d.dM[p] = -A * sfw * bq * rD * rD;
+            d.de[p] = -A * sfw * (bq * sE * rD * rD + sE * rD / bq) + A * pc.cw;
+            d.dw[p] = -A * (sfw + e * pc.sw);
+            d.dA[p] = cfw + e * pc.cw;
+            d.tau[p] = pc.tp_mode ? (t - pc.tp) : t;
+        }
+    }
+}
+
+// Accumulates w = dlnL/dmu_i times the epoch's derivatives.
+__device__ __forceinline__ void accumulate(const ModelDev& md, const EpochDeriv& d, double w, double x,
+                                           double (&acc_nat)[KMAX][5], double (&acc_acc)[AMAX]) {
+#pragma unroll
+    for (int p = 0; p < KMAX; ++p) {
+        acc_nat[p][0] = __builtin_fma(w * d.dM[p], d.tau[p], acc_nat[p][0]);
+        acc_nat[p][1] = __builtin_fma(w, d.dM[p], acc_nat[p][1]);
+        acc_nat[p][2] = __builtin_fma(w, d.de[p], acc_nat[p][2]);
+        acc_nat[p][3] = __builtin_fma(w, d.dw[p], acc_nat[p][3]);
+        acc_nat[p][4] = __builtin_fma(w, d.dA[p], acc_nat[p][4]);
+    }
+    if (md.acc) {
+        double xp = x;
+#pragma unroll
+        for (int a = AMAX - 1; a >= 0; --a) {
+            if (a < md.acc) {
+                acc_acc[a] = __builtin_fma(w, xp, acc_acc[a]);
+                xp *= x;
+            }
+        }
+    }
+}
+
+// MA(1) multiplier of epoch n (device order) from its instrument and its MA
+// column (the fixed-timescale weight, or dt with -1 at a segment start).
+__device__ __forceinline__ double ma_mult(const ModelDev& md, const SlotLDS& sl, int k, double col) {
+    if (md.ma_wfixed) return -(sl.ma_c[k] * col);
+    return col < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-col / sl.ma_tau[k]));
+}
+
 template <int WPT>
 __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                     int64_t ld, double* __restrict__ out_lnl,
                                                     double* __restrict__ out_grad,
-                                                    unsigned long long* nonfinite) {
+                                                    unsigned long long* nonfinite, double* eps_scratch) {
     constexpr int SLOTS = NWAVES / WPT;
     __shared__ SlotLDS s_slot[SLOTS];
     __shared__ GradLDS s_grad[SLOTS];
@@ -54,78 +138,124 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     double acc_acc[AMAX];
 #pragma unroll
     for (int a = 0; a < AMAX; ++a) acc_acc[a] = 0.0;
-    double chi = 0.0, prod = 1.0;
-    int pexp = 0;
+    double lnl_v = 0.0;
 
-    for (int tb = 0; tb < md.N; tb += TILE) {
-        const int tn = min(TILE, md.N - tb);
-        __syncthreads();
-        tile_load(md, s_tile, tb, tn);
-        __syncthreads();
-        for (int cb = 0; cb < tn; cb += G) {
-            const int j = cb + g;
-            if (!(live && j < tn)) continue;
-            const double t = s_tile.t[j];
-            const int k = s_tile.ins[j];
-            double mu = sl.gam[k];
-            const double x = t - md.t0;
-            if (md.acc) {
-                double v = 0.0;
-                for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
-                mu += v * x;
+    if (!md.ma_active) {
+        double chi = 0.0, prod = 1.0;
+        int pexp = 0;
+        for (int tb = 0; tb < md.N; tb += TILE) {
+            const int tn = min(TILE, md.N - tb);
+            __syncthreads();
+            tile_load(md, s_tile, tb, tn);
+            __syncthreads();
+            for (int cb = 0; cb < tn; cb += G) {
+                const int j = cb + g;
+                if (!(live && j < tn)) continue;
+                const double t = s_tile.t[j];
+                const int k = s_tile.ins[j];
+                EpochDeriv d;
+                epoch_deriv(md, sl, s_tile.tab, t, k, d);
+                const double err2 = s_tile.s2[j] + sl.jit2[k];
+                const double r = s_tile.y[j] - d.mu;
+                const double ie = rcp64(err2);
+                const double w = r * ie;
+                chi = __builtin_fma(r * r, ie, chi);
+                prod *= err2;
+                pexp += __builtin_amdgcn_frexp_exp(prod);
+                prod = __builtin_amdgcn_frexp_mant(prod);
+                accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
+                atomicAdd(&gl.g[md.off_off + k], w);
+                if (md.has_jit) atomicAdd(&gl.g[md.off_jit + k], sl.jit[k] * (w * w - ie));
             }
-            double dM[KMAX], de[KMAX], dw[KMAX], dA[KMAX], tau[KMAX];
-#pragma unroll
-            for (int p = 0; p < KMAX; ++p) {
-                dM[p] = de[p] = dw[p] = dA[p] = tau[p] = 0.0;
-                if (p < md.K) {
-                    const PlanetC& pc = sl.planet[p];
-                    const double M = reduce_2pi(planet_M(pc, t));
-                    double sE, cE;
-                    kepler_fp64(M, pc, s_tile.tab, sE, cE);
-                    const double e = pc.e;
-                    const double rD = rcp64(__builtin_fma(-e, cE, 1.0));
-                    const double bq = sqrt((1.0 - e) * (1.0 + e));
-                    const double cosf = (cE - e) * rD, sinf = bq * sE * rD;
-                    const double cfw = cosf * pc.cw - sinf * pc.sw;
-                    const double sfw = sinf * pc.cw + cosf * pc.sw;
-                    const double A = pc.K;
-                    mu += A * (cfw + e * pc.cw);
-                    dM[p] = -A * sfw * bq * rD * rD;
-                    de[p] = -A * sfw * (bq * sE * rD * rD + sE * rD / bq) + A * pc.cw;
-                    dw[p] = -A * (sfw + e * pc.sw);
-                    dA[p] = cfw + e * pc.cw;
-                    tau[p] = pc.tp_mode ? (t - pc.tp) : t;
+        }
+        double ldet = live ? log(prod) + (double)pexp * LN2 : 0.0;
+        chi = wave_sum(chi);
+        ldet = wave_sum(ldet);
+        if (lane == 0) { s_scan.red[wave][0] = chi; s_scan.red[wave][1] = ldet; }
+        __syncthreads();
+        double c2 = 0.0, l2 = 0.0;
+        const int w0 = wave - wsub;
+        for (int w = 0; w < WPT; ++w) { c2 += s_scan.red[w0 + w][0]; l2 += s_scan.red[w0 + w][1]; }
+        lnl_v = -0.5 * (c2 + l2) + md.lnl_const;
+    } else {
+        // Pass 1: the likelihood, storing eps.
+        double* eps = eps_scratch + (live ? b : 0) * (int64_t)md.N;
+        lnl_v = block_lnl<0, WPT, false, true>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr, false, eps);
+        __syncthreads();   // eps rows written by every wave of the slot
+        // Pass 2: epochs in reverse, lambda by suffix scans.
+        double lam_carry = 0.0;
+        int parity = 0;
+        const int ntile = (md.N + TILE - 1) / TILE;
+        for (int it = ntile - 1; it >= 0; --it) {
+            const int tb = it * TILE, tn = min(TILE, md.N - tb);
+            __syncthreads();
+            tile_load(md, s_tile, tb, tn);
+            __syncthreads();
+            for (int cb = ((tn - 1) / G) * G; cb >= 0; cb -= G) {
+                const int j = cb + g;
+                const bool valid = live && j < tn;
+                EpochDeriv d;
+                double u = 0.0, amul = 1.0, e_i = 0.0, ie = 0.0, t = 0.0;
+                int k = 0;
+                if (valid) {
+                    t = s_tile.t[j];
+                    k = s_tile.ins[j];
+                    epoch_deriv(md, sl, s_tile.tab, t, k, d);
+                    const double err2 = s_tile.s2[j] + sl.jit2[k];
+                    ie = rcp64(err2);
+                    e_i = eps[tb + j];
+                    u = -e_i * ie;
+                    const int n1 = tb + j + 1;     // the next epoch's multiplier
+                    amul = 0.0;
+                    if (n1 < md.N) {
+                        const bool in_tile = j + 1 < tn;
+                        const int k1 = in_tile ? s_tile.ins[j + 1] : md.ins[n1];
+                        const double c1 = in_tile ? s_tile.dt[j + 1] : md.dt[n1];
+                        amul = ma_mult(md, sl, k1, c1);
+                    }
                 }
-            }
-            const double err2 = s_tile.s2[j] + sl.jit2[k];
-            const double r = s_tile.y[j] - mu;
-            const double ie = rcp64(err2);
-            const double w = r * ie;
-            chi = __builtin_fma(r * r, ie, chi);
-            prod *= err2;
-            pexp += __builtin_amdgcn_frexp_exp(prod);
-            prod = __builtin_amdgcn_frexp_mant(prod);
-#pragma unroll
-            for (int p = 0; p < KMAX; ++p) {
-                acc_nat[p][0] = __builtin_fma(w * dM[p], tau[p], acc_nat[p][0]);
-                acc_nat[p][1] = __builtin_fma(w, dM[p], acc_nat[p][1]);
-                acc_nat[p][2] = __builtin_fma(w, de[p], acc_nat[p][2]);
-                acc_nat[p][3] = __builtin_fma(w, dw[p], acc_nat[p][3]);
-                acc_nat[p][4] = __builtin_fma(w, dA[p], acc_nat[p][4]);
-            }
-            if (md.acc) {
-                double xp = x;
-#pragma unroll
-                for (int a = AMAX - 1; a >= 0; --a) {
-                    if (a < md.acc) {
-                        acc_acc[a] = __builtin_fma(w, xp, acc_acc[a]);
-                        xp *= x;
+                double sa = amul, sb = u;
+                wave_affine_suffix_scan(sa, sb, lane);
+                double lam;
+                if (WPT == 1) {
+                    lam = __builtin_fma(sa, lam_carry, sb);
+                    lam_carry = __shfl(lam, 0, 64);
+                } else {
+                    if (lane == 0) { s_scan.tot[parity][wave][0] = sa; s_scan.tot[parity][wave][1] = sb; }
+                    __syncthreads();
+                    const int w0 = wave - wsub;
+                    double pa = 1.0, pb = 0.0;
+                    for (int w = WPT - 1; w >= 0; --w) {
+                        if (w == wsub) lam = __builtin_fma(sa, __builtin_fma(pa, lam_carry, pb), sb);
+                        const double ta = s_scan.tot[parity][w0 + w][0], tbv = s_scan.tot[parity][w0 + w][1];
+                        pb = __builtin_fma(ta, pb, tbv);
+                        pa = ta * pa;
+                    }
+                    lam_carry = __builtin_fma(pa, lam_carry, pb);
+                    parity ^= 1;
+                }
+                if (valid) {
+                    const double w = -lam;
+                    accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
+                    atomicAdd(&gl.g[md.off_off + k], w);
+                    if (md.has_jit) atomicAdd(&gl.g[md.off_jit + k], sl.jit[k] * (e_i * e_i * ie * ie - ie));
+                    // da_i/dc = a_i / c, da_i/dtau = a_i dt / tau^2 (a_i = 0 at a segment start)
+                    const double col = s_tile.dt[j];
+                    const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
+                    if (!start) {
+                        const int n = tb + j;
+                        const double e_prev = eps[n - 1];
+                        const int set = md.ma_global ? 0 : k;
+                        const double wgt = md.ma_wfixed ? col : exp(-col / sl.ma_tau[k]);
+                        atomicAdd(&gl.g[md.off_ma + 2 * set], lam * e_prev * -wgt);
+                        if (!md.ma_wfixed) {
+                            const double tau = sl.ma_tau[k];
+                            atomicAdd(&gl.g[md.off_ma + 2 * set + 1],
+                                      lam * e_prev * -(sl.ma_c[k] * wgt * col / (tau * tau)));
+                        }
                     }
                 }
             }
-            atomicAdd(&gl.g[md.off_off + k], w);
-            if (md.has_jit) atomicAdd(&gl.g[md.off_jit + k], sl.jit[k] * (w * w - ie));
         }
     }
     // Reduce the natural-parameter accumulators over the slot's waves.
@@ -145,10 +275,6 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
             if (live && lane == 0) atomicAdd(&gl.g[md.off_acc + a], v);
         }
     }
-    double ldet = live ? log(prod) + (double)pexp * LN2 : 0.0;
-    chi = wave_sum(chi);
-    ldet = wave_sum(ldet);
-    if (lane == 0) { s_scan.red[wave][0] = chi; s_scan.red[wave][1] = ldet; }
     __syncthreads();
     if (live && wsub == 0) {
         // Map natural gradients to the parameterisation (one lane per planet).
@@ -187,9 +313,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     }
     __syncthreads();
     if (live && wsub == 0) {
-        double c2 = 0.0, l2 = 0.0;
-        for (int w = 0; w < WPT; ++w) { c2 += s_scan.red[wave + w][0]; l2 += s_scan.red[wave + w][1]; }
-        double v = -0.5 * (c2 + l2) + md.lnl_const;
+        double v = lnl_v;
         bool bad = !(v == v) || v == __builtin_inf();
         for (int d = lane; d < md.D; d += 64) {
             double gv = gl.g[md.free_idx[d]];
@@ -205,19 +329,38 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
 
 template <int WPT>
 static void launch_grad_t(const ModelDev& md, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
-                          unsigned long long* nf, hipStream_t s) {
+                          unsigned long long* nf, double* scratch, hipStream_t s) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t grid = (B + SLOTS - 1) / SLOTS;
-    hipLaunchKernelGGL((k_lnl_grad<WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, lnl, grad, nf);
+    hipLaunchKernelGGL((k_lnl_grad<WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, lnl, grad, nf,
+                       scratch);
 }
 
-static int launch_grad(const EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
-                       hipStream_t s) {
+// The MA adjoint keeps one row of eps per vector: a model-owned scratch of
+// B x N doubles, grown (never shrunk) on demand.
+static int ensure_grad_scratch(EmpModel* m, int64_t B) {
+    if (!m->dev.ma_active) return EMP_OK;
+    const size_t need = (size_t)B * (size_t)m->dev.N;
+    if (m->grad_scratch_cap >= need) return EMP_OK;
+    if (m->d_grad_scratch) {
+        EMP_HIP(hipDeviceSynchronize());
+        (void)hipFree(m->d_grad_scratch);
+        m->d_grad_scratch = nullptr;
+        m->grad_scratch_cap = 0;
+    }
+    EMP_HIP(hipMalloc((void**)&m->d_grad_scratch, need * sizeof(double)));
+    m->grad_scratch_cap = need;
+    return EMP_OK;
+}
+
+int emp_launch_grad(EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad, hipStream_t s) {
+    int rc = ensure_grad_scratch(m, B);
+    if (rc) return rc;
     int wpt = emp_pick_wpt(m, B, EMP_FP64);
-    if (wpt == 1) launch_grad_t<1>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, s);
-    else if (wpt == 2) launch_grad_t<2>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, s);
-    else if (wpt == 4) launch_grad_t<4>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, s);
-    else launch_grad_t<8>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, s);
+    if (wpt == 1) launch_grad_t<1>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
+    else if (wpt == 2) launch_grad_t<2>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
+    else if (wpt == 4) launch_grad_t<4>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
+    else launch_grad_t<8>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
     EMP_HIP(hipGetLastError());
     return EMP_OK;
 }
@@ -227,14 +370,10 @@ extern "C" int emp_lnl_grad_batch(EmpModel* m, const double* theta, int64_t B, i
     if (!m) { emp_set_error("null model"); return EMP_EINVAL; }
     if (B < 0 || (B > 0 && (!theta || !lnl || !grad))) { emp_set_error("bad batch arguments"); return EMP_EINVAL; }
     if (ld < m->desc.ndim) { emp_set_error("ld < ndim"); return EMP_EINVAL; }
-    if (m->dev.ma_active) {
-        emp_set_error("gradients with an active moving-average term are not implemented");
-        return EMP_EUNSUP;
-    }
     if (emp_fault_injected()) return EMP_EFAULT_INJECTED;
     if (B == 0) return EMP_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (!(flags & EMP_HOST_POINTERS)) return launch_grad(m, theta, B, ld, lnl, grad, s);
+    if (!(flags & EMP_HOST_POINTERS)) return emp_launch_grad(m, theta, B, ld, lnl, grad, s);
     const size_t nin = (size_t)B * ld, D = (size_t)m->desc.ndim;
     double *din = nullptr, *dl = nullptr, *dg = nullptr;
     EMP_HIP(hipMalloc(&din, nin * 8));
@@ -242,7 +381,7 @@ extern "C" int emp_lnl_grad_batch(EmpModel* m, const double* theta, int64_t B, i
     EMP_HIP(hipMalloc(&dg, std::max<size_t>((size_t)B * D, 1) * 8));
     int rc = EMP_OK;
     if (hipMemcpyAsync(din, theta, nin * 8, hipMemcpyHostToDevice, s) != hipSuccess) rc = EMP_EHIP;
-    if (!rc) rc = launch_grad(m, din, B, ld, dl, dg, s);
+    if (!rc) rc = emp_launch_grad(m, din, B, ld, dl, dg, s);
     if (!rc && hipMemcpyAsync(lnl, dl, (size_t)B * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = EMP_EHIP;
     if (!rc && D && hipMemcpyAsync(grad, dg, (size_t)B * D * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = EMP_EHIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = EMP_EHIP;

@@ -72,6 +72,8 @@ struct EmpModel {
     int wpt_override = 0;
     int device = 0;
     int n_cu = 256;
+    double* d_grad_scratch = nullptr;   // MA adjoint: eps rows [B][N]
+    size_t grad_scratch_cap = 0;
 };
 
 // Host-side preparation of one model: validated, epochs in device order
@@ -105,3 +107,6 @@ int emp_launch_lnl(const EmpModel* m, const double* d_theta, int64_t B, int64_t 
                    double* d_lnl, int precision, int wpt, hipStream_t s);
 int emp_launch_prior(const EmpModel* m, const double* d_theta, int64_t B, int64_t ld,
                      double* d_lnp, hipStream_t s);
+// lnL and its gradient (emp_grad.hip); grows the model's MA scratch if needed.
+int emp_launch_grad(EmpModel* m, const double* d_theta, int64_t B, int64_t ld, double* d_lnl,
+                    double* d_grad, hipStream_t s);

@@ -144,6 +144,21 @@ __device__ __forceinline__ void wave_affine_scan(double& a, double& b, int lane)
     }
 }
 
+// Suffix composition for the adjoint of that recurrence, lambda_i = b_i +
+// a_i lambda_{i+1}: afterwards lane i holds the map of lanes i..63, so that
+// lambda_i = b + a lambda_in with lambda_in the value after lane 63.
+__device__ __forceinline__ void wave_affine_suffix_scan(double& a, double& b, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        double pa = __shfl_down(a, o, 64);
+        double pb = __shfl_down(b, o, 64);
+        if (lane + o < 64) {
+            b = __builtin_fma(a, pb, b);
+            a = a * pa;
+        }
+    }
+}
+
 struct ScanLDS {
     double tot[2][MAX_WAVES][2];
     double red[MAX_WAVES][2];
@@ -153,11 +168,13 @@ struct ScanLDS {
 // call this together (it contains workgroup barriers); `live` == false makes
 // a slot skip the arithmetic while still taking part in the barriers.  The
 // slot must be prepared (slot_prepare) and the sin/cos table loaded.
-// With WRITE the per-epoch model and err2 are stored (device order).
+// With WRITE the per-epoch model and err2 are stored (device order); with
+// WEPS the per-epoch MA-filtered residual eps (= r without MA).
 // `resident`: the caller has already staged all epochs (md.N <= TILE) in tl.
-template <int PREC, int WPT, bool WRITE>
+template <int PREC, int WPT, bool WRITE, bool WEPS = false>
 __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
-                            bool live, int wsub, double* out_model, double* out_err2, bool resident = false) {
+                            bool live, int wsub, double* out_model, double* out_err2, bool resident = false,
+                            double* out_eps = nullptr) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int G = 64 * WPT;
@@ -223,6 +240,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
                     out_model[tb + j] = mu + (r - eps);
                     out_err2[tb + j] = err2;
                 }
+                if (WEPS) out_eps[tb + j] = eps;
             }
         }
     }

@@ -396,6 +396,7 @@ extern "C" void emp_model_destroy(EmpModel* m) {
     for (void* p : m->allocations) (void)hipFree(p);
     if (m->d_stage_in) (void)hipFree(m->d_stage_in);
     if (m->d_stage_out) (void)hipFree(m->d_stage_out);
+    if (m->d_grad_scratch) (void)hipFree(m->d_grad_scratch);
     delete m;
 }
 

@@ -126,7 +126,9 @@ int emp_lnprior_batch(EmpModel* model, const double* theta, int64_t B, int64_t l
 int emp_model_eval(EmpModel* model, const double* theta, double* model_out, double* err2_out);
 
 /* No reference counterpart: lnL and dlnL/dtheta[free]
- * for a batch, by a hand-written adjoint kernel (FP64). grad is [B, ndim]. */
+ * for a batch, by a hand-written adjoint kernel (FP64). grad is [B, ndim].
+ * With an active MA(1) term the model keeps a device scratch of B x n_epochs
+ * doubles (grown on demand, freed with the model). */
 int emp_lnl_grad_batch(EmpModel* model, const double* theta, int64_t B, int64_t ld,
                        double* lnl, double* grad, int32_t flags, void* stream);
 

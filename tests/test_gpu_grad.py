@@ -38,6 +38,20 @@ def _fd(model, th, h_rel=1e-6):
     return plain, (4 * half - plain) / 3
 
 
+def _fd_judged(g, rich, lnl, th, h_rel=1e-6):
+    """Relative FD mismatch over the entries the differences can judge.
+
+    Each lnL carries a rounding error of a few 2^-52 |lnL|, which the central
+    difference divides by h: an entry is judged where 4 * 2^-52 |lnL| / h is at
+    most 1e-7 max(|fd|, 1).  Returns (largest relative mismatch over judged
+    entries, their number)."""
+    h = h_rel * np.maximum(np.abs(th), 1.0)
+    scale = np.maximum(np.abs(rich), 1.0)
+    judged = 4 * 2.0 ** -52 * abs(lnl) / h <= 1e-7 * scale
+    err = np.abs(g - rich) / scale
+    return (float(np.max(err[judged])) if judged.any() else 0.0), int(judged.sum())
+
+
 @pytest.mark.parametrize("p", [0, 1, 2, 3, 4, 6, 7])
 def test_adjoint_vs_autograd_and_fd(p):
     rng = np.random.default_rng(50 + p)
@@ -106,3 +120,39 @@ def test_map_search_51peg_reaches_kat():
     # the max-posterior lnL sits at the recorded KAT (-869.480) within the sampler's spread
     assert o.my_likelihood(best) > -869.55, (o.my_likelihood(best), best)
     assert np.nanmax(post) >= np.nanmax([o.my_prior(s) + o.my_likelihood(s) for s in starts])
+
+
+@pytest.mark.parametrize("case", ["per_instrument", "per_instrument_fixed_tau", "global"])
+def test_adjoint_ma1(case):
+    """MA(1) adjoint (reverse recurrence of the residual filter): against the
+    torch reference's autograd and Richardson-extrapolated central differences,
+    with free timescales, fixed ones (precomputed weights) and the global series."""
+    rng = np.random.default_rng({"per_instrument": 1, "per_instrument_fixed_tau": 2, "global": 3}[case])
+    conds = []
+    if case == "per_instrument_fixed_tau":
+        conds = [(f"MATimescale {i} Order 1", "fixed", 5.0) for i in range(1, 9)]
+    sim = make_sim("GJ876", 1, parameterisation=1, moav={"order": 1, "global": case == "global"},
+                   semantics="intended", conditions=conds)
+    m = sim.model
+    # waves per vector: 1 (no cross-wave carry), 8, and the default (2 at 752 epochs)
+    if case == "global":
+        m.set_waves_per_theta(1)
+    elif case == "per_instrument_fixed_tau":
+        m.set_waves_per_theta(8)
+    th = _draws(m, 4, rng)
+    lnl, g = m.lnl_grad_batch(th)
+    assert np.allclose(lnl, m.lnl_batch(th), rtol=1e-12, atol=0)
+    worst_ad, worst_fd, judged = 0.0, 0.0, 0
+    for b in range(len(th)):
+        t = torch.tensor(th[b], dtype=torch.float64, requires_grad=True)
+        lnl_torch(m, t).backward()
+        ga = t.grad.numpy()
+        worst_ad = max(worst_ad, np.max(np.abs(g[b] - ga) / np.maximum(np.abs(ga), 1.0)))
+        _, rich = _fd(m, th[b])
+        err, n = _fd_judged(g[b], rich, lnl[b], th[b])
+        worst_fd, judged = max(worst_fd, err), judged + n
+    print(f"{case}: adjoint vs autograd {worst_ad:.2e}, vs Richardson FD {worst_fd:.2e} "
+          f"over {judged} of {g.size} entries", flush=True)
+    assert worst_ad < 1e-8
+    assert judged >= 0.4 * g.size
+    assert worst_fd < 1e-6

@@ -40,3 +40,20 @@ def test_torch_reference_gradient_vs_fd():
         b[j] -= h
         fd = (o.my_likelihood(a) - o.my_likelihood(b)) / (2 * h)
         assert abs(fd - g[j]) <= 1e-5 * max(abs(g[j]), 1.0), (j, fd, g[j])
+
+
+@pytest.mark.parametrize("glob", [False, True])
+def test_torch_reference_ma1_equals_oracle(glob):
+    """The MA(1) recurrence of the torch reference against the oracle's
+    (per-instrument series with the intended semantics, or the global one)."""
+    rng = np.random.default_rng(11 + glob)
+    sim = make_sim("GJ876", 1, parameterisation=1, moav={"order": 1, "global": glob}, semantics="intended")
+    m = sim.model
+    o = oracle_for(m)
+    lim = m.limits_free()
+    for _ in range(3):
+        th = lim[:, 0] + (lim[:, 1] - lim[:, 0]) * rng.uniform(0.2, 0.8, size=len(lim))
+        th[3:5] *= 0.5
+        ref = o.my_likelihood(th)
+        got = lnl_torch(m, torch.tensor(th)).item()
+        assert abs(got - ref) <= 1e-10 * abs(ref), (got, ref)

@@ -75,4 +75,23 @@ def lnl_torch(model, theta):
             v = v * x + c
         mu = mu + v * x
     r = Y - mu
+    ma = model.block("MOAV")
+    if ma is not None and (model.semantics == "intended" or ma.is_global):
+        # MA(1) (moav01.model:7-15): eps_i = r_i - c exp(-|t_i - t_prev|/tau) eps_prev
+        # along the time-ordered series (global) or each instrument's series.
+        if ma.moav != 1:
+            raise NotImplementedError("MA order 1 only")
+        s0 = ma.slice.start
+        eps = [None] * len(X)
+        groups = [np.arange(len(X))] if ma.is_global else \
+            [np.where(d.FLAG == i + 1)[0] for i in range(model.nins__)]
+        for gi, idx in enumerate(groups):
+            c, tau = full[s0 + 2 * gi], full[s0 + 2 * gi + 1]
+            prev = None
+            for i in idx:
+                e_i = r[i] if prev is None else \
+                    r[i] - c * torch.exp(-torch.abs(X[i] - X[prev]) / tau) * eps[prev]
+                eps[i] = e_i
+                prev = i
+        r = torch.stack(eps)
     return -0.5 * torch.sum(r ** 2 / err2 + torch.log(err2)) - 0.5 * np.log(2 * np.pi) * len(X)
