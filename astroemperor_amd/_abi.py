@@ -47,6 +47,13 @@ class EmpModelDesc(C.Structure):
                 ("prior", C.POINTER(EmpPriorTerm)), ("ecc_prior", C.POINTER(EmpPriorTerm))]
 
 
+class EmpMoveDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("weight", C.c_double), ("scale", C.c_double)]
+
+
+EMP_MOVE_STRETCH, EMP_MOVE_GRADIENT = 0, 1
+
+
 class EmpPTDesc(C.Structure):
     _fields_ = [("ntemps", C.c_int32), ("nwalkers", C.c_int32), ("nsteps", C.c_int32),
                 ("precision", C.c_int32), ("a", C.c_double), ("seed", C.c_uint64),
@@ -84,6 +91,7 @@ _SIGS = {
     "emp_pt_get": (C.c_int, [_P, C.c_int32, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "emp_pt_set_state": (C.c_int, [_P, _P, _P]),
     "emp_pt_set_adaptation": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
+    "emp_pt_set_moves": (C.c_int, [_P, _P, C.c_int32]),
     "emp_pt_checkpoint": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_restore": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -36,6 +36,15 @@ struct GradLDS {
     double nat[KMAX][5];       // per planet: (M*tau, M, e, w, A) accumulators
 };
 
+// Per-wave partial sums, added over a slot's waves in wave order so that the
+// gradient does not depend on how the waves were scheduled.
+constexpr int WA_OFF = 0, WA_JIT = IMAX, WA_MAC = 2 * IMAX, WA_MAT = 3 * IMAX, WA_N = 4 * IMAX;
+constexpr int WN_ACC = KMAX * 5, WN_N = KMAX * 5 + AMAX;
+struct GradWaveLDS {
+    double acc[NWAVES][WA_N];   // offsets, jitters, MA coefficients, MA timescales (per instrument / set)
+    double nat[NWAVES][WN_N];   // natural-parameter sums per planet, then the acceleration terms
+};
+
 // Model value of one epoch and its planet derivatives (natural parameters).
 struct EpochDeriv {
     double mu;
@@ -114,6 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     constexpr int SLOTS = NWAVES / WPT;
     __shared__ SlotLDS s_slot[SLOTS];
     __shared__ GradLDS s_grad[SLOTS];
+    __shared__ GradWaveLDS s_gw;
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -126,6 +136,9 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     if (live) expand_theta(md, sl, theta + b * ld, g, G);
     for (int j = g; j < MAXD; j += G) gl.g[j] = 0.0;
     for (int j = g; j < KMAX * 5; j += G) (&gl.nat[0][0])[j] = 0.0;
+    double* wacc = s_gw.acc[wave];
+    for (int j = lane; j < WA_N; j += 64) wacc[j] = 0.0;
+    for (int j = lane; j < WN_N; j += 64) s_gw.nat[wave][j] = 0.0;
     tab_load(md, s_tile);
     __syncthreads();
     slot_prepare<0, WPT>(md, sl, s_tile, live, wsub, lane);
@@ -164,8 +177,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 pexp += __builtin_amdgcn_frexp_exp(prod);
                 prod = __builtin_amdgcn_frexp_mant(prod);
                 accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
-                atomicAdd(&gl.g[md.off_off + k], w);
-                if (md.has_jit) atomicAdd(&gl.g[md.off_jit + k], sl.jit[k] * (w * w - ie));
+                atomicAdd(&wacc[WA_OFF + k], w);
+                if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (w * w - ie));
             }
         }
         double ldet = live ? log(prod) + (double)pexp * LN2 : 0.0;
@@ -237,8 +250,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 if (valid) {
                     const double w = -lam;
                     accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
-                    atomicAdd(&gl.g[md.off_off + k], w);
-                    if (md.has_jit) atomicAdd(&gl.g[md.off_jit + k], sl.jit[k] * (e_i * e_i * ie * ie - ie));
+                    atomicAdd(&wacc[WA_OFF + k], w);
+                    if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (e_i * e_i * ie * ie - ie));
                     // da_i/dc = a_i / c, da_i/dtau = a_i dt / tau^2 (a_i = 0 at a segment start)
                     const double col = s_tile.dt[j];
                     const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
@@ -247,10 +260,10 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                         const double e_prev = eps[n - 1];
                         const int set = md.ma_global ? 0 : k;
                         const double wgt = md.ma_wfixed ? col : exp(-col / sl.ma_tau[k]);
-                        atomicAdd(&gl.g[md.off_ma + 2 * set], lam * e_prev * -wgt);
+                        atomicAdd(&wacc[WA_MAC + set], lam * e_prev * -wgt);
                         if (!md.ma_wfixed) {
                             const double tau = sl.ma_tau[k];
-                            atomicAdd(&gl.g[md.off_ma + 2 * set + 1],
+                            atomicAdd(&wacc[WA_MAT + set],
                                       lam * e_prev * -(sl.ma_c[k] * wgt * col / (tau * tau)));
                         }
                     }
@@ -265,14 +278,36 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
         for (int q = 0; q < 5; ++q) {
             if (p < md.K) {
                 const double v = wave_sum(acc_nat[p][q]);
-                if (live && lane == 0) atomicAdd(&gl.nat[p][q], v);
+                if (lane == 0) s_gw.nat[wave][p * 5 + q] = v;
             }
         }
 #pragma unroll
     for (int a = 0; a < AMAX; ++a) {
         if (a < md.acc) {
             const double v = wave_sum(acc_acc[a]);
-            if (live && lane == 0) atomicAdd(&gl.g[md.off_acc + a], v);
+            if (lane == 0) s_gw.nat[wave][WN_ACC + a] = v;
+        }
+    }
+    __syncthreads();
+    if (live && wsub == 0) {
+        const int w0 = wave;
+        for (int e = lane; e < WA_N; e += 64) {
+            double v = 0.0;
+            for (int w = 0; w < WPT; ++w) v += s_gw.acc[w0 + w][e];
+            const int grp = e / IMAX, k = e - grp * IMAX;
+            if (grp == 0 && k < md.I) gl.g[md.off_off + k] = v;
+            else if (grp == 1 && md.has_jit && k < md.I) gl.g[md.off_jit + k] = v;
+            else if (grp >= 2 && md.ma_active && k < (md.ma_global ? 1 : md.I))
+                gl.g[md.off_ma + 2 * k + (grp - 2)] = v;
+        }
+        for (int e = lane; e < WN_N; e += 64) {
+            double v = 0.0;
+            for (int w = 0; w < WPT; ++w) v += s_gw.nat[w0 + w][e];
+            if (e < WN_ACC) {
+                if (e < md.K * 5) (&gl.nat[0][0])[e] = v;
+            } else if (e - WN_ACC < md.acc) {
+                gl.g[md.off_acc + (e - WN_ACC)] = v;
+            }
         }
     }
     __syncthreads();
@@ -353,16 +388,23 @@ static int ensure_grad_scratch(EmpModel* m, int64_t B) {
     return EMP_OK;
 }
 
+int emp_launch_grad_ws(const EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
+                       double* eps_scratch, hipStream_t s) {
+    if (B <= 0) return EMP_OK;
+    if (m->dev.ma_active && !eps_scratch) { emp_set_error("moving-average gradient needs a scratch buffer"); return EMP_EINVAL; }
+    int wpt = emp_pick_wpt(m, B, EMP_FP64);
+    if (wpt == 1) launch_grad_t<1>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
+    else if (wpt == 2) launch_grad_t<2>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
+    else if (wpt == 4) launch_grad_t<4>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
+    else launch_grad_t<8>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
+    EMP_HIP(hipGetLastError());
+    return EMP_OK;
+}
+
 int emp_launch_grad(EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad, hipStream_t s) {
     int rc = ensure_grad_scratch(m, B);
     if (rc) return rc;
-    int wpt = emp_pick_wpt(m, B, EMP_FP64);
-    if (wpt == 1) launch_grad_t<1>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
-    else if (wpt == 2) launch_grad_t<2>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
-    else if (wpt == 4) launch_grad_t<4>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
-    else launch_grad_t<8>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, m->d_grad_scratch, s);
-    EMP_HIP(hipGetLastError());
-    return EMP_OK;
+    return emp_launch_grad_ws(m, th, B, ld, lnl, grad, m->d_grad_scratch, s);
 }
 
 extern "C" int emp_lnl_grad_batch(EmpModel* m, const double* theta, int64_t B, int64_t ld, double* lnl,

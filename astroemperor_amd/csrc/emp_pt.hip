@@ -48,6 +48,18 @@ struct PTDev {
     double* h_betas;    // [S][T]
     double* h_tsw;      // [S][T-1]
     double* h_smd;      // [S][T-1]
+    // Gradient (Langevin) move, allocated when the move is enabled.
+    double lg_eps, lg_e2h;  // step eps and eps^2 / 2
+    double* lg_g;       // [slots][D] grad lnL at each slot's position
+    double* lg_l;       // [slots] lnL from the adjoint kernel at each slot's position
+    double* lg_L1;      // [T][D][D] Cholesky factor L of the complementary half's covariance, L1[k D + i] = L_ik
+    double* lg_L2;      // [T][D][D] the same factor, L2[k D + i] = L_ki
+    int* lg_act;        // [T][D] parameter has a nonzero spread in the complementary half
+    double* lg_Xp;      // [T W/2][D] proposals
+    double* lg_gp;      // [T W/2][D] grad lnL at the proposals
+    double* lg_lp;      // [T W/2] lnL at the proposals
+    double* lg_lq;      // [T W/2] log density of the forward proposal (up to the shared constant)
+    double* lg_scr;     // [T W][N] moving-average adjoint scratch (null without the moving average)
 };
 
 struct EmpPT {
@@ -56,7 +68,9 @@ struct EmpPT {
     PTDev dev;
     std::vector<void*> allocations;
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec = nullptr;      // stretch sweep
+    hipGraphExec_t graph_grad = nullptr;      // gradient-move sweep
+    double w_stretch = 1.0, w_grad = 0.0;     // move weights (one move per sweep)
     int wpt = 1;
     size_t swap_lds = 0;         // dynamic LDS of the swap kernel, 0 = global path
     long long host_sweeps = 0;   // mirror of the device counter
@@ -210,6 +224,332 @@ __global__ void k_identity(int* map, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// Gradient move: ensemble-preconditioned Metropolis-adjusted Langevin
+// (Roberts & Tweedie 1996; the preconditioner is the covariance of the
+// complementary half of the ensemble, Leimkuhler, Matthews & Weare 2018).
+// Walker k of half h at temperature t proposes
+//     y = x + L u,   u = (eps^2 / 2) beta_t L^T g(x) + eps xi,   xi ~ N(0, I),
+// with L L^T the covariance of the other half's walkers at t (population
+// covariance, diagonal times 1 + 1e-6; a parameter without spread there does
+// not move) and g = grad lnL, and accepts with the Metropolis-Hastings ratio of
+// the tempered posterior beta_t lnL + ln prior times q(x | y) / q(y | x).  The
+// lnL values and gradients at both points come from the FP64 adjoint kernel.
+// A non-finite gradient switches the drift off at that point; the drift stays
+// a function of the point alone, so the ratio remains exact.  If the
+// factorisation meets a non-positive pivot, L is the diagonal of standard
+// deviations for that temperature.
+//
+// Random numbers: Philox counters (t W + k, sweep_lo, sweep_hi,
+// 2^29 | (2 step + half) << 8 | r); r < 64 gives the normals of parameters
+// 2r and 2r + 1 (Box-Muller), r = 255 the acceptance uniform.  The move of a
+// sweep is chosen on the host from counter (0, sweep_lo, sweep_hi, 2^28).
+// ---------------------------------------------------------------------------
+constexpr uint32_t LG_TAG = 0x20000000u;
+constexpr uint32_t MOVE_TAG = 0x10000000u;
+constexpr int LG_BLOCK = 256;
+constexpr int LG_CHUNK = 32;
+constexpr int LG_PAIRS = (MAXD * (MAXD + 1) / 2 + LG_BLOCK - 1) / LG_BLOCK;
+constexpr double LG_RIDGE = 1e-6;
+
+__device__ __forceinline__ U4 lg_draw(const PTDev& pd, long long sweep, int half, int step, int t, int k, uint32_t r) {
+    const U4 c{(uint32_t)(t * pd.W + k), (uint32_t)sweep, (uint32_t)((unsigned long long)sweep >> 32),
+               LG_TAG | ((uint32_t)(2 * step + half) << 8) | r};
+    return philox4x32_10(c, pd.key0, pd.key1);
+}
+
+__device__ inline double lg_normal(const PTDev& pd, long long sweep, int half, int step, int t, int k, int j) {
+    const U4 r = lg_draw(pd, sweep, half, step, t, k, (uint32_t)(j >> 1));
+    const double rad = sqrt(-2.0 * log(u01(r.x, r.y)));
+    double sn, cs;
+    sincospi(2.0 * u01(r.z, r.w), &sn, &cs);
+    return (j & 1) ? rad * sn : rad * cs;
+}
+
+// Value of parameter k (uniform across the wave) from the lane that holds it:
+// lane k & 63, register k >> 6.
+__device__ __forceinline__ double lg_bcast(const double v[2], int k) {
+    const double x = k < 64 ? v[0] : v[1];
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)bits, k & 63);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), k & 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// (L^T g)_j for the two parameters j = lane, lane + 64 of this lane.
+__device__ inline void lg_lt_times(const double* L2, const double g[2], int D, int lane, double v[2]) {
+    v[0] = v[1] = 0.0;
+    for (int k = 0; k < D; ++k) {
+        const double gk = lg_bcast(g, k);
+        for (int h = 0; h < 2; ++h) {
+            const int j = lane + 64 * h;
+            if (j < D) v[h] = __builtin_fma(L2[(size_t)k * D + j], gk, v[h]);
+        }
+    }
+}
+
+// Covariance of the complementary half at each temperature and its Cholesky
+// factor; one workgroup per temperature.  Deterministic: every sum runs in a
+// fixed order.  Block 0 of the first half-step of the sweep also counts the
+// likelihoods of the current-position gradient launch.
+__global__ __launch_bounds__(LG_BLOCK) void k_lg_precond(PTDev pd, int half, int step) {
+    __shared__ double s_x[LG_CHUNK][MAXD];
+    __shared__ double s_mean[MAXD], s_diag[MAXD];
+    __shared__ int s_fail;
+    const int t = blockIdx.x, D = pd.D, W2 = pd.W2, tid = threadIdx.x;
+    const int* cmap = pd.map + t * pd.W + (1 - half) * W2;
+    double* A = pd.lg_L2 + (size_t)t * D * D;
+    double* L1 = pd.lg_L1 + (size_t)t * D * D;
+    if (t == 0 && tid == 0 && half == 0 && step == 0)
+        atomicAdd((unsigned long long*)(pd.sweep + 1), (unsigned long long)pd.T * pd.W);
+    // mean
+    double sum = 0.0;
+    for (int w0 = 0; w0 < W2; w0 += LG_CHUNK) {
+        const int nw = min(LG_CHUNK, W2 - w0);
+        __syncthreads();
+        for (int e = tid; e < nw * D; e += LG_BLOCK) {
+            const int w = e / D, j = e - w * D;
+            s_x[w][j] = pd.X[(size_t)cmap[w0 + w] * D + j];
+        }
+        __syncthreads();
+        if (tid < D)
+            for (int w = 0; w < nw; ++w) sum += s_x[w][tid];
+    }
+    if (tid < D) s_mean[tid] = sum / (double)W2;
+    // lower triangle of the covariance: this thread's pairs p = tid + m LG_BLOCK,
+    // p = i (i + 1) / 2 + j, j <= i
+    const int np = D * (D + 1) / 2;
+    double acc[LG_PAIRS];
+    for (int m = 0; m < LG_PAIRS; ++m) acc[m] = 0.0;
+    for (int w0 = 0; w0 < W2; w0 += LG_CHUNK) {
+        const int nw = min(LG_CHUNK, W2 - w0);
+        __syncthreads();
+        for (int e = tid; e < nw * D; e += LG_BLOCK) {
+            const int w = e / D, j = e - w * D;
+            s_x[w][j] = pd.X[(size_t)cmap[w0 + w] * D + j] - s_mean[j];
+        }
+        __syncthreads();
+        int i = 0, j = tid;
+        while (j > i) { j -= i + 1; ++i; }
+#pragma unroll
+        for (int m = 0; m < LG_PAIRS; ++m) {
+            if (tid + m * LG_BLOCK < np) {
+                double a = acc[m];
+                for (int w = 0; w < nw; ++w) a = __builtin_fma(s_x[w][i], s_x[w][j], a);
+                acc[m] = a;
+            }
+            j += LG_BLOCK;
+            while (j > i) { j -= i + 1; ++i; }
+        }
+    }
+    {
+        int i = 0, j = tid;
+        while (j > i) { j -= i + 1; ++i; }
+        for (int m = 0; m < LG_PAIRS; ++m) {
+            if (tid + m * LG_BLOCK < np) {
+                const double c = acc[m] / (double)W2;
+                A[(size_t)i * D + j] = c;
+                if (i == j) s_diag[i] = c;
+            }
+            j += LG_BLOCK;
+            while (j > i) { j -= i + 1; ++i; }
+        }
+    }
+    __syncthreads();
+    // parameters without spread: row and column cleared, unit pivot, inactive
+    if (tid < D) {
+        const bool act = s_diag[tid] > 0.0;
+        pd.lg_act[t * D + tid] = act ? 1 : 0;
+        if (act) {
+            A[(size_t)tid * D + tid] = s_diag[tid] * (1.0 + LG_RIDGE);
+        } else {
+            for (int k = 0; k < tid; ++k) A[(size_t)tid * D + k] = 0.0;
+            for (int i = tid + 1; i < D; ++i) A[(size_t)i * D + tid] = 0.0;
+            A[(size_t)tid * D + tid] = 1.0;
+        }
+    }
+    if (tid == 0) s_fail = 0;
+    // right-looking Cholesky on the lower triangle
+    for (int j = 0; j < D; ++j) {
+        __syncthreads();
+        if (tid == 0) {
+            const double piv = A[(size_t)j * D + j];
+            if (!(piv > 0.0) || !(piv < __builtin_inf())) s_fail = 1;
+            else A[(size_t)j * D + j] = sqrt(piv);
+        }
+        __syncthreads();
+        if (s_fail) break;
+        const double d = A[(size_t)j * D + j];
+        for (int i = j + 1 + tid; i < D; i += LG_BLOCK) A[(size_t)i * D + j] /= d;
+        __syncthreads();
+        const int m = D - j - 1, nq = m * (m + 1) / 2;
+        int ii = 0, kk = tid;
+        while (kk > ii) { kk -= ii + 1; ++ii; }
+        for (int q = tid; q < nq; q += LG_BLOCK) {
+            const int i = j + 1 + ii, k = j + 1 + kk;
+            A[(size_t)i * D + k] = __builtin_fma(-A[(size_t)i * D + j], A[(size_t)k * D + j], A[(size_t)i * D + k]);
+            kk += LG_BLOCK;
+            while (kk > ii) { kk -= ii + 1; ++ii; }
+        }
+    }
+    __syncthreads();
+    const bool fail = s_fail != 0;
+    for (int e = tid; e < D * D; e += LG_BLOCK) {
+        const int i = e / D, k = e - i * D;
+        double v = 0.0;
+        if (fail) {
+            if (i == k) v = s_diag[i] > 0.0 ? sqrt(s_diag[i]) : 1.0;
+        } else if (k <= i) {
+            v = A[e];
+        }
+        A[e] = v;
+        L1[(size_t)k * D + i] = v;
+    }
+}
+
+// Proposal of every walker of the moving half (one wave per walker), its
+// prior, and the row handed to the adjoint kernel: the proposal when its
+// prior is finite, else the current position (a valid point whose result is
+// ignored).
+__global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_propose(ModelDev md, PTDev pd, int half, int step) {
+    __shared__ double s_row[PRIOR_WAVES][MAXD];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x * PRIOR_WAVES + wave;
+    if (b >= pd.T * pd.W2) return;            // whole waves; no workgroup barrier follows
+    const int D = pd.D;
+    const long long sweep = *pd.sweep;
+    const int t = b / pd.W2, k = half * pd.W2 + b % pd.W2;
+    const int sk = pd.map[t * pd.W + k];
+    const double beta = pd.betas[t];
+    const double* x = pd.X + (size_t)sk * D;
+    const double* g = pd.lg_g + (size_t)sk * D;
+    const double* L1 = pd.lg_L1 + (size_t)t * D * D;
+    const double* L2 = pd.lg_L2 + (size_t)t * D * D;
+    const int* act = pd.lg_act + t * D;
+    double xv[2], gv[2];
+    bool fin = true;
+    for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        xv[h] = j < D ? x[j] : 0.0;
+        gv[h] = j < D ? g[j] : 0.0;
+        fin = fin && __builtin_isfinite(gv[h]);
+    }
+    const bool drift = __ballot(!fin) == 0ull;
+    double v[2] = {0.0, 0.0};
+    if (drift) lg_lt_times(L2, gv, D, lane, v);
+    double u[2] = {0.0, 0.0}, lq = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        if (j < D && act[j]) {
+            const double xi = lg_normal(pd, sweep, half, step, t, k, j);
+            u[h] = __builtin_fma(pd.lg_e2h * beta, v[h], pd.lg_eps * xi);
+            lq -= 0.5 * xi * xi;
+        }
+    }
+    lq = wave_sum(lq);
+    double dy[2] = {0.0, 0.0};
+    for (int kk = 0; kk < D; ++kk) {
+        const double uk = lg_bcast(u, kk);
+        for (int h = 0; h < 2; ++h) {
+            const int j = lane + 64 * h;
+            if (j < D) dy[h] = __builtin_fma(L1[(size_t)kk * D + j], uk, dy[h]);
+        }
+    }
+    double y[2];
+    for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        y[h] = xv[h] + dy[h];
+        if (j < D) s_row[wave][j] = y[h];
+    }
+    __threadfence_block();     // the row is read back across lanes
+    const double lp = prior_proposal_group<64>(md, s_row[wave], s_row[wave], 0.0, lane, 0);
+    const bool live = lp > -__builtin_inf();
+    double* yp = pd.lg_Xp + (size_t)b * D;
+    for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        if (j < D) yp[j] = live ? y[h] : xv[h];
+    }
+    if (lane == 0) {
+        pd.prop_lp[b] = lp;
+        pd.lg_lq[b] = lq;
+    }
+}
+
+// Metropolis-Hastings acceptance of the gradient move (one wave per walker).
+__global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_accept(PTDev pd, int half, int step) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x * PRIOR_WAVES + wave;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd((unsigned long long*)(pd.sweep + 1), (unsigned long long)pd.T * pd.W2);
+    if (b >= pd.T * pd.W2) return;
+    const double lp = pd.prop_lp[b];
+    if (!(lp > -__builtin_inf())) return;
+    const int D = pd.D;
+    const long long sweep = *pd.sweep;
+    const int t = b / pd.W2, k = half * pd.W2 + b % pd.W2;
+    const int sk = pd.map[t * pd.W + k];
+    const double beta = pd.betas[t];
+    const double* x = pd.X + (size_t)sk * D;
+    const double* y = pd.lg_Xp + (size_t)b * D;
+    const double* gy = pd.lg_gp + (size_t)b * D;
+    const double* L1 = pd.lg_L1 + (size_t)t * D * D;
+    const double* L2 = pd.lg_L2 + (size_t)t * D * D;
+    const int* act = pd.lg_act + t * D;
+    double yv[2], gv[2], dv[2];
+    bool fin = true;
+    for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        yv[h] = j < D ? y[j] : 0.0;
+        gv[h] = j < D ? gy[j] : 0.0;
+        dv[h] = j < D ? x[j] - yv[h] : 0.0;
+        fin = fin && __builtin_isfinite(gv[h]);
+    }
+    const bool drift = __ballot(!fin) == 0ull;
+    double v[2] = {0.0, 0.0};
+    if (drift) lg_lt_times(L2, gv, D, lane, v);
+    // r = L^-1 (x - y) by forward substitution, one column at a time
+    for (int kk = 0; kk < D; ++kk) {
+        const double rk = lg_bcast(dv, kk) / L1[(size_t)kk * D + kk];
+        for (int h = 0; h < 2; ++h) {
+            const int j = lane + 64 * h;
+            if (j == kk) dv[h] = rk;
+            else if (j > kk && j < D) dv[h] = __builtin_fma(-L1[(size_t)kk * D + j], rk, dv[h]);
+        }
+    }
+    // reverse proposal: xi' = r / eps - (eps / 2) beta L^T g(y)
+    double lq = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int j = lane + 64 * h;
+        if (j < D && act[j]) {
+            const double rho = dv[h] / pd.lg_eps - 0.5 * pd.lg_eps * beta * v[h];
+            lq -= 0.5 * rho * rho;
+        }
+    }
+    lq = wave_sum(lq);
+    double ll = pd.lg_lp[b];
+    if (!(ll == ll) || ll == __builtin_inf()) ll = -__builtin_inf();
+    double lo = pd.lg_l[sk];
+    if (!(lo == lo)) lo = -__builtin_inf();
+    const double tn = beta == 0.0 ? 0.0 : beta * ll;
+    const double to = beta == 0.0 ? 0.0 : beta * lo;
+    const double lnalpha = (tn - to) + (lp - pd.lnp[sk]) + (lq - pd.lg_lq[b]);
+    const U4 r = lg_draw(pd, sweep, half, step, t, k, 255u);
+    if (log(u01(r.x, r.y)) < lnalpha) {
+        double* xs = pd.X + (size_t)sk * D;
+        double* gs = pd.lg_g + (size_t)sk * D;
+        for (int h = 0; h < 2; ++h) {
+            const int j = lane + 64 * h;
+            if (j < D) { xs[j] = yv[h]; gs[j] = gv[h]; }
+        }
+        if (lane == 0) {
+            pd.lnl[sk] = ll;
+            pd.lnp[sk] = lp;
+            pd.lg_l[sk] = ll;
+            pd.accepted[t * pd.W + k] += 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
 template <int PREC, int WPT>
@@ -237,12 +577,9 @@ static int launch_half(EmpPT* p, int half, int step) {
     return EMP_EINVAL;
 }
 
-static int launch_sweep(EmpPT* p) {
-    for (int st = 0; st < p->desc.nsteps; ++st)
-        for (int h = 0; h < 2; ++h) {
-            int rc = launch_half(p, h, st);
-            if (rc) return rc;
-        }
+// Swap round (with ladder and sweep counter) and history record: the end of
+// every sweep, whichever move it made.
+static int launch_tail(EmpPT* p) {
     if (p->swap_lds)
         hipLaunchKernelGGL(k_pt_swap<true>, dim3(1), dim3(SWAP_BLOCK), p->swap_lds, p->stream, p->dev);
     else
@@ -255,6 +592,57 @@ static int launch_sweep(EmpPT* p) {
         EMP_HIP(hipGetLastError());
     }
     return EMP_OK;
+}
+
+static int launch_sweep(EmpPT* p) {
+    for (int st = 0; st < p->desc.nsteps; ++st)
+        for (int h = 0; h < 2; ++h) {
+            int rc = launch_half(p, h, st);
+            if (rc) return rc;
+        }
+    return launch_tail(p);
+}
+
+// A sweep of gradient moves: gradients at every current position, then per
+// half-step the preconditioner, the proposals, their gradients and the
+// acceptance.  Allocates nothing (the scratch is the engine's own), so it
+// can be captured.
+static int launch_grad_sweep(EmpPT* p) {
+    const PTDev& d = p->dev;
+    const ModelDev& md = p->model->dev;
+    const int64_t TW = (int64_t)d.T * d.W, tasks = (int64_t)d.T * d.W2;
+    const unsigned tgrid = (unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES);
+    int rc = emp_launch_grad_ws(p->model, d.X, TW, d.D, d.lg_l, d.lg_g, d.lg_scr, p->stream);
+    if (rc) return rc;
+    for (int st = 0; st < p->desc.nsteps; ++st)
+        for (int h = 0; h < 2; ++h) {
+            hipLaunchKernelGGL(k_lg_precond, dim3((unsigned)d.T), dim3(LG_BLOCK), 0, p->stream, d, h, st);
+            EMP_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_lg_propose, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, md, d, h, st);
+            EMP_HIP(hipGetLastError());
+            rc = emp_launch_grad_ws(p->model, d.lg_Xp, tasks, d.D, d.lg_lp, d.lg_gp, d.lg_scr, p->stream);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_lg_accept, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, d, h, st);
+            EMP_HIP(hipGetLastError());
+        }
+    return launch_tail(p);
+}
+
+// The move of sweep `sweep`: 1 (gradient) with probability w_grad / (w_stretch
+// + w_grad), from the engine's Philox stream; a run stays a function of the
+// seed and resumes bitwise.
+static int pick_move(const EmpPT* p, long long sweep) {
+    if (!(p->w_grad > 0.0)) return 0;
+    if (!(p->w_stretch > 0.0)) return 1;
+    const U4 r = philox4x32_10(U4{0u, (uint32_t)sweep, (uint32_t)((unsigned long long)sweep >> 32), MOVE_TAG},
+                               p->dev.key0, p->dev.key1);
+    return u01(r.x, r.y) * (p->w_stretch + p->w_grad) < p->w_grad ? 1 : 0;
+}
+
+static void drop_graphs(EmpPT* p) {
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph_grad) (void)hipGraphExecDestroy(p->graph_grad);
+    p->graph_exec = p->graph_grad = nullptr;
 }
 
 template <typename T>
@@ -286,7 +674,7 @@ static int eval_state(EmpPT* p) {
 
 extern "C" void emp_pt_destroy(EmpPT* p) {
     if (!p) return;
-    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    drop_graphs(p);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (void* q : p->allocations) (void)hipFree(q);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -303,7 +691,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
         emp_set_error("nwalkers must be even and in [2, %d]", SWAP_BLOCK);
         return EMP_EINVAL;
     }
-    if (d.nsteps < 1) { emp_set_error("nsteps must be >= 1"); return EMP_EINVAL; }
+    if (d.nsteps < 1 || d.nsteps > (1 << 20)) { emp_set_error("nsteps must be in [1, 2^20]"); return EMP_EINVAL; }
     if (d.precision != EMP_FP64 && d.precision != EMP_MIXED) { emp_set_error("bad precision"); return EMP_EINVAL; }
     if (!(d.a > 1.0)) { emp_set_error("stretch scale a must be > 1"); return EMP_EINVAL; }
     if (d.adapt_mode != -1 && d.adapt_mode != 0) { emp_set_error("adapt_mode must be -1 (frozen) or 0"); return EMP_EUNSUP; }
@@ -375,21 +763,25 @@ extern "C" int emp_pt_run(EmpPT* p, int64_t nsweeps, int32_t use_graph) {
     EMP_HIP(hipEventCreate(&e1));
     int rc = EMP_OK;
     EMP_HIP(hipEventRecord(e0, p->stream));
-    if (use_graph) {
-        if (!p->graph_exec) {
-            hipGraph_t g;
-            EMP_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            rc = launch_sweep(p);
-            hipError_t ce = hipStreamEndCapture(p->stream, &g);
+    for (int64_t s = 0; s < nsweeps; ++s) {
+        const int mv = pick_move(p, p->host_sweeps + s);
+        if (use_graph) {
+            hipGraphExec_t& ge = mv ? p->graph_grad : p->graph_exec;
+            if (!ge) {
+                hipGraph_t g;
+                EMP_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+                rc = mv ? launch_grad_sweep(p) : launch_sweep(p);
+                hipError_t ce = hipStreamEndCapture(p->stream, &g);
+                if (rc) return rc;
+                EMP_HIP(ce);
+                EMP_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                EMP_HIP(hipGraphDestroy(g));
+            }
+            EMP_HIP(hipGraphLaunch(ge, p->stream));
+        } else {
+            rc = mv ? launch_grad_sweep(p) : launch_sweep(p);
             if (rc) return rc;
-            EMP_HIP(ce);
-            EMP_HIP(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
-            EMP_HIP(hipGraphDestroy(g));
         }
-        for (int64_t s = 0; s < nsweeps; ++s) EMP_HIP(hipGraphLaunch(p->graph_exec, p->stream));
-    } else {
-        for (int64_t s = 0; s < nsweeps && !rc; ++s) rc = launch_sweep(p);
-        if (rc) return rc;
     }
     EMP_HIP(hipEventRecord(e1, p->stream));
     EMP_HIP(hipEventSynchronize(e1));
@@ -467,10 +859,7 @@ extern "C" int emp_pt_autotune(EmpPT* p, int32_t trials, int32_t* best_wpt, doub
     release();
     p->wpt = rc ? keep : best;
     if (rc) return rc;
-    if (p->graph_exec) {
-        EMP_HIP(hipGraphExecDestroy(p->graph_exec));
-        p->graph_exec = nullptr;
-    }
+    drop_graphs(p);
     if (best_wpt) *best_wpt = best;
     if (best_ms) *best_ms = best_t;
     return EMP_OK;
@@ -557,10 +946,53 @@ extern "C" int emp_pt_set_adaptation(EmpPT* p, int32_t mode, double tau, double 
     p->dev.adapt_mode = mode;
     p->dev.adapt_tau = tau;
     p->dev.adapt_nu = nu;
-    if (p->graph_exec) {   // kernel arguments changed: capture again on the next run
-        EMP_HIP(hipGraphExecDestroy(p->graph_exec));
-        p->graph_exec = nullptr;
+    drop_graphs(p);        // kernel arguments changed: capture again on the next run
+    return EMP_OK;
+}
+
+extern "C" int emp_pt_set_moves(EmpPT* p, const EmpMoveDesc* moves, int32_t n) {
+    if (!p || (n > 0 && !moves)) { emp_set_error("null argument"); return EMP_EINVAL; }
+    if (n < 1 || n > 2) { emp_set_error("between one and two moves"); return EMP_EINVAL; }
+    double ws = 0.0, wg = 0.0, a = p->dev.a, eps = 0.0;
+    bool seen[2] = {false, false};
+    for (int i = 0; i < n; ++i) {
+        const EmpMoveDesc& m = moves[i];
+        if (m.kind != EMP_MOVE_STRETCH && m.kind != EMP_MOVE_GRADIENT) { emp_set_error("unknown move kind %d", m.kind); return EMP_EINVAL; }
+        if (seen[m.kind]) { emp_set_error("move kind %d given twice", m.kind); return EMP_EINVAL; }
+        seen[m.kind] = true;
+        if (!(m.weight >= 0.0 && m.weight < __builtin_inf())) { emp_set_error("move weights must be finite and >= 0"); return EMP_EINVAL; }
+        if (m.kind == EMP_MOVE_STRETCH) {
+            if (!(m.scale > 1.0 && m.scale < __builtin_inf())) { emp_set_error("stretch scale a must be > 1"); return EMP_EINVAL; }
+            ws = m.weight;
+            a = m.scale;
+        } else {
+            if (!(m.scale > 0.0 && m.scale < __builtin_inf())) { emp_set_error("gradient step must be > 0"); return EMP_EINVAL; }
+            wg = m.weight;
+            eps = m.scale;
+        }
     }
+    if (!(ws + wg > 0.0)) { emp_set_error("move weights sum to zero"); return EMP_EINVAL; }
+    PTDev& v = p->dev;
+    EMP_HIP(hipStreamSynchronize(p->stream));
+    if (wg > 0.0 && !v.lg_g) {
+        const size_t TW = (size_t)v.T * v.W, T2 = TW / 2, D = (size_t)v.D;
+        const EmpModel* m = p->model;
+        int rc = 0;
+        if ((rc = alloc(p, &v.lg_g, TW * D)) || (rc = alloc(p, &v.lg_l, TW)) ||
+            (rc = alloc(p, &v.lg_L1, (size_t)v.T * D * D)) || (rc = alloc(p, &v.lg_L2, (size_t)v.T * D * D)) ||
+            (rc = alloc(p, &v.lg_act, (size_t)v.T * D)) || (rc = alloc(p, &v.lg_Xp, T2 * D)) ||
+            (rc = alloc(p, &v.lg_gp, T2 * D)) || (rc = alloc(p, &v.lg_lp, T2)) || (rc = alloc(p, &v.lg_lq, T2)) ||
+            (m->dev.ma_active && (rc = alloc(p, &v.lg_scr, TW * (size_t)m->dev.N)))) {
+            v.lg_g = nullptr;     // the allocations stay with the engine; a later call retries
+            return rc;
+        }
+    }
+    v.a = a;
+    v.lg_eps = eps;
+    v.lg_e2h = 0.5 * eps * eps;
+    p->w_stretch = ws;
+    p->w_grad = wg;
+    drop_graphs(p);
     return EMP_OK;
 }
 

@@ -26,6 +26,7 @@ import os
 import numpy as np
 
 from . import _abi
+from .moves import move_descs, normalise_moves
 
 
 def default_ladder(ntemps, ndim):
@@ -81,11 +82,13 @@ class PTSampler:
     ``my_prior`` callables of one ReddModel (bound methods), or the model
     itself may be passed as ``log_likelihood_fn``.  ``pool`` is accepted and
     ignored (the device replaces the process pool, support/pools/01.pool).
+    ``moves`` takes emcee's forms (see :mod:`astroemperor_amd.moves`); the
+    default is the stretch move with scale ``a``.
     """
 
     def __init__(self, nwalkers, ndim, log_likelihood_fn, log_prior_fn=None, ntemps=None, pool=None,
                  backend=None, betas=None, tsw_history=True, smd_history=True, adapt_tau=1000,
-                 adapt_nu=1, adapt_mode=0, precision="fp64", seed=None, a=2.0, record=True):
+                 adapt_nu=1, adapt_mode=0, precision="fp64", seed=None, a=2.0, record=True, moves=None):
         from .model import ReddModel
         model = log_likelihood_fn if isinstance(log_likelihood_fn, ReddModel) else \
             getattr(log_likelihood_fn, "__self__", None)
@@ -114,6 +117,7 @@ class PTSampler:
         self.adapt_mode = 0 if adapt_mode in (0, None) else -1
         self.precision = precision
         self.a = float(a)
+        self.moves = normalise_moves(moves, self.a)
         self.seed = int(seed) if seed is not None else int.from_bytes(os.urandom(8), "little")
         self.record = bool(record)
         self.backend = backend
@@ -144,6 +148,8 @@ class PTSampler:
         self._pt = h
         self._capacity = capacity
         self._nsteps = nsteps
+        descs = move_descs(self.moves, self.ndim)
+        _abi.check(L.emp_pt_set_moves(h, C.addressof(descs), len(descs)))
 
     def close(self):
         if self._pt:
@@ -176,6 +182,13 @@ class PTSampler:
         _abi.check(_abi.lib().emp_pt_run(self._pt, int(nsweeps), int(bool(use_graph))))
         self._iterations += int(nsweeps)
         return self.get_last_sample()
+
+    def set_moves(self, moves):
+        """Replace the move mix (emcee's forms); takes effect from the next sweep."""
+        self.moves = normalise_moves(moves, self.a)
+        if self._pt is not None:
+            descs = move_descs(self.moves, self.ndim)
+            _abi.check(_abi.lib().emp_pt_set_moves(self._pt, C.addressof(descs), len(descs)))
 
     def select_adjustment(self, code):
         """reddemcee's select_adjustment('00') freezes the ladder

@@ -163,7 +163,7 @@ typedef struct EmpPTDesc {
 #define EMP_PT_BETAS 3      /* [S, T] ladder after each sweep */
 #define EMP_PT_TSW 4        /* [S, T-1] swap acceptance per adjacent pair per sweep */
 #define EMP_PT_SMD 5        /* [S, T-1] mean |log swap ratio| per pair per sweep */
-#define EMP_PT_ACCEPTED 6   /* [T, W] accepted stretch moves (int64) */
+#define EMP_PT_ACCEPTED 6   /* [T, W] accepted moves (int64) */
 #define EMP_PT_STATE_X 7    /* [T, W, D] current positions */
 #define EMP_PT_STATE_LNL 8  /* [T, W] */
 #define EMP_PT_STATE_LNP 9  /* [T, W] */
@@ -179,7 +179,7 @@ int emp_pt_create(EmpModel* model, const EmpPTDesc* desc, const double* betas,
 void emp_pt_destroy(EmpPT* pt);
 
 /* Replaces: sampler.run_mcmc(p1, nsweeps, nsteps) (support/endit_reddemcee.scr:3).
- * Each sweep = nsteps red/blue stretch steps at every temperature, one round
+ * Each sweep = nsteps red/blue steps of one move (emp_pt_set_moves) at every temperature, one round
  * of adjacent swaps (hottest pair first), an optional ladder update and the
  * history record.  use_graph: capture one sweep into a hipGraph and replay it.
  * Enqueued on the engine's own stream; returns after completion. */
@@ -198,6 +198,30 @@ int emp_pt_set_state(EmpPT* pt, const double* p, const double* betas);
 /* Replaces: sampler.select_adjustment('00') (support/endit_freeze1.scr:11),
  * which freezes the ladder; mode -1 = frozen, 0 = Vousden et al. 2016. */
 int emp_pt_set_adaptation(EmpPT* pt, int32_t adapt_mode, double adapt_tau, double adapt_nu);
+
+/* Moves a sweep draws from (emcee's `moves=[(move, weight), ...]`, which
+ * reddemcee.PTSampler accepts and emp/emp.py:2373 leaves at None, i.e. the
+ * stretch move alone).  Each sweep makes one kind of move, chosen with
+ * probability weight / sum(weights) from the engine's random stream.
+ *   EMP_MOVE_STRETCH:  scale = a (> 1), Goodman & Weare 2010.
+ *   EMP_MOVE_GRADIENT: scale = Langevin step eps (> 0); ensemble-
+ *     preconditioned Metropolis-adjusted Langevin (no reference counterpart):
+ *     y = x + L ((eps^2/2) beta L^T grad lnL(x) + eps xi), L L^T the
+ *     covariance of the other half of the walkers at that temperature, exact
+ *     Metropolis-Hastings acceptance; lnL and gradients in FP64 (adjoint
+ *     kernel).  Needs 2 x T x W x (D + 1) + T x D x (2 D + 1) doubles of
+ *     device memory, plus T x W x N with the moving average. */
+#define EMP_MOVE_STRETCH 0
+#define EMP_MOVE_GRADIENT 1
+typedef struct EmpMoveDesc {
+    int32_t kind;               /* EMP_MOVE_* */
+    double weight;              /* >= 0; the weights need not sum to one */
+    double scale;               /* a for the stretch move, eps for the gradient move */
+} EmpMoveDesc;
+
+/* One or two moves, each kind at most once (default: stretch alone, weight 1,
+ * a from EmpPTDesc). */
+int emp_pt_set_moves(EmpPT* pt, const EmpMoveDesc* moves, int32_t n);
 
 /* No reference counterpart (the reference cannot resume a run): binary
  * checkpoint of the full engine state, bitwise-resumable. */

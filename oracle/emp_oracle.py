@@ -26,6 +26,11 @@ Pinning (see the tests that exercise each function):
 * ``ma_residuals``             -- a literal scalar loop of moav01 on tiny N.
 * ``stretch_step`` / ``swap_step`` -- injected-random unit tests and the
                                   tempered-Gaussian moment tests.
+* ``langevin_factor`` / ``langevin_logq`` / ``langevin_halfstep`` -- the
+                                  proposal density against scipy's multivariate
+                                  normal, detailed balance on random pairs, and
+                                  the stationary moments of correlated Gaussian
+                                  targets (tests/test_oracle_langevin.py).
 """
 from __future__ import annotations
 
@@ -488,3 +493,85 @@ def ladder_update(betas, swap_rates, iteration, adapt_tau, adapt_nu):
     newT = temps[0] + np.cumsum(dT)
     betas[1:-1] = 1.0 / newT
     return betas
+
+
+# ---------------------------------------------------------------------------
+# Gradient move with injected randoms.  No reference counterpart: EMPEROR
+# leaves reddemcee's `moves` at None (emp/emp.py:2373), so this is the
+# published method itself -- Metropolis-adjusted Langevin (Roberts & Tweedie
+# 1996) preconditioned by the covariance of the complementary half of the
+# ensemble (Leimkuhler, Matthews & Weare 2018).
+# ---------------------------------------------------------------------------
+def langevin_factor(C, ridge=1e-6):
+    """Preconditioner from the complementary walkers C[n, D]: the Cholesky
+    factor L of their population covariance with the diagonal times 1 + ridge.
+    Parameters without spread (zero variance) get a unit pivot, no coupling,
+    and are inactive (they do not move).  A factorisation that fails falls back
+    to the diagonal of standard deviations.  Returns (L, active mask)."""
+    C = np.asarray(C, dtype=np.float64)
+    mu = C.mean(0)
+    Z = C - mu
+    S = Z.T @ Z / len(C)
+    var = np.diag(S).copy()
+    act = var > 0
+    A = S.copy()
+    A[~act, :] = 0.0
+    A[:, ~act] = 0.0
+    A[np.diag_indices_from(A)] = np.where(act, var * (1.0 + ridge), 1.0)
+    try:
+        L = np.linalg.cholesky(A)
+    except np.linalg.LinAlgError:
+        L = np.diag(np.where(act, np.sqrt(np.where(act, var, 1.0)), 1.0))
+    return L, act
+
+
+def langevin_logq(x, y, gx, L, act, beta, eps):
+    """log q(y | x) up to the constant shared by both directions: y - x =
+    L((eps^2/2) beta L^T g(x) + eps xi) gives -|xi|^2 / 2 over the active
+    parameters; a non-finite g(x) switches the drift off."""
+    drift = np.all(np.isfinite(gx))
+    v = L.T @ gx if drift else np.zeros_like(x)
+    r = np.linalg.solve(L, y - x)
+    xi = np.where(act, r / eps - 0.5 * eps * beta * v, 0.0)
+    return -0.5 * float(np.sum(xi ** 2))
+
+
+def langevin_halfstep(X, lnl, lnp, grad, beta, active, complement, xi, u_acc, logpost_fn, eps):
+    """One half of the gradient move at one temperature.
+
+    X[W, D], lnl[W], lnp[W] and grad[W, D] (grad lnL at X) are updated in
+    place for the walkers in `active`; `complement` lists the other half.
+    xi[n, D] are the normals and u_acc[n] the uniform that the sampler drew for
+    active walker n.  logpost_fn.prior / .like / .grad give ln prior, lnL and
+    grad lnL.  The proposal is y = x + L u, u = (eps^2/2) beta L^T g(x) + eps xi,
+    accepted with probability min(1, exp(beta (lnL(y) - lnL(x)) + ln pi(y) -
+    ln pi(x) + log q(x | y) - log q(y | x))).  Returns the accepted mask."""
+    L, act = langevin_factor(X[complement])
+    acc = np.zeros(len(active), dtype=bool)
+    for n, k in enumerate(active):
+        x = X[k].copy()
+        g = grad[k]
+        drift = np.all(np.isfinite(g))
+        v = L.T @ g if drift else np.zeros_like(x)
+        u = np.where(act, 0.5 * eps * eps * beta * v + eps * xi[n], 0.0)
+        y = x + L @ u
+        lq_f = -0.5 * float(np.sum(np.where(act, xi[n], 0.0) ** 2))
+        lp_new = logpost_fn.prior(y)
+        if not lp_new > -np.inf:
+            continue
+        ll_new = logpost_fn.like(y)
+        if not (ll_new == ll_new) or ll_new == np.inf:
+            ll_new = -np.inf
+        g_new = np.asarray(logpost_fn.grad(y), dtype=np.float64)
+        lq_r = langevin_logq(y, x, g_new, L, act, beta, eps)
+        tn = 0.0 if beta == 0 else beta * ll_new
+        to = 0.0 if beta == 0 else beta * lnl[k]
+        with np.errstate(invalid="ignore"):
+            lnalpha = (tn - to) + (lp_new - lnp[k]) + (lq_r - lq_f)
+        if np.log(u_acc[n]) < lnalpha:
+            X[k] = y
+            lnl[k] = ll_new
+            lnp[k] = lp_new
+            grad[k] = g_new
+            acc[n] = True
+    return acc
