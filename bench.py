@@ -284,19 +284,23 @@ def run_c4(seed, batch=65536, reps=5, map_starts=1024):
 
 def run_c5(seed, steps, warmup):
     """Large-scale mixed: 7 Keplerians, N = 10000 over 5 instruments with
-    MA(1), 16 temps x 1024 walkers; a shortened timed run."""
-    from astroemperor_amd import PTSampler, synth
+    MA(1), 16 temps x 1024 walkers, gradient sweeps mixed 1:4 with stretch
+    sweeps; a shortened timed run.  Each sweep kind is also timed alone."""
+    from astroemperor_amd import GradientMove, PTSampler, StretchMove, synth
     from astroemperor_amd.simulation import test_init
     work = synth.c5()
     m = work["model"]
     T, W, _, _ = work["setup"]
     p0 = test_init(m, T, W, np.random.default_rng(seed))
+    mix = [(StretchMove(), 4.0), (GradientMove(), 1.0)]
     out = {"workload": f"7 Keplerians, 5 instruments, {m.data.ndat} epochs, MA(1), {T} temps x {W} walkers, "
-                       f"{steps} timed sweeps (stretch moves only)"}
+                       f"{steps} timed sweeps, gradient (Langevin) sweeps mixed 1:4 with stretch sweeps",
+           "gradient_step": GradientMove().scale(m.ndim__)}
     last = None
     for prec in ("mixed", "fp64"):
-        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec, record=False)
-        s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=warmup + steps + 1)
+        extra = 2 * 6
+        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec, record=False, moves=mix)
+        s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=warmup + steps + extra + 1)
         wpt, _ = s.autotune(3)
         n0 = s.n_lnl_evaluated
         t0 = time.perf_counter()
@@ -307,8 +311,18 @@ def run_c5(seed, steps, warmup):
         n2 = s.n_lnl_evaluated
         flops = flops_per_theta(m, prec) * (n2 - n1) / 6
         ach = flops / (h * 1e-3) / 1e12
+        kinds = {}
+        for name, mv in (("stretch", [(StretchMove(), 1.0)]), ("gradient", [(GradientMove(), 1.0)])):
+            s.set_moves(mv)
+            s.run_mcmc(None, nsweeps=1)            # captures this kind's graph
+            a1 = s._get(6, np.int64).sum()
+            s.run_mcmc(None, nsweeps=5)
+            kinds[name] = {"ms_per_sweep": round(s.last_timing()[0], 3),
+                           "acceptance": round(float(s._get(6, np.int64).sum() - a1) / (5 * T * W), 4)}
+        s.set_moves(mix)
         out[prec] = {"sweeps_per_s": round(steps / wall, 2), "lnl_evals_per_s": round(steps * T * W / wall, 1),
                      "lnl_evaluated_per_s": round((n1 - n0) / wall, 1), "waves_per_walker": wpt,
+                     "by_move": kinds,
                      "roofline": {"bound": "valu", "kernel": f"k_pt_half<{prec}>", "achieved": round(ach, 3),
                                   "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4)}}
         if prec == "mixed":
@@ -382,7 +396,7 @@ def main():
     ap.add_argument("--systems-warmup", type=int, default=5)
     ap.add_argument("--no-configs", action="store_true", help="skip the other workloads (c1, c4, c5)")
     ap.add_argument("--c5-steps", type=int, default=20)
-    ap.add_argument("--only", choices=["all", "c2", "systems", "configs", "lnl", "copy"], default="all",
+    ap.add_argument("--only", choices=["all", "c2", "systems", "configs", "c5", "lnl", "copy"], default="all",
                     help="run one leg only (profiling); 'lnl': batched lnL alone on 20480 prior draws; "
                          "'copy': the HBM-counter calibration copy")
     args = ap.parse_args()
@@ -403,6 +417,9 @@ def main():
     if args.only == "systems":
         print(json.dumps({"batched_systems": run_systems(args.systems, args.systems_steps, args.systems_warmup,
                                                          args.seed), "device": info}), flush=True)
+        return
+    if args.only == "c5":
+        print(json.dumps({"c5": run_c5(args.seed, args.c5_steps, 3), "device": info}), flush=True)
         return
     cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
     if args.only == "configs":
