@@ -34,10 +34,38 @@ constexpr double MK_A1 = 1.6 * PI_D / (PI_D * PI_D - 6.0);
 // ----------------------------------------------------------------------------
 // Wave helpers (wave64).
 // ----------------------------------------------------------------------------
+// Cross-lane moves by DPP (register to register, no LDS round trip).
+// update_dpp keeps `old` in lanes whose source is outside the row or whose
+// row is masked off.  Controls: row_shr:n = 0x110 + n (within rows of 16
+// lanes), row_bcast:15 = 0x142 (lane 15 of each row to the next row),
+// row_bcast:31 = 0x143 (lane 31 to rows 2 and 3).  Callers run with every
+// lane of the wave active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double old, double src) {
+    const long long s = __double_as_longlong(src), o = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)s, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(s >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Value of lane k (uniform k) in every lane.
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bits, k);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the wave (inclusive prefix sum by DPP, then lane 63's total),
+// the same value in every lane.  Whole wave active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_f64<0x111, 0xf>(0.0, v);
+    v += dpp_f64<0x112, 0xf>(0.0, v);
+    v += dpp_f64<0x114, 0xf>(0.0, v);
+    v += dpp_f64<0x118, 0xf>(0.0, v);
+    v += dpp_f64<0x142, 0xa>(0.0, v);
+    v += dpp_f64<0x143, 0xc>(0.0, v);
+    return readlane_f64(v, 63);
 }
 
 __device__ __forceinline__ int wave_sum_int(int v) {

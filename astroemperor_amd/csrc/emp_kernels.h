@@ -131,17 +131,25 @@ __device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl
 }
 
 // Affine map composition for the MA(1) recurrence eps_i = a_i eps_{i-1} + b_i:
-// applying (a1, b1) then (a2, b2) gives (a2 a1, a2 b1 + b2).
+// applying (a1, b1) then (a2, b2) gives (a2 a1, a2 b1 + b2).  Inclusive scan
+// over the wave by DPP: within rows of 16 lanes, then across rows; lanes
+// without a source compose with the identity (1, 0).  Whole wave active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void affine_step(double& a, double& b) {
+    const double pa = dpp_f64<CTRL, ROW_MASK>(1.0, a);
+    const double pb = dpp_f64<CTRL, ROW_MASK>(0.0, b);
+    b = __builtin_fma(a, pb, b);
+    a = a * pa;
+}
+
 __device__ __forceinline__ void wave_affine_scan(double& a, double& b, int lane) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        double pa = __shfl_up(a, o, 64);
-        double pb = __shfl_up(b, o, 64);
-        if (lane >= o) {
-            b = __builtin_fma(a, pb, b);
-            a = a * pa;
-        }
-    }
+    (void)lane;
+    affine_step<0x111, 0xf>(a, b);
+    affine_step<0x112, 0xf>(a, b);
+    affine_step<0x114, 0xf>(a, b);
+    affine_step<0x118, 0xf>(a, b);
+    affine_step<0x142, 0xa>(a, b);
+    affine_step<0x143, 0xc>(a, b);
 }
 
 // Suffix composition for the adjoint of that recurrence, lambda_i = b_i +
@@ -213,7 +221,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
                 wave_affine_scan(a, b, lane);
                 if (WPT == 1) {
                     eps = __builtin_fma(a, carry, b);
-                    carry = __shfl(eps, 63, 64);
+                    carry = readlane_f64(eps, 63);
                 } else {
                     if (lane == 63) { sc.tot[parity][wave][0] = a; sc.tot[parity][wave][1] = b; }
                     __syncthreads();
