@@ -39,9 +39,9 @@ struct PTDev {
     double* betas;      // [T]
     long long* accepted;  // [T][W] (logical positions)
     long long* sweep;     // [2] sweeps completed, likelihoods evaluated
-    double* prop_lp;      // [T W/2] prior of each task's proposal (current half-step)
-    int* list;            // [T W/2] tasks whose proposal has a finite prior
-    int* nlive;           // [2] list lengths of the two halves
+    double* prop_lp;      // [T W/2] prior of each task's proposal (gradient move)
+    TaskOps* rec;         // [T W/2] acceptance operands of each task's proposal (lp = -inf: outside the support)
+    double* prow;         // [T W/2][D] the proposals inside the prior support
     double* h_chain;    // [S][T][W][D]
     double* h_lnl;      // [S][T][W]
     double* h_lnp;      // [S][T][W]
@@ -83,50 +83,49 @@ static __host__ __device__ inline PTView view_of(const PTDev& pd) {
 }
 
 // ---------------------------------------------------------------------------
-// Half-step, part 1: the prior of every proposal (one wave per walker of the
-// moving half, its lanes on the prior's terms) and the list of those inside
-// the prior support.  Each workgroup reserves its part of the list with one
-// atomic (the list order varies from run to run; what each task computes
-// does not).  The other half's counter is cleared here: nothing reads it
-// until that half's own part 1 has run again.
+// Half-step, part 1: every proposal of the moving half (one wave per walker,
+// its lanes on the prior's terms), its prior, and its record: the acceptance
+// operands that do not depend on the likelihood (ln prior -inf marks a
+// proposal outside the prior support) and, inside the support, the proposal
+// row.  The loads for the operands are issued with the rows they sit beside.
 // ---------------------------------------------------------------------------
 constexpr int PRIOR_BLOCK = 256;
 constexpr int PRIOR_WAVES = PRIOR_BLOCK / 64;
 
 __global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd, int half, int step) {
-    __shared__ int s_f[PRIOR_WAVES];
-    __shared__ int s_base;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.x * PRIOR_WAVES + wave;
+    if (b >= pd.T * pd.W2) return;            // whole waves
     const PTView pv = view_of(pd);
-    int f = 0;
-    if (b < pd.T * pd.W2) {
-        const Proposal pr = pt_propose(pv, *pd.sweep, half, step, b);
-        const double lp = prior_proposal_group<64>(md, pd.X + (size_t)pr.sk * pd.D, pd.X + (size_t)pr.sj * pd.D,
-                                                   pr.z, lane, 0);
-        if (lane == 0) pd.prop_lp[b] = lp;
-        f = lp > -__builtin_inf();
+    const Proposal pr = pt_propose(pv, *pd.sweep, half, step, b);
+    const double* Xk = pd.X + (size_t)pr.sk * pd.D;
+    const double* Xj = pd.X + (size_t)pr.sj * pd.D;
+    const double beta = pd.betas[pr.t], lnl_old = pd.lnl[pr.sk], lnp_old = pd.lnp[pr.sk];
+    const double lp = prior_proposal_group<64>(md, Xk, Xj, pr.z, lane, 0);
+    const bool live = lp > -__builtin_inf();
+    if (live) {
+        double* row = pd.prow + (size_t)b * pd.D;
+        for (int d = lane; d < pd.D; d += 64) row[d] = pt_prop_value(Xk, Xj, pr.z, d);
     }
-    if (lane == 0) s_f[wave] = f;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int tot = 0;
-        for (int w = 0; w < PRIOR_WAVES; ++w) tot += s_f[w];
-        s_base = tot ? atomicAdd(&pd.nlive[half], tot) : 0;
-        if (blockIdx.x == 0) pd.nlive[1 - half] = 0;
-    }
-    __syncthreads();
-    if (lane == 0 && f) {
-        int off = s_base;
-        for (int w = 0; w < wave; ++w) off += s_f[w];
-        pd.list[off] = b;
+    if (lane == 0) {
+        TaskOps o;
+        o.t = pr.t;
+        o.k = pr.k;
+        o.sk = pr.sk;
+        o.lp = lp;
+        o.beta = beta;
+        o.lz = (double)(pd.D - 1) * log(pr.z);
+        o.dp = lp - lnp_old;
+        o.to = beta == 0.0 ? 0.0 : beta * lnl_old;
+        o.logu = log(pr.uacc);
+        pd.rec[b] = o;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Half-step, part 2: likelihood and acceptance of the listed tasks, one slot
-// of the workgroup per task.  The grid covers every task; workgroups past
-// the list's end leave at once.
+// Half-step, part 2: likelihood and acceptance of the proposals inside the
+// prior support, one slot of the workgroup per task; a workgroup whose tasks
+// all lie outside leaves at once.
 // ---------------------------------------------------------------------------
 template <int PREC, int WPT>
 __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int half, int step) {
@@ -135,17 +134,32 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     __shared__ int s_acc[SLOTS];
-    const int n = pd.nlive[half];
-    if ((int)blockIdx.x * SLOTS >= n) return;
-    const int wave = threadIdx.x >> 6;
+    __shared__ TaskOps s_ops[SLOTS];
+    (void)half;
+    (void)step;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int slot = wave / WPT, wsub = wave % WPT;
-    const int q = blockIdx.x * SLOTS + slot;
-    const int b = q < n ? pd.list[q] : -1;
+    const int g = wsub * 64 + lane, G = 64 * WPT;
+    const int b = blockIdx.x * SLOTS + slot;
+    const bool live = b < pd.T * pd.W2 && pd.rec[b].lp > -__builtin_inf();
+    if (SLOTS == 1) {
+        if (!live) return;                   // the whole workgroup
+    } else if (!__syncthreads_or(live)) {
+        return;
+    }
     const PTView pv = view_of(pd);
-    tab_load(md, s_tile);      // the task's first barrier publishes it
-    pt_half_task<PREC, WPT>(md, pv, *pd.sweep, half, step, b, b >= 0 ? pd.prop_lp[b] : -__builtin_inf(),
-                            s_slot[slot], s_tile, s_scan, s_acc, slot, wsub, false,
-                            (unsigned long long*)(pd.sweep + 1));
+    SlotLDS& sl = s_slot[slot];
+    tab_load(md, s_tile);      // published by pt_eval_accept's first barrier
+    if (live) {
+        if (wsub == 0 && lane == 0) s_ops[slot] = pd.rec[b];
+        const double* row = pd.prow + (size_t)b * pd.D;
+        for (int j = g; j < md.Dfull; j += G) {
+            const int fi = md.inv_free[j];
+            sl.theta[j] = fi >= 0 ? row[fi] : md.fixed_val[j];
+        }
+    }
+    pt_eval_accept<PREC, WPT>(md, pv, s_ops, live, sl, s_tile, s_scan, s_acc, slot, wsub, false,
+                              (unsigned long long*)(pd.sweep + 1));
 }
 
 // ---------------------------------------------------------------------------
@@ -667,7 +681,6 @@ static int eval_state(EmpPT* p) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_mask_lnl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, d.lnl, d.lnp, n);
     EMP_HIP(hipGetLastError());
-    EMP_HIP(hipMemsetAsync(d.nlive, 0, 2 * sizeof(int), p->stream));
     EMP_HIP(hipStreamSynchronize(p->stream));
     return EMP_OK;
 }
@@ -716,7 +729,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
         (rc = alloc(p, &v.betas, (size_t)v.T)) || (rc = alloc(p, &v.accepted, TW)) || (rc = alloc(p, &v.sweep, 2)) ||
         (rc = alloc(p, &v.h_betas, S * v.T)) || (rc = alloc(p, &v.h_tsw, S * std::max(v.T - 1, 1))) ||
         (rc = alloc(p, &v.h_smd, S * std::max(v.T - 1, 1))) || (rc = alloc(p, &v.prop_lp, TW / 2)) ||
-        (rc = alloc(p, &v.list, TW / 2)) || (rc = alloc(p, &v.nlive, 2))) {
+        (rc = alloc(p, &v.rec, TW / 2)) || (rc = alloc(p, &v.prow, TW / 2 * v.D))) {
         emp_pt_destroy(p);
         return rc;
     }
@@ -738,7 +751,6 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     EMP_TRY(hipMemcpy(v.betas, betas, (size_t)v.T * sizeof(double), hipMemcpyHostToDevice));
     EMP_TRY(hipMemset(v.accepted, 0, TW * sizeof(long long)));
     EMP_TRY(hipMemset(v.sweep, 0, 2 * sizeof(long long)));
-    EMP_TRY(hipMemset(v.nlive, 0, 2 * sizeof(int)));
     EMP_TRY(hipDeviceSynchronize());     // the engine stream does not wait for the null stream
 #undef EMP_TRY
     p->wpt = emp_pick_wpt(m, (int64_t)v.T * v.W2, d.precision);
@@ -800,15 +812,12 @@ extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     if (!p || n < 1 || !ms_avg) { emp_set_error("bad argument"); return EMP_EINVAL; }
     std::vector<hipEvent_t> ev(2 * (size_t)n);
     for (auto& e : ev) EMP_HIP(hipEventCreate(&e));
-    EMP_HIP(hipMemsetAsync(p->dev.nlive, 0, 2 * sizeof(int), p->stream));
     for (int i = 0; i < n; ++i) {
         EMP_HIP(hipEventRecord(ev[2 * i], p->stream));
         int rc = launch_half(p, i & 1, 0);
         if (rc) return rc;
         EMP_HIP(hipEventRecord(ev[2 * i + 1], p->stream));
     }
-    // leave both list counters clear for the next sweep's first half
-    EMP_HIP(hipMemsetAsync(p->dev.nlive, 0, 2 * sizeof(int), p->stream));
     EMP_HIP(hipStreamSynchronize(p->stream));
     double tot = 0.0;
     for (int i = 0; i < n; ++i) {
@@ -829,7 +838,7 @@ extern "C" int emp_pt_autotune(EmpPT* p, int32_t trials, int32_t* best_wpt, doub
     PTDev& v = p->dev;
     const size_t TW = (size_t)v.T * v.W;
     const std::pair<void*, size_t> items[] = {{v.X, TW * v.D * 8}, {v.lnl, TW * 8}, {v.lnp, TW * 8},
-                                              {v.accepted, TW * 8}, {v.sweep, 16}, {v.nlive, 8}};
+                                              {v.accepted, TW * 8}, {v.sweep, 16}};
     std::vector<void*> copies;
     auto release = [&]() { for (void* q : copies) (void)hipFree(q); };
     for (const auto& it : items) {
@@ -1070,7 +1079,6 @@ extern "C" int emp_pt_restore(EmpPT* p, const char* path) {
     const int64_t n = (int64_t)TW;
     hipLaunchKernelGGL(k_identity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, v.map, n);
     EMP_HIP(hipGetLastError());
-    EMP_HIP(hipMemsetAsync(v.nlive, 0, 2 * sizeof(int), p->stream));
     EMP_HIP(hipStreamSynchronize(p->stream));
     p->host_sweeps = sw;
     return EMP_OK;

@@ -149,29 +149,47 @@ __device__ inline int compact_live(const double* lp, int n, int* list, int* s_wc
     return base;
 }
 
-// One slot's task of a fused stretch half-step, for a task whose proposal
-// prior lp is already known (b < 0: no task for this slot in this pass):
-// proposal into LDS, likelihood, Metropolis acceptance of
-//   (D - 1) ln z + beta_t (lnL' - lnL) + ln pi' - ln pi.
-// Every evaluated likelihood adds one to *n_lnl (global or LDS).  The
-// sin/cos table must be in `tile`; with `resident` all epochs are too.
-// All threads of the workgroup call this together (it contains barriers).
+// Acceptance operands of one stretch proposal that are fixed before its
+// likelihood is known: the Metropolis log ratio is
+//   ((D - 1) ln z + (beta lnL' - beta lnL)) + (ln pi' - ln pi),
+// with beta lnL taken as 0 at beta = 0.
+struct TaskOps {
+    int t, k, sk;         // temperature, walker, slot of the walker
+    double lp;            // ln prior of the proposal
+    double lz;            // (D - 1) ln z
+    double dp;            // ln pi' - ln pi
+    double to;            // beta lnL of the current position (0 at beta = 0)
+    double beta, logu;    // inverse temperature, log of the acceptance uniform
+};
+
+__device__ __forceinline__ TaskOps pt_task_ops(const PTView& pv, const Proposal& pr, double lp, int D) {
+    TaskOps o;
+    o.t = pr.t;
+    o.k = pr.k;
+    o.sk = pr.sk;
+    o.lp = lp;
+    o.beta = pv.betas[pr.t];
+    o.lz = (double)(D - 1) * log(pr.z);
+    o.dp = lp - pv.lnp[pr.sk];
+    o.to = o.beta == 0.0 ? 0.0 : o.beta * pv.lnl[pr.sk];
+    o.logu = log(pr.uacc);
+    return o;
+}
+
+// Likelihood and Metropolis acceptance of one slot's task whose proposal is
+// already in sl.theta (free and fixed parameters) and whose operands are in
+// s_ops[slot] (LDS, read only at the acceptance so that they do not occupy
+// registers through the likelihood): per-slot constants, the mixed path's
+// Kepler tables, lnL, acceptance, and the accepted position written back.
+// Every evaluated likelihood adds one to *n_lnl.  The sin/cos table must be
+// in `tile`; with `resident` all epochs are too.  All threads of the
+// workgroup call this together (it contains barriers).
 template <int PREC, int WPT>
-__device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long sweep, int half, int step, int b,
-                             double lp, SlotLDS& sl, TileLDS& tile, ScanLDS& scan, int* s_acc, int slot, int wsub,
-                             bool resident, unsigned long long* n_lnl) {
+__device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskOps* s_ops, bool live, SlotLDS& sl,
+                               TileLDS& tile, ScanLDS& scan, int* s_acc, int slot, int wsub, bool resident,
+                               unsigned long long* n_lnl) {
     const int lane = threadIdx.x & 63;
     const int g = wsub * 64 + lane, G = 64 * WPT;
-    const bool live = b >= 0 && lp > -__builtin_inf();
-    Proposal pr{0, 0, 0, 0, 1.0, 0.5};
-    if (live) {
-        pr = pt_propose(pv, sweep, half, step, b);
-        const double* Xk = pv.X + (size_t)pr.sk * pv.D;
-        const double* Xj = pv.X + (size_t)pr.sj * pv.D;
-        for (int d = g; d < md.D; d += G) sl.theta[md.free_idx[d]] = pt_prop_value(Xk, Xj, pr.z, d);
-        for (int q = g; q < md.Dfull; q += G)
-            if (md.inv_free[q] < 0) sl.theta[q] = md.fixed_val[q];
-    }
     __syncthreads();
     if (live && wsub == 0) slot_setup(md, sl, lane);
     __syncthreads();
@@ -182,26 +200,47 @@ __device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long swe
     __syncthreads();
     double ll = block_lnl<PREC, WPT, false>(md, sl, tile, scan, live, wsub, nullptr, nullptr, resident);
     if (live && wsub == 0 && lane == 0) {
+        const TaskOps& ops = s_ops[slot];
         int acc = 0;
         atomicAdd(n_lnl, 1ull);
         if (!(ll == ll) || ll == __builtin_inf()) ll = -__builtin_inf();
-        const double beta = pv.betas[pr.t];
-        const double tn = beta == 0.0 ? 0.0 : beta * ll;
-        const double to = beta == 0.0 ? 0.0 : beta * pv.lnl[pr.sk];
-        const double lnalpha = (double)(md.D - 1) * log(pr.z) + (tn - to) + (lp - pv.lnp[pr.sk]);
-        if (log(pr.uacc) < lnalpha) {
-            pv.lnl[pr.sk] = ll;
-            pv.lnp[pr.sk] = lp;
-            pv.accepted[pr.t * pv.W + pr.k] += 1;
+        const double tn = ops.beta == 0.0 ? 0.0 : ops.beta * ll;
+        const double lnalpha = (ops.lz + (tn - ops.to)) + ops.dp;
+        if (ops.logu < lnalpha) {
+            pv.lnl[ops.sk] = ll;
+            pv.lnp[ops.sk] = ops.lp;
+            pv.accepted[ops.t * pv.W + ops.k] += 1;
             acc = 1;
         }
         s_acc[slot] = acc;
     }
     __syncthreads();
     if (live && s_acc[slot]) {
-        double* Xk = pv.X + (size_t)pr.sk * pv.D;
+        double* Xk = pv.X + (size_t)s_ops[slot].sk * pv.D;
         for (int d = g; d < md.D; d += G) Xk[d] = sl.theta[md.free_idx[d]];
     }
+}
+
+// One slot's task of a fused stretch half-step, for a task whose proposal
+// prior lp is already known (b < 0: no task for this slot in this pass):
+// proposal into LDS, then pt_eval_accept.
+template <int PREC, int WPT>
+__device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long sweep, int half, int step, int b,
+                             double lp, SlotLDS& sl, TileLDS& tile, ScanLDS& scan, int* s_acc, TaskOps* s_ops,
+                             int slot, int wsub, bool resident, unsigned long long* n_lnl) {
+    const int lane = threadIdx.x & 63;
+    const int g = wsub * 64 + lane, G = 64 * WPT;
+    const bool live = b >= 0 && lp > -__builtin_inf();
+    if (live) {
+        const Proposal pr = pt_propose(pv, sweep, half, step, b);
+        const double* Xk = pv.X + (size_t)pr.sk * pv.D;
+        const double* Xj = pv.X + (size_t)pr.sj * pv.D;
+        for (int d = g; d < md.D; d += G) sl.theta[md.free_idx[d]] = pt_prop_value(Xk, Xj, pr.z, d);
+        for (int q = g; q < md.Dfull; q += G)
+            if (md.inv_free[q] < 0) sl.theta[q] = md.fixed_val[q];
+        if (wsub == 0 && lane == 0) s_ops[slot] = pt_task_ops(pv, pr, lp, md.D);
+    }
+    pt_eval_accept<PREC, WPT>(md, pv, s_ops, live, sl, tile, scan, s_acc, slot, wsub, resident, n_lnl);
 }
 
 __device__ __forceinline__ uint32_t swap_partner(const PTView& pv, long long sweep, int h, int i) {

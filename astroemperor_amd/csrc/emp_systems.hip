@@ -140,6 +140,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     __shared__ int s_acc[SLOTS];
+    __shared__ TaskOps s_ops[SLOTS];
     __shared__ double s_red[64][2];
     __shared__ double s_rate[64];
     __shared__ double s_tlp[SYS_MAX_TASKS];    // prior of each task's proposal
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
                     const int q = q0 + slot;
                     const int b = q < nlive ? s_list[q] : -1;
                     pt_half_task<PREC, WPT>(md, pv, sweep, half, st, b, b >= 0 ? s_tlp[b] : -__builtin_inf(),
-                                            s_slot[slot], s_tile, s_scan, s_acc, slot, wsub, res, &s_cnt);
+                                            s_slot[slot], s_tile, s_scan, s_acc, s_ops, slot, wsub, res, &s_cnt);
                 }
                 __syncthreads();     // this half's writes before the other half reads them
             }
