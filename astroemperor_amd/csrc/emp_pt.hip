@@ -36,6 +36,8 @@ struct PTDev {
     double* lnp;        // [slots]
     int* map;           // [T][W] -> slot
     double* swap_l;     // [T][W] scratch for the global-memory swap path
+    double* swap_lu;    // [T-1][W] log uniforms of this sweep's swap round, drawn by the last prior pass
+    int* swap_jp;       // [T-1][W] partners of this sweep's swap round
     double* betas;      // [T]
     long long* accepted;  // [T][W] (logical positions)
     long long* sweep;     // [2] sweeps completed, likelihoods evaluated
@@ -92,11 +94,22 @@ static __host__ __device__ inline PTView view_of(const PTDev& pd) {
 constexpr int PRIOR_BLOCK = 256;
 constexpr int PRIOR_WAVES = PRIOR_BLOCK / 64;
 
-__global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd, int half, int step) {
+__global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd, int half, int step, int draw_swaps) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.x * PRIOR_WAVES + wave;
-    if (b >= pd.T * pd.W2) return;            // whole waves
     const PTView pv = view_of(pd);
+    if (draw_swaps) {
+        // The swap round's pairings and uniforms depend on (seed, sweep, pair)
+        // only; this grid has T W / 2 x 64 threads >= (T - 1) W pairs.
+        const int e = blockIdx.x * PRIOR_BLOCK + threadIdx.x;
+        if (e < (pd.T - 1) * pd.W) {
+            const long long sweep = *pd.sweep;
+            const int h = e / pd.W + 1, i = e - (h - 1) * pd.W;
+            pd.swap_jp[e] = (int)swap_partner(pv, sweep, h, i);
+            pd.swap_lu[e] = swap_logu(pv, sweep, h, i);
+        }
+    }
+    if (b >= pd.T * pd.W2) return;            // whole waves
     const Proposal pr = pt_propose(pv, *pd.sweep, half, step, b);
     const double* Xk = pd.X + (size_t)pr.sk * pd.D;
     const double* Xj = pd.X + (size_t)pr.sj * pd.D;
@@ -186,13 +199,22 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     int* Mp = USE_LDS ? (int*)(s_dyn + n + np) : pd.map;
     int* JP = (int*)(s_dyn + n + np) + n;            // [T-1][W] partners (LDS path)
     if (USE_LDS) {
-        pt_swap_stage(pv, sweep, L, Mp, LU, JP, SWAP_BLOCK);
+        for (int e = tid; e < n; e += SWAP_BLOCK) {
+            const int m = pd.map[e];
+            Mp[e] = m;
+            L[e] = pd.lnl[m];
+        }
+        for (int e = tid; e < np; e += SWAP_BLOCK) {
+            LU[e] = pd.swap_lu[e];
+            JP[e] = pd.swap_jp[e];
+        }
     } else {
         for (int e = tid; e < n; e += SWAP_BLOCK) L[e] = pd.lnl[pd.map[e]];
         __threadfence();
     }
     __syncthreads();
-    pt_swap_round(pv, sweep, L, Mp, LU, JP, USE_LDS, s_red, SWAP_BLOCK);
+    pt_swap_round(pv, sweep, L, Mp, USE_LDS ? LU : pd.swap_lu, USE_LDS ? JP : pd.swap_jp, true, s_red,
+                  SWAP_BLOCK);
     if (tid < T - 1) {
         const double na = s_red[tid][0], di = s_red[tid][1];
         s_rate[tid] = na / W;
@@ -210,6 +232,18 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
             for (int i = 0; i < T; ++i) pd.h_betas[(size_t)sweep * T + i] = pd.betas[i];
         *pd.sweep = sweep + 1;
     }
+}
+
+// The swap round's randoms for a sweep without stretch half-steps (the
+// gradient-move sweep).
+__global__ void k_pt_swap_draw(PTDev pd) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (pd.T - 1) * pd.W) return;
+    const PTView pv = view_of(pd);
+    const long long sweep = *pd.sweep;
+    const int h = e / pd.W + 1, i = e - (h - 1) * pd.W;
+    pd.swap_jp[e] = (int)swap_partner(pv, sweep, h, i);
+    pd.swap_lu[e] = swap_logu(pv, sweep, h, i);
 }
 
 __global__ void k_pt_record(PTDev pd) {
@@ -574,13 +608,15 @@ static void launch_half_t(const ModelDev& md, const PTDev& pd, int half, int ste
     hipLaunchKernelGGL((k_pt_half<PREC, WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, pd, half, step);
 }
 
-static int launch_half(EmpPT* p, int half, int step) {
+// draw_swaps: this prior pass also draws the sweep's swap randoms (the last
+// half-step of a sweep).
+static int launch_half(EmpPT* p, int half, int step, int draw_swaps) {
     const ModelDev& md = p->model->dev;
     const int prec = p->desc.precision, w = p->wpt;
     {
         const int64_t tasks = (int64_t)p->dev.T * p->dev.W2;
         hipLaunchKernelGGL(k_pt_prior, dim3((unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES)), dim3(PRIOR_BLOCK), 0,
-                           p->stream, md, p->dev, half, step);
+                           p->stream, md, p->dev, half, step, draw_swaps);
         EMP_HIP(hipGetLastError());
     }
 #define EMP_CASE(P, W) if (prec == P && w == W) { launch_half_t<P, W>(md, p->dev, half, step, p->stream); EMP_HIP(hipGetLastError()); return EMP_OK; }
@@ -611,7 +647,7 @@ static int launch_tail(EmpPT* p) {
 static int launch_sweep(EmpPT* p) {
     for (int st = 0; st < p->desc.nsteps; ++st)
         for (int h = 0; h < 2; ++h) {
-            int rc = launch_half(p, h, st);
+            int rc = launch_half(p, h, st, h == 1 && st == p->desc.nsteps - 1);
             if (rc) return rc;
         }
     return launch_tail(p);
@@ -639,6 +675,11 @@ static int launch_grad_sweep(EmpPT* p) {
             hipLaunchKernelGGL(k_lg_accept, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, d, h, st);
             EMP_HIP(hipGetLastError());
         }
+    if (d.T > 1) {
+        const int np = (d.T - 1) * d.W;
+        hipLaunchKernelGGL(k_pt_swap_draw, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, p->stream, d);
+        EMP_HIP(hipGetLastError());
+    }
     return launch_tail(p);
 }
 
@@ -726,6 +767,8 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     int rc = 0;
     if ((rc = alloc(p, &v.X, TW * v.D)) || (rc = alloc(p, &v.lnl, TW)) || (rc = alloc(p, &v.lnp, TW)) ||
         (rc = alloc(p, &v.map, TW)) || (rc = alloc(p, &v.swap_l, TW)) ||
+        (rc = alloc(p, &v.swap_lu, (size_t)std::max(v.T - 1, 1) * v.W)) ||
+        (rc = alloc(p, &v.swap_jp, (size_t)std::max(v.T - 1, 1) * v.W)) ||
         (rc = alloc(p, &v.betas, (size_t)v.T)) || (rc = alloc(p, &v.accepted, TW)) || (rc = alloc(p, &v.sweep, 2)) ||
         (rc = alloc(p, &v.h_betas, S * v.T)) || (rc = alloc(p, &v.h_tsw, S * std::max(v.T - 1, 1))) ||
         (rc = alloc(p, &v.h_smd, S * std::max(v.T - 1, 1))) || (rc = alloc(p, &v.prop_lp, TW / 2)) ||
@@ -814,7 +857,7 @@ extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     for (auto& e : ev) EMP_HIP(hipEventCreate(&e));
     for (int i = 0; i < n; ++i) {
         EMP_HIP(hipEventRecord(ev[2 * i], p->stream));
-        int rc = launch_half(p, i & 1, 0);
+        int rc = launch_half(p, i & 1, 0, 0);
         if (rc) return rc;
         EMP_HIP(hipEventRecord(ev[2 * i + 1], p->stream));
     }
