@@ -101,6 +101,37 @@ __device__ __forceinline__ void tile_load(const ModelDev& md, TileLDS& tl, int t
     }
 }
 
+// One epoch of a tile in registers.  Every launch that stages tiles has at
+// least TILE threads, so thread j holds epoch tb + j of the tile at tb: its
+// global loads are issued a tile ahead and stored to LDS when the tile's
+// turn comes.
+struct EpochRegs {
+    double t, y, s2, dt;
+    int ins;
+};
+
+__device__ __forceinline__ void epoch_prefetch(const ModelDev& md, int tb, EpochRegs& r) {
+    const int j = threadIdx.x, n = tb + j;
+    if (j < TILE && n < md.N) {
+        r.t = md.t[n];
+        r.y = md.y[n];
+        r.s2 = md.s2[n];
+        r.ins = md.ins[n];
+        r.dt = md.ma_active ? md.dt[n] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void epoch_store(const ModelDev& md, TileLDS& tl, int tb, const EpochRegs& r) {
+    const int j = threadIdx.x;
+    if (j < TILE && tb + j < md.N) {
+        tl.t[j] = r.t;
+        tl.y[j] = r.y;
+        tl.s2[j] = r.s2;
+        tl.ins[j] = r.ins;
+        if (md.ma_active) tl.dt[j] = r.dt;
+    }
+}
+
 // Per-slot constants, then (mixed path) the per-vector Kepler tables, one
 // planet per wave of the slot.  The sin/cos table must already be in LDS.
 // All threads of the workgroup call this (it holds two barriers).
@@ -179,10 +210,14 @@ struct ScanLDS {
 // With WRITE the per-epoch model and err2 are stored (device order); with
 // WEPS the per-epoch MA-filtered residual eps (= r without MA).
 // `resident`: the caller has already staged all epochs (md.N <= TILE) in tl.
-template <int PREC, int WPT, bool WRITE, bool WEPS = false>
+// `first`: the first tile's epochs, already loaded by the caller (else loaded
+// here).  With PF the next tile's loads are issued a tile ahead (launches
+// have at least TILE threads); without, each tile is loaded when its turn
+// comes (fewer registers).
+template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true>
 __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
                             bool live, int wsub, double* out_model, double* out_err2, bool resident = false,
-                            double* out_eps = nullptr) {
+                            double* out_eps = nullptr, const EpochRegs* first = nullptr) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int G = 64 * WPT;
@@ -191,12 +226,27 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
     int pexp = 0;
     double carry = 0.0;
     int parity = 0;
+    const bool staged = resident && md.N <= TILE;
+    // With the moving average over several waves, every chunk ends with the
+    // carry's barrier, after the chunk's last read of the tile.
+    const bool ma_sync = md.ma_active && WPT > 1;
+    EpochRegs nx{};
+    if (PF && !staged) {
+        if (first) nx = *first;
+        else epoch_prefetch(md, 0, nx);
+    }
     for (int tb = 0; tb < md.N; tb += TILE) {
         const int tn = min(TILE, md.N - tb);
-        if (!(resident && md.N <= TILE)) {
-            __syncthreads();
-            tile_load(md, tl, tb, tn);
-            __syncthreads();
+        if (!staged) {
+            if (tb == 0 || !ma_sync) __syncthreads();     // the tile's previous readers are done
+            if (PF) {
+                epoch_store(md, tl, tb, nx);
+                __syncthreads();
+                if (tb + TILE < md.N) epoch_prefetch(md, tb + TILE, nx);
+            } else {
+                tile_load(md, tl, tb, tn);
+                __syncthreads();
+            }
         }
         for (int cb = 0; cb < tn; cb += G) {
             const int j = cb + g;

@@ -184,12 +184,15 @@ __device__ __forceinline__ TaskOps pt_task_ops(const PTView& pv, const Proposal&
 // Every evaluated likelihood adds one to *n_lnl.  The sin/cos table must be
 // in `tile`; with `resident` all epochs are too.  All threads of the
 // workgroup call this together (it contains barriers).
-template <int PREC, int WPT>
+template <int PREC, int WPT, bool PF = true>
 __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskOps* s_ops, bool live, SlotLDS& sl,
                                TileLDS& tile, ScanLDS& scan, int* s_acc, int slot, int wsub, bool resident,
                                unsigned long long* n_lnl) {
     const int lane = threadIdx.x & 63;
     const int g = wsub * 64 + lane, G = 64 * WPT;
+    EpochRegs e0{};
+    const bool staged = resident && md.N <= TILE;
+    if (PF && !staged) epoch_prefetch(md, 0, e0);      // the first tile's loads overlap the set-up
     __syncthreads();
     if (live && wsub == 0) slot_setup(md, sl, lane);
     __syncthreads();
@@ -198,7 +201,8 @@ __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskO
             if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
     }
     __syncthreads();
-    double ll = block_lnl<PREC, WPT, false>(md, sl, tile, scan, live, wsub, nullptr, nullptr, resident);
+    double ll = block_lnl<PREC, WPT, false, false, PF>(md, sl, tile, scan, live, wsub, nullptr, nullptr, resident,
+                                                       nullptr, (PF && !staged) ? &e0 : nullptr);
     if (live && wsub == 0 && lane == 0) {
         const TaskOps& ops = s_ops[slot];
         int acc = 0;
@@ -223,7 +227,8 @@ __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskO
 
 // One slot's task of a fused stretch half-step, for a task whose proposal
 // prior lp is already known (b < 0: no task for this slot in this pass):
-// proposal into LDS, then pt_eval_accept.
+// proposal into LDS, then pt_eval_accept (without the tile prefetch: the
+// batched-systems kernel that calls this is register-bound).
 template <int PREC, int WPT>
 __device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long sweep, int half, int step, int b,
                              double lp, SlotLDS& sl, TileLDS& tile, ScanLDS& scan, int* s_acc, TaskOps* s_ops,
@@ -240,7 +245,7 @@ __device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long swe
             if (md.inv_free[q] < 0) sl.theta[q] = md.fixed_val[q];
         if (wsub == 0 && lane == 0) s_ops[slot] = pt_task_ops(pv, pr, lp, md.D);
     }
-    pt_eval_accept<PREC, WPT>(md, pv, s_ops, live, sl, tile, scan, s_acc, slot, wsub, resident, n_lnl);
+    pt_eval_accept<PREC, WPT, false>(md, pv, s_ops, live, sl, tile, scan, s_acc, slot, wsub, resident, n_lnl);
 }
 
 __device__ __forceinline__ uint32_t swap_partner(const PTView& pv, long long sweep, int h, int i) {

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_init(SysDev sd) {
         __syncthreads();
         if (live && wsub == 0 && lane == 0) s_lp[slot] = prior_seq(md, sl.theta);
         slot_prepare<PREC, WPT>(md, sl, s_tile, live, wsub, lane);
-        double v = block_lnl<PREC, WPT, false>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
+        double v = block_lnl<PREC, WPT, false, false, false>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
         if (live && wsub == 0 && lane == 0) {
             if (!(v == v) || v == __builtin_inf()) v = -__builtin_inf();
             const double lp = s_lp[slot];
