@@ -72,6 +72,7 @@ struct EmpPT {
     hipStream_t stream = nullptr;
     hipGraphExec_t graph_exec = nullptr;      // stretch sweep
     hipGraphExec_t graph_grad = nullptr;      // gradient-move sweep
+    hipGraphExec_t graph_run[5] = {};         // 2, 4, 8, 16, 32 stretch sweeps in one graph
     double w_stretch = 1.0, w_grad = 0.0;     // move weights (one move per sweep)
     int wpt = 1;
     size_t swap_lds = 0;         // dynamic LDS of the swap kernel, 0 = global path
@@ -697,7 +698,27 @@ static int pick_move(const EmpPT* p, long long sweep) {
 static void drop_graphs(EmpPT* p) {
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph_grad) (void)hipGraphExecDestroy(p->graph_grad);
+    for (auto& g : p->graph_run) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
     p->graph_exec = p->graph_grad = nullptr;
+}
+
+// Captures `n` consecutive sweeps of one kind into a graph.  The sweep
+// counter lives on the device, so replaying a graph of n sweeps equals n
+// replays of the one-sweep graph.
+static int capture_sweeps(EmpPT* p, int kind, int n, hipGraphExec_t* out) {
+    hipGraph_t g;
+    EMP_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    int rc = EMP_OK;
+    for (int i = 0; i < n && !rc; ++i) rc = kind ? launch_grad_sweep(p) : launch_sweep(p);
+    hipError_t ce = hipStreamEndCapture(p->stream, &g);
+    if (rc) return rc;
+    EMP_HIP(ce);
+    EMP_HIP(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+    EMP_HIP(hipGraphDestroy(g));
+    return EMP_OK;
 }
 
 template <typename T>
@@ -818,25 +839,27 @@ extern "C" int emp_pt_run(EmpPT* p, int64_t nsweeps, int32_t use_graph) {
     EMP_HIP(hipEventCreate(&e1));
     int rc = EMP_OK;
     EMP_HIP(hipEventRecord(e0, p->stream));
-    for (int64_t s = 0; s < nsweeps; ++s) {
+    for (int64_t s = 0; s < nsweeps;) {
         const int mv = pick_move(p, p->host_sweeps + s);
-        if (use_graph) {
-            hipGraphExec_t& ge = mv ? p->graph_grad : p->graph_exec;
-            if (!ge) {
-                hipGraph_t g;
-                EMP_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-                rc = mv ? launch_grad_sweep(p) : launch_sweep(p);
-                hipError_t ce = hipStreamEndCapture(p->stream, &g);
-                if (rc) return rc;
-                EMP_HIP(ce);
-                EMP_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-                EMP_HIP(hipGraphDestroy(g));
-            }
-            EMP_HIP(hipGraphLaunch(ge, p->stream));
-        } else {
+        if (!use_graph) {
             rc = mv ? launch_grad_sweep(p) : launch_sweep(p);
             if (rc) return rc;
+            ++s;
+            continue;
         }
+        // A run of stretch sweeps goes out in graphs of 32, 16, ..., 2, 1
+        // sweeps: one graph launch per graph instead of per sweep.
+        int64_t run = 1;
+        if (!mv)
+            while (run < 32 && s + run < nsweeps && !pick_move(p, p->host_sweeps + s + run)) ++run;
+        int lvl = -1;
+        for (int l = 4; l >= 0; --l)
+            if (run >= (2 << l)) { lvl = l; break; }
+        hipGraphExec_t& ge = lvl >= 0 ? p->graph_run[lvl] : (mv ? p->graph_grad : p->graph_exec);
+        const int n = lvl >= 0 ? (2 << lvl) : 1;
+        if (!ge && (rc = capture_sweeps(p, mv, n, &ge))) return rc;
+        EMP_HIP(hipGraphLaunch(ge, p->stream));
+        s += n;
     }
     EMP_HIP(hipEventRecord(e1, p->stream));
     EMP_HIP(hipEventSynchronize(e1));

@@ -1,11 +1,14 @@
 """Text summaries of rocprofv3 result databases (rocpd SQLite).
 
-    python -m astroemperor_amd.profiling RESULTS.db [--counters] [--top N]
+    python -m astroemperor_amd.profiling RESULTS.db [--counters | --gaps] [--top N]
 
-Without --counters: one row per kernel (dispatches, total and mean device
-time, VGPRs, LDS) from a ``--kernel-trace`` run.  With --counters: the mean
-value per dispatch of every collected counter, per kernel, from a ``--pmc``
-run (FETCH_SIZE / WRITE_SIZE are in KiB).
+Without options: one row per kernel (dispatches, total and mean device
+time, VGPRs, LDS) from a ``--kernel-trace`` run.  With --gaps: dispatches in
+time order grouped by (previous kernel, kernel), with the mean duration and
+the mean idle time between the previous kernel's end and this one's start:
+the launch sequence of a captured sweep and what each boundary costs.  With
+--counters: the mean value per dispatch of every collected counter, per
+kernel, from a ``--pmc`` run (FETCH_SIZE / WRITE_SIZE are in KiB).
 """
 from __future__ import annotations
 
@@ -27,6 +30,28 @@ def kernel_table(db, top=12):
     return "\n".join(out)
 
 
+def gap_table(db, top=12, min_count=50):
+    c = sqlite3.connect(db)
+    cols = [r[1] for r in c.execute("pragma table_info(kernels)").fetchall()]
+    t0, t1 = ("start", "end") if "start" in cols else ("start_time", "end_time")
+    rows = c.execute(f'select name, "{t0}", "{t1}" from kernels order by "{t0}"').fetchall()
+    stats = {}
+    prev = None
+    for name, a, b in rows:
+        if prev is not None:
+            key = (prev[0], name)
+            st = stats.setdefault(key, [0, 0.0, 0.0])
+            st[0] += 1
+            st[1] += b - a
+            st[2] += a - prev[2]
+        prev = (name, a, b)
+    items = sorted(((k, v) for k, v in stats.items() if v[0] >= min_count), key=lambda kv: -kv[1][1])[:top]
+    out = ["| after | kernel | dispatches | mean us | mean gap before, us |", "|---|---|---|---|---|"]
+    for (pn, n), (cnt, dur, gap) in items:
+        out.append(f"| `{pn[:40]}` | `{n[:40]}` | {cnt} | {dur / cnt / 1e3:.2f} | {gap / cnt / 1e3:.2f} |")
+    return "\n".join(out)
+
+
 def counter_table(db, top=12):
     c = sqlite3.connect(db)
     rows = c.execute(
@@ -42,9 +67,15 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("db")
     ap.add_argument("--counters", action="store_true")
+    ap.add_argument("--gaps", action="store_true")
     ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args()
-    print(counter_table(a.db, a.top) if a.counters else kernel_table(a.db, a.top))
+    if a.counters:
+        print(counter_table(a.db, a.top))
+    elif a.gaps:
+        print(gap_table(a.db, a.top))
+    else:
+        print(kernel_table(a.db, a.top))
 
 
 if __name__ == "__main__":
