@@ -189,7 +189,7 @@ template <bool USE_LDS>
 __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     extern __shared__ double s_dyn[];
     __shared__ double s_red[64][2];
-    __shared__ double s_rate[64];
+    __shared__ double s_rate[64], s_bet[64], s_ld[64];
     const int tid = threadIdx.x;
     const long long sweep = *pd.sweep;
     const PTView pv = view_of(pd);
@@ -199,6 +199,7 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     double* LU = s_dyn + n;                          // [T-1][W] log uniforms (LDS path)
     int* Mp = USE_LDS ? (int*)(s_dyn + n + np) : pd.map;
     int* JP = (int*)(s_dyn + n + np) + n;            // [T-1][W] partners (LDS path)
+    for (int e = tid; e < T; e += SWAP_BLOCK) s_bet[e] = pd.betas[e];
     if (USE_LDS) {
         for (int e = tid; e < n; e += SWAP_BLOCK) {
             const int m = pd.map[e];
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
         __threadfence();
     }
     __syncthreads();
-    pt_swap_round(pv, sweep, L, Mp, USE_LDS ? LU : pd.swap_lu, USE_LDS ? JP : pd.swap_jp, true, s_red,
+    pt_swap_round(pv, sweep, L, Mp, USE_LDS ? LU : pd.swap_lu, USE_LDS ? JP : pd.swap_jp, true, s_red, s_bet,
                   SWAP_BLOCK);
     if (tid < T - 1) {
         const double na = s_red[tid][0], di = s_red[tid][1];
@@ -227,12 +228,10 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     if (USE_LDS)
         for (int e = tid; e < n; e += SWAP_BLOCK) pd.map[e] = Mp[e];
     __syncthreads();
-    if (tid == 0) {
-        pt_ladder_update(pv, sweep, s_rate);
-        if (sweep < pd.max_sweeps)
-            for (int i = 0; i < T; ++i) pd.h_betas[(size_t)sweep * T + i] = pd.betas[i];
-        *pd.sweep = sweep + 1;
-    }
+    pt_ladder_update(pv, sweep, s_rate, s_bet, s_ld, SWAP_BLOCK);
+    __syncthreads();
+    if (tid < T && sweep < pd.max_sweeps) pd.h_betas[(size_t)sweep * T + tid] = s_bet[tid];
+    if (tid == 0) *pd.sweep = sweep + 1;
 }
 
 // The swap round's randoms for a sweep without stretch half-steps (the

@@ -286,15 +286,16 @@ __device__ inline void pt_swap_stage(const PTView& pv, long long sweep, double* 
 // uniforms and partners of every pair) or in global memory (pre == false,
 // randoms drawn inline).  On return s_red[c] = (accepted, sum |log ratio|)
 // of pair (c, c+1).  Walker states never move; only map entries do.
+// bet: the ladder (LDS copy).
 __device__ inline void pt_swap_round(const PTView& pv, long long sweep, double* L, int* Mp, const double* LU,
-                                     const int* JP, bool pre, double (*s_red)[2], int nthr) {
+                                     const int* JP, bool pre, double (*s_red)[2], const double* bet, int nthr) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int T = pv.T, W = pv.W;
     for (int e = tid; e < 2 * 64; e += nthr) (&s_red[0][0])[e] = 0.0;
     __syncthreads();
     for (int h = T - 1; h >= 1; --h) {
         const int c = h - 1;
-        const double dbeta = pv.betas[c] - pv.betas[h];
+        const double dbeta = bet[c] - bet[h];
         double nacc = 0.0, dist = 0.0;
         for (int i = tid; i < W; i += nthr) {
             const int e = (h - 1) * W + i;
@@ -326,21 +327,31 @@ __device__ inline void pt_swap_round(const PTView& pv, long long sweep, double* 
 }
 
 // Vousden, Farr & Mandel (2016) ladder dynamics from this sweep's swap
-// acceptance `rate` (one thread): S_i = ln(T_i - T_{i-1}) moves by
-// kappa (A_{i-1} - A_i), kappa = tau / (n + tau) / nu; the coldest and
-// hottest temperatures stay fixed.
-__device__ inline void pt_ladder_update(const PTView& pv, long long sweep, const double* rate) {
+// acceptance `rate`: S_i = ln(T_i - T_{i-1}) moves by kappa (A_{i-1} - A_i),
+// kappa = tau / (n + tau) / nu; the coldest and hottest temperatures stay
+// fixed.  The new spacings (T_i - T_{i-1}) exp(dS_i) are formed in parallel
+// from the ladder `bet` (LDS copy of the current betas), then one thread
+// adds them up in order and writes pv.betas and bet.  All nthr threads call
+// this (it holds a barrier); s_d is LDS scratch of 64 doubles.
+__device__ inline void pt_ladder_update(const PTView& pv, long long sweep, const double* rate, double* bet,
+                                        double* s_d, int nthr) {
     const int T = pv.T;
     if (pv.adapt_mode != 0 || T < 3) return;
     const double kappa = pv.adapt_tau / ((double)sweep + pv.adapt_tau) / pv.adapt_nu;
-    double Tprev_old = 1.0 / pv.betas[0];
-    double acc_T = Tprev_old;
-    for (int i = 1; i < T - 1; ++i) {
-        const double Ti_old = 1.0 / pv.betas[i];
+    for (int i = (int)threadIdx.x + 1; i < T - 1; i += nthr) {
         const double dS = kappa * (rate[i - 1] - rate[i]);
-        acc_T += (Ti_old - Tprev_old) * exp(dS);
-        Tprev_old = Ti_old;
-        pv.betas[i] = 1.0 / acc_T;
+        const double spacing = 1.0 / bet[i] - 1.0 / bet[i - 1];
+        s_d[i] = spacing * exp(dS);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc_T = 1.0 / bet[0];
+        for (int i = 1; i < T - 1; ++i) {
+            acc_T += s_d[i];
+            const double b = 1.0 / acc_T;
+            bet[i] = b;
+            pv.betas[i] = b;
+        }
     }
 }
 

@@ -141,6 +141,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
     __shared__ ScanLDS s_scan;
     __shared__ int s_acc[SLOTS];
     __shared__ TaskOps s_ops[SLOTS];
+    __shared__ double s_bet[64], s_ld[64];
     __shared__ double s_red[64][2];
     __shared__ double s_rate[64];
     __shared__ double s_tlp[SYS_MAX_TASKS];    // prior of each task's proposal
@@ -190,8 +191,9 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
         int* Mp = (int*)(LU + (T - 1) * W);
         int* JP = Mp + TW;
         pt_swap_stage(pv, sweep, L, Mp, LU, JP, NT);
+        for (int e = tid; e < T; e += NT) s_bet[e] = pv.betas[e];
         __syncthreads();
-        pt_swap_round(pv, sweep, L, Mp, LU, JP, true, s_red, NT);
+        pt_swap_round(pv, sweep, L, Mp, LU, JP, true, s_red, s_bet, NT);
         for (int e = tid; e < TW; e += NT) pv.map[e] = Mp[e];
         if (tid < T - 1) {
             const double r = s_red[tid][0] / W;
@@ -216,10 +218,8 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
             }
         }
         __syncthreads();
-        if (tid == 0) {
-            pt_ladder_update(pv, sweep, s_rate);
-            atomicAdd(&sd.ctl[3], s_cnt);
-        }
+        pt_ladder_update(pv, sweep, s_rate, s_bet, s_ld, NT);
+        if (tid == 0) atomicAdd(&sd.ctl[3], s_cnt);
     }
     if (tid == 0) {
         __threadfence();
