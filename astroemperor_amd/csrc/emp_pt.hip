@@ -36,6 +36,7 @@ struct PTDev {
     double* lnp;        // [slots]
     int* map;           // [T][W] -> slot
     double* swap_l;     // [T][W] scratch for the global-memory swap path
+    int* rec_owed;      // 1: the previous sweep's state is still to be recorded (set by the swap round)
     double* swap_lu;    // [T-1][W] log uniforms of this sweep's swap round, drawn by the last prior pass
     int* swap_jp;       // [T-1][W] partners of this sweep's swap round
     double* betas;      // [T]
@@ -95,10 +96,34 @@ static __host__ __device__ inline PTView view_of(const PTDev& pd) {
 constexpr int PRIOR_BLOCK = 256;
 constexpr int PRIOR_WAVES = PRIOR_BLOCK / 64;
 
-__global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd, int half, int step, int draw_swaps) {
+// Chain record of logical walker w (temperature-major) into history row s.
+__device__ __forceinline__ void record_walker(const PTDev& pd, long long s, int w, int lane) {
+    const size_t nw = (size_t)pd.T * pd.W;
+    const int slot = pd.map[w];
+    double* cx = pd.h_chain + (size_t)s * nw * pd.D + (size_t)w * pd.D;
+    const double* x = pd.X + (size_t)slot * pd.D;
+    for (int d = lane; d < pd.D; d += 64) cx[d] = x[d];
+    if (lane == 0) {
+        pd.h_lnl[(size_t)s * nw + w] = pd.lnl[slot];
+        pd.h_lnp[(size_t)s * nw + w] = pd.lnp[slot];
+    }
+}
+
+__global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd, int half, int step, int draw_swaps,
+                                                          int first) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.x * PRIOR_WAVES + wave;
     const PTView pv = view_of(pd);
+    if (first && pd.record && *pd.rec_owed && b < pd.T * pd.W2) {
+        // The sweep's first pass records the previous sweep's final state
+        // (nothing has moved since): walker k of half 0 and its half-1 twin.
+        const long long s = *pd.sweep - 1;
+        if (s >= 0 && s < pd.max_sweeps) {
+            const int t = b / pd.W2, k = b % pd.W2;
+            record_walker(pd, s, t * pd.W + k, lane);
+            record_walker(pd, s, t * pd.W + k + pd.W2, lane);
+        }
+    }
     if (draw_swaps) {
         // The swap round's pairings and uniforms depend on (seed, sweep, pair)
         // only; this grid has T W / 2 x 64 threads >= (T - 1) W pairs.
@@ -231,7 +256,10 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     pt_ladder_update(pv, sweep, s_rate, s_bet, s_ld, SWAP_BLOCK);
     __syncthreads();
     if (tid < T && sweep < pd.max_sweeps) pd.h_betas[(size_t)sweep * T + tid] = s_bet[tid];
-    if (tid == 0) *pd.sweep = sweep + 1;
+    if (tid == 0) {
+        *pd.sweep = sweep + 1;
+        if (pd.record) *pd.rec_owed = 1;
+    }
 }
 
 // The swap round's randoms for a sweep without stretch half-steps (the
@@ -246,8 +274,12 @@ __global__ void k_pt_swap_draw(PTDev pd) {
     pd.swap_lu[e] = swap_logu(pv, sweep, h, i);
 }
 
-__global__ void k_pt_record(PTDev pd) {
+// Records the previous sweep's state if it is still owed (only_owed), or
+// unconditionally (the end of a run, which also clears the debt).
+__global__ void k_pt_record(PTDev pd, int only_owed) {
     const long long s = *pd.sweep - 1;
+    if (only_owed && !*pd.rec_owed) return;
+    if (!only_owed && blockIdx.x == 0 && threadIdx.x == 0) *pd.rec_owed = 0;
     if (s < 0 || s >= pd.max_sweeps) return;
     const size_t nw = (size_t)pd.T * pd.W, nx = nw * pd.D;
     double* cx = pd.h_chain + (size_t)s * nx;
@@ -610,13 +642,13 @@ static void launch_half_t(const ModelDev& md, const PTDev& pd, int half, int ste
 
 // draw_swaps: this prior pass also draws the sweep's swap randoms (the last
 // half-step of a sweep).
-static int launch_half(EmpPT* p, int half, int step, int draw_swaps) {
+static int launch_half(EmpPT* p, int half, int step, int draw_swaps, int first = 0) {
     const ModelDev& md = p->model->dev;
     const int prec = p->desc.precision, w = p->wpt;
     {
         const int64_t tasks = (int64_t)p->dev.T * p->dev.W2;
         hipLaunchKernelGGL(k_pt_prior, dim3((unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES)), dim3(PRIOR_BLOCK), 0,
-                           p->stream, md, p->dev, half, step, draw_swaps);
+                           p->stream, md, p->dev, half, step, draw_swaps, first);
         EMP_HIP(hipGetLastError());
     }
 #define EMP_CASE(P, W) if (prec == P && w == W) { launch_half_t<P, W>(md, p->dev, half, step, p->stream); EMP_HIP(hipGetLastError()); return EMP_OK; }
@@ -627,27 +659,34 @@ static int launch_half(EmpPT* p, int half, int step, int draw_swaps) {
     return EMP_EINVAL;
 }
 
-// Swap round (with ladder and sweep counter) and history record: the end of
-// every sweep, whichever move it made.
+// Swap round (with ladder and sweep counter): the end of every sweep,
+// whichever move it made.  The sweep's state is recorded by the next sweep's
+// first pass, or by launch_record at the end of the run.
 static int launch_tail(EmpPT* p) {
     if (p->swap_lds)
         hipLaunchKernelGGL(k_pt_swap<true>, dim3(1), dim3(SWAP_BLOCK), p->swap_lds, p->stream, p->dev);
     else
         hipLaunchKernelGGL(k_pt_swap<false>, dim3(1), dim3(SWAP_BLOCK), 0, p->stream, p->dev);
     EMP_HIP(hipGetLastError());
-    if (p->dev.record) {
-        const size_t nx = (size_t)p->dev.T * p->dev.W * p->dev.D;
-        const unsigned grid = (unsigned)std::min<size_t>((nx + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_pt_record, dim3(grid), dim3(256), 0, p->stream, p->dev);
-        EMP_HIP(hipGetLastError());
-    }
+    return EMP_OK;
+}
+
+// The history record of the previous sweep: only if still owed (the start
+// of a gradient sweep), or unconditionally, clearing the debt (the end of a
+// run).
+static int launch_record(EmpPT* p, int only_owed) {
+    if (!p->dev.record) return EMP_OK;
+    const size_t nx = (size_t)p->dev.T * p->dev.W * p->dev.D;
+    const unsigned grid = (unsigned)std::min<size_t>((nx + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_pt_record, dim3(grid), dim3(256), 0, p->stream, p->dev, only_owed);
+    EMP_HIP(hipGetLastError());
     return EMP_OK;
 }
 
 static int launch_sweep(EmpPT* p) {
     for (int st = 0; st < p->desc.nsteps; ++st)
         for (int h = 0; h < 2; ++h) {
-            int rc = launch_half(p, h, st, h == 1 && st == p->desc.nsteps - 1);
+            int rc = launch_half(p, h, st, h == 1 && st == p->desc.nsteps - 1, h == 0 && st == 0);
             if (rc) return rc;
         }
     return launch_tail(p);
@@ -662,7 +701,9 @@ static int launch_grad_sweep(EmpPT* p) {
     const ModelDev& md = p->model->dev;
     const int64_t TW = (int64_t)d.T * d.W, tasks = (int64_t)d.T * d.W2;
     const unsigned tgrid = (unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES);
-    int rc = emp_launch_grad_ws(p->model, d.X, TW, d.D, d.lg_l, d.lg_g, d.lg_scr, p->stream);
+    int rc = launch_record(p, 1);
+    if (rc) return rc;
+    rc = emp_launch_grad_ws(p->model, d.X, TW, d.D, d.lg_l, d.lg_g, d.lg_scr, p->stream);
     if (rc) return rc;
     for (int st = 0; st < p->desc.nsteps; ++st)
         for (int h = 0; h < 2; ++h) {
@@ -787,7 +828,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     int rc = 0;
     if ((rc = alloc(p, &v.X, TW * v.D)) || (rc = alloc(p, &v.lnl, TW)) || (rc = alloc(p, &v.lnp, TW)) ||
         (rc = alloc(p, &v.map, TW)) || (rc = alloc(p, &v.swap_l, TW)) ||
-        (rc = alloc(p, &v.swap_lu, (size_t)std::max(v.T - 1, 1) * v.W)) ||
+        (rc = alloc(p, &v.swap_lu, (size_t)std::max(v.T - 1, 1) * v.W)) || (rc = alloc(p, &v.rec_owed, 1)) ||
         (rc = alloc(p, &v.swap_jp, (size_t)std::max(v.T - 1, 1) * v.W)) ||
         (rc = alloc(p, &v.betas, (size_t)v.T)) || (rc = alloc(p, &v.accepted, TW)) || (rc = alloc(p, &v.sweep, 2)) ||
         (rc = alloc(p, &v.h_betas, S * v.T)) || (rc = alloc(p, &v.h_tsw, S * std::max(v.T - 1, 1))) ||
@@ -814,6 +855,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     EMP_TRY(hipMemcpy(v.betas, betas, (size_t)v.T * sizeof(double), hipMemcpyHostToDevice));
     EMP_TRY(hipMemset(v.accepted, 0, TW * sizeof(long long)));
     EMP_TRY(hipMemset(v.sweep, 0, 2 * sizeof(long long)));
+    EMP_TRY(hipMemset(v.rec_owed, 0, sizeof(int)));
     EMP_TRY(hipDeviceSynchronize());     // the engine stream does not wait for the null stream
 #undef EMP_TRY
     p->wpt = emp_pick_wpt(m, (int64_t)v.T * v.W2, d.precision);
@@ -860,6 +902,7 @@ extern "C" int emp_pt_run(EmpPT* p, int64_t nsweeps, int32_t use_graph) {
         EMP_HIP(hipGraphLaunch(ge, p->stream));
         s += n;
     }
+    if ((rc = launch_record(p, 0))) return rc;    // the run's last sweep
     EMP_HIP(hipEventRecord(e1, p->stream));
     EMP_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1141,6 +1184,7 @@ extern "C" int emp_pt_restore(EmpPT* p, const char* path) {
     EMP_HIP(hipMemcpy(v.betas, b.data(), b.size() * 8, hipMemcpyHostToDevice));
     EMP_HIP(hipMemcpy(v.accepted, acc.data(), acc.size() * 8, hipMemcpyHostToDevice));
     EMP_HIP(hipMemcpy(v.sweep, &sw, 8, hipMemcpyHostToDevice));
+    EMP_HIP(hipMemset(v.rec_owed, 0, sizeof(int)));
     const int64_t n = (int64_t)TW;
     hipLaunchKernelGGL(k_identity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, v.map, n);
     EMP_HIP(hipGetLastError());

@@ -229,3 +229,27 @@ def test_ladder_adaptation_matches_oracle():
         assert np.allclose(bh[n], want, rtol=1e-12, atol=0), n
         prev = bh[n]
     assert bh[-1][0] == 1.0 and bh[-1][-1] == 0.0 and np.all(np.diff(bh[-1]) < 0)
+
+
+def test_history_survives_set_state_between_runs():
+    """Each sweep's state is recorded by the next sweep's first pass or at the
+    end of the run; a new state loaded between runs must not overwrite the
+    record of the previous run's last sweep."""
+    from astroemperor_amd import PTSampler
+    from astroemperor_amd.simulation import test_init
+    sim = make_sim("51Peg", 1, parameterisation=1,
+                   conditions=[("Period 1", "limits", [4.0, 4.5]), ("Amplitude 1", "limits", [40, 70])])
+    m = sim.model
+    T, W = 2, 16
+    s = PTSampler(W, m.ndim__, m, ntemps=T, seed=5)
+    s.run_mcmc(test_init(m, T, W, np.random.default_rng(1)), nsweeps=40, capacity=60)
+    last = s.get_last_sample()
+    first_run = s.get_chain().copy()
+    assert np.array_equal(first_run[:, 39], last.coords)
+    assert np.array_equal(s.get_log_like()[:, 39], last.log_like)
+    p1 = test_init(m, T, W, np.random.default_rng(2))
+    s.run_mcmc(p1, nsweeps=20)                 # loads p1, then continues the history
+    chain = s.get_chain()
+    assert chain.shape[1] == 60
+    assert np.array_equal(chain[:, :40], first_run)
+    assert np.array_equal(chain[:, 59], s.get_last_sample().coords)
