@@ -203,10 +203,10 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
 
 // ---------------------------------------------------------------------------
 // Adjacent-temperature swaps (hottest pair first), swap statistics, ladder
-// adaptation and the sweep counter.  One workgroup.  With USE_LDS the map,
-// the per-position likelihoods and the pre-drawn pairings/uniforms are
-// staged in LDS; otherwise the round works on the map in global memory and
-// draws its randoms inline.
+// adaptation and the sweep counter.  One workgroup.  The pairings and
+// log-uniforms were drawn by the sweep's last prior pass.  With USE_LDS the
+// map, the per-position likelihoods and those randoms are staged in LDS;
+// otherwise the round works on them in global memory.
 // ---------------------------------------------------------------------------
 constexpr int SWAP_BLOCK = 1024;
 
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
         __threadfence();
     }
     __syncthreads();
-    pt_swap_round(pv, sweep, L, Mp, USE_LDS ? LU : pd.swap_lu, USE_LDS ? JP : pd.swap_jp, true, s_red, s_bet,
+    pt_swap_round(pv, sweep, L, Mp, USE_LDS ? LU : pd.swap_lu, USE_LDS ? JP : pd.swap_jp, !USE_LDS, s_red, s_bet,
                   SWAP_BLOCK);
     if (tid < T - 1) {
         const double na = s_red[tid][0], di = s_red[tid][1];

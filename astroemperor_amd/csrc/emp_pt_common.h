@@ -282,25 +282,26 @@ __device__ inline void pt_swap_stage(const PTView& pv, long long sweep, double* 
 
 // Adjacent-temperature swaps of one ensemble, hottest pair first, by one
 // workgroup of `nthr` threads.  L (likelihood by position) and Mp (the map)
-// are working copies: in LDS (pre == true, then LU/JP hold the pre-drawn
-// uniforms and partners of every pair) or in global memory (pre == false,
-// randoms drawn inline).  On return s_red[c] = (accepted, sum |log ratio|)
-// of pair (c, c+1).  Walker states never move; only map entries do.
+// are working copies; LU/JP hold the pre-drawn log-uniforms and partners of
+// every pair (h - 1) W + i, h = 1..T-1 (in LDS, or in global memory with
+// `global_mem`).  Once a pair is decided its slots are reused for its
+// statistics (LU: |log ratio|, 0 if infinite; JP: accepted), which are
+// summed per temperature pair after the last round, so a round carries only
+// the decision and the swap.  On return s_red[c] = (accepted, sum |log
+// ratio|) of pair (c, c+1).  Walker states never move; only map entries do.
 // bet: the ladder (LDS copy).
-__device__ inline void pt_swap_round(const PTView& pv, long long sweep, double* L, int* Mp, const double* LU,
-                                     const int* JP, bool pre, double (*s_red)[2], const double* bet, int nthr) {
-    const int tid = threadIdx.x, lane = tid & 63;
+__device__ inline void pt_swap_round(const PTView& pv, long long sweep, double* L, int* Mp, double* LU, int* JP,
+                                     bool global_mem, double (*s_red)[2], const double* bet, int nthr) {
+    (void)sweep;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
     const int T = pv.T, W = pv.W;
-    for (int e = tid; e < 2 * 64; e += nthr) (&s_red[0][0])[e] = 0.0;
-    __syncthreads();
     for (int h = T - 1; h >= 1; --h) {
         const int c = h - 1;
         const double dbeta = bet[c] - bet[h];
-        double nacc = 0.0, dist = 0.0;
         for (int i = tid; i < W; i += nthr) {
             const int e = (h - 1) * W + i;
-            const int jp = pre ? JP[e] : (int)swap_partner(pv, sweep, h, i);
-            const double lu = pre ? LU[e] : swap_logu(pv, sweep, h, i);
+            const int jp = JP[e];
+            const double lu = LU[e];
             const int ih = h * W + i, jc = c * W + jp;
             const double lh = L[ih], lc = L[jc];
             double lr = dbeta * (lh - lc);
@@ -313,17 +314,27 @@ __device__ inline void pt_swap_round(const PTView& pv, long long sweep, double* 
                 Mp[ih] = Mp[jc];
                 Mp[jc] = mh;
             }
-            nacc += acc ? 1.0 : 0.0;
-            if (lr > -__builtin_inf() && lr < __builtin_inf()) dist += fabs(lr);
+            LU[e] = (lr > -__builtin_inf() && lr < __builtin_inf()) ? fabs(lr) : 0.0;
+            JP[e] = acc ? 1 : 0;
         }
-        if (tid - lane < W) {          // whole waves only; waves without walkers add nothing
-            nacc = wave_sum(nacc);
-            dist = wave_sum(dist);
-            if (lane == 0) { atomicAdd(&s_red[c][0], nacc); atomicAdd(&s_red[c][1], dist); }
-        }
-        if (!pre) __threadfence();
+        if (global_mem) __threadfence();
         __syncthreads();
     }
+    for (int h = 1 + wave; h < T; h += nw) {      // one wave per temperature pair
+        double na = 0.0, di = 0.0;
+        for (int i = lane; i < W; i += 64) {
+            const int e = (h - 1) * W + i;
+            na += (double)JP[e];
+            di += LU[e];
+        }
+        na = wave_sum(na);
+        di = wave_sum(di);
+        if (lane == 0) {
+            s_red[h - 1][0] = na;
+            s_red[h - 1][1] = di;
+        }
+    }
+    __syncthreads();
 }
 
 // Vousden, Farr & Mandel (2016) ladder dynamics from this sweep's swap

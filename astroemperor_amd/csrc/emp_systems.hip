@@ -193,7 +193,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
         pt_swap_stage(pv, sweep, L, Mp, LU, JP, NT);
         for (int e = tid; e < T; e += NT) s_bet[e] = pv.betas[e];
         __syncthreads();
-        pt_swap_round(pv, sweep, L, Mp, LU, JP, true, s_red, s_bet, NT);
+        pt_swap_round(pv, sweep, L, Mp, LU, JP, false, s_red, s_bet, NT);
         for (int e = tid; e < TW; e += NT) pv.map[e] = Mp[e];
         if (tid < T - 1) {
             const double r = s_red[tid][0] / W;
