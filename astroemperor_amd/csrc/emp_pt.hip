@@ -136,11 +136,15 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd,
         }
     }
     if (b >= pd.T * pd.W2) return;            // whole waves
+    // What does not depend on the random draws is loaded first, so that it
+    // is in flight while the counter is read and Philox runs.
+    const int t = b / pd.W2, k = half * pd.W2 + b % pd.W2;
+    const int sk = pd.map[t * pd.W + k];
+    const double beta = pd.betas[t], lnl_old = pd.lnl[sk], lnp_old = pd.lnp[sk];
     const Proposal pr = pt_propose(pv, *pd.sweep, half, step, b);
     const double* Xk = pd.X + (size_t)pr.sk * pd.D;
     const double* Xj = pd.X + (size_t)pr.sj * pd.D;
-    const double beta = pd.betas[pr.t], lnl_old = pd.lnl[pr.sk], lnp_old = pd.lnp[pr.sk];
-    const double lp = prior_proposal_group<64>(md, Xk, Xj, pr.z, lane, 0);
+    const double lp = prior_proposal_wave(md, Xk, Xj, pr.z, lane);
     const bool live = lp > -__builtin_inf();
     if (live) {
         double* row = pd.prow + (size_t)b * pd.D;
@@ -541,7 +545,7 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_propose(ModelDev md, PTDev p
         if (j < D) s_row[wave][j] = y[h];
     }
     __threadfence_block();     // the row is read back across lanes
-    const double lp = prior_proposal_group<64>(md, s_row[wave], s_row[wave], 0.0, lane, 0);
+    const double lp = prior_proposal_wave(md, s_row[wave], s_row[wave], 0.0, lane);
     const bool live = lp > -__builtin_inf();
     double* yp = pd.lg_Xp + (size_t)b * D;
     for (int h = 0; h < 2; ++h) {

@@ -75,21 +75,21 @@ __device__ __forceinline__ double pt_prop_value(const double* Xk, const double* 
 }
 
 // Prior of a proposal, in the order of prior_seq, read straight from the two
-// walkers' rows, by a group of G lanes (a power of two dividing 64; gl =
-// lane within the group, g0 = the group's first lane of the wave): the lanes
-// evaluate the terms of the summation sequence md.pseq in parallel, with
-// their loads in flight together, then every lane adds them up in sequence
-// order with prior_seq's -inf return, so the sum is bit-identical.
-template <int G>
-__device__ inline double prior_proposal_group(const ModelDev& md, const double* Xk, const double* Xj, double z,
-                                              int gl, int g0) {
+// walkers' rows, by one whole wave: the lanes evaluate the terms of the
+// summation sequence md.pseq in parallel, with their loads in flight
+// together, then the terms are added up in sequence order (a scalar read of
+// each lane) with prior_seq's -inf return at the block ends, which a ballot
+// marks, so the sum is bit-identical.
+__device__ inline double prior_proposal_wave(const ModelDev& md, const double* Xk, const double* Xj, double z,
+                                             int lane) {
 #pragma clang fp contract(off)
     double lp = 0.0;
-    for (int i0 = 0; i0 < md.npseq; i0 += G) {
-        const int i = i0 + gl;
+    for (int i0 = 0; i0 < md.npseq; i0 += 64) {
+        const int i = i0 + lane;
         double v = 0.0;
+        int q = 0;
         if (i < md.npseq) {
-            const int q = md.pseq[i];
+            q = md.pseq[i];
             if (q >= 0) {
                 v = prior_term(md.prior[q], pt_prop_value(Xk, Xj, z, md.inv_free[q]));
             } else if (q <= PSEQ_ECC) {
@@ -102,21 +102,13 @@ __device__ inline double prior_proposal_group(const ModelDev& md, const double* 
                 v = prior_term(md.ecc_prior[b], x);
             }
         }
-        const int n = min(G, md.npseq - i0);
+        const unsigned long long chk = __ballot(i < md.npseq && q == PSEQ_CHECK);
+        const int n = min(64, md.npseq - i0);
         for (int k = 0; k < n; ++k) {
-            double vk;
-            if (G == 64) {   // k is uniform: a scalar read of lane k
-                const long long bits = __double_as_longlong(v);
-                const int lo = __builtin_amdgcn_readlane((int)bits, k);
-                const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
-                vk = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-            } else {
-                vk = __shfl(v, g0 + k, 64);
-            }
-            if (md.pseq[i0 + k] == PSEQ_CHECK) {
+            if ((chk >> k) & 1ull) {
                 if (lp == -__builtin_inf()) return lp;
             } else {
-                lp += vk;
+                lp += readlane_f64(v, k);
             }
         }
     }
