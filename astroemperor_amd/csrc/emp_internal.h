@@ -23,6 +23,18 @@ constexpr int TILE = 512;        // epochs staged in LDS per tile
 constexpr int PSEQ_CHECK = -1;
 constexpr int PSEQ_ECC = -2;
 
+// One entry of the prior's summation sequence, resolved on the host: the
+// term and where its argument comes from (free indices, or fixed values for
+// the S, C of a derived eccentricity), so that a lane goes straight from its
+// entry to the walker's row.
+struct PseqEntry {
+    EmpPriorTerm term;
+    double va, vb;        // fixed values of S and C (eccentricity entries whose S or C is fixed)
+    int kind;             // 0: parameter prior (fa), 1: eccentricity S^2 + C^2 (fa, fb), 2: -inf check
+    int fa, fb;           // free indices, -1 when fixed
+    int pad;
+};
+
 // Everything a kernel needs to know about a model, passed by value.
 struct ModelDev {
     int N, I, K, param, use_c, acc, has_jit;
@@ -43,6 +55,7 @@ struct ModelDev {
     const int* block_start;           // [nblocks + 1]
     const EmpPriorTerm* ecc_prior;    // [K]
     const int* pseq;                  // [npseq] prior terms in summation order (see PSEQ_*)
+    const PseqEntry* pseq_e;          // [npseq] the same, resolved (single-model engines; null in batches)
     int npseq;
     const double2* sctab;             // [SC_N + 1] (sin, cos) of k pi / SC_N
     double lnl_const;                 // -0.5 ln(2 pi) N
@@ -85,6 +98,7 @@ struct ModelHost {
     std::vector<int> order;   // device position -> caller position
     std::vector<double> t, y, s2, dt, fixed;
     std::vector<int> ins, inv, free_idx, bstart, pseq;
+    std::vector<emp::PseqEntry> pseq_e;
     std::vector<EmpPriorTerm> prior, ecc;
 };
 int emp_model_host(const EmpModelDesc* d, ModelHost* h);

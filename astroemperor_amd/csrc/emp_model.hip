@@ -295,6 +295,27 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
         h->pseq.push_back(PSEQ_CHECK);
         if (b < d->n_planets && h->ecc[b].kind != EMP_PRIOR_NONE) h->pseq.push_back(PSEQ_ECC - b);
     }
+    h->pseq_e.assign(h->pseq.size(), PseqEntry{});
+    for (size_t i = 0; i < h->pseq.size(); ++i) {
+        const int q = h->pseq[i];
+        PseqEntry& e = h->pseq_e[i];
+        e.fa = e.fb = -1;
+        if (q >= 0) {
+            e.kind = 0;
+            e.term = h->prior[q];
+            e.fa = h->inv[q];
+        } else if (q <= PSEQ_ECC) {
+            const int b = PSEQ_ECC - q, o = d->off_kep + 5 * b + 3;
+            e.kind = 1;
+            e.term = h->ecc[b];
+            e.fa = h->inv[o];
+            e.fb = h->inv[o + 1];
+            e.va = d->fixed_value[o];
+            e.vb = d->fixed_value[o + 1];
+        } else {
+            e.kind = 2;
+        }
+    }
     ModelDev& md = h->dev;
     md = ModelDev{};
     md.N = N; md.I = d->n_ins; md.K = d->n_planets; md.param = d->parameterisation; md.use_c = d->use_c;
@@ -369,6 +390,7 @@ extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
     EMP_UP(h.bstart, md.block_start);
     EMP_UP(h.ecc, md.ecc_prior);
     EMP_UP(h.pseq, md.pseq);
+    EMP_UP(h.pseq_e, md.pseq_e);
     {
         std::vector<double2> tab;
         emp_sincos_table(tab);

@@ -87,22 +87,20 @@ __device__ inline double prior_proposal_wave(const ModelDev& md, const double* X
     for (int i0 = 0; i0 < md.npseq; i0 += 64) {
         const int i = i0 + lane;
         double v = 0.0;
-        int q = 0;
+        int kind = 0;
         if (i < md.npseq) {
-            q = md.pseq[i];
-            if (q >= 0) {
-                v = prior_term(md.prior[q], pt_prop_value(Xk, Xj, z, md.inv_free[q]));
-            } else if (q <= PSEQ_ECC) {
-                const int b = PSEQ_ECC - q;
-                const int o = md.off_kep + 5 * b + 3;
-                const int f3 = md.inv_free[o], f4 = md.inv_free[o + 1];
-                const double s3 = f3 >= 0 ? pt_prop_value(Xk, Xj, z, f3) : md.fixed_val[o];
-                const double s4 = f4 >= 0 ? pt_prop_value(Xk, Xj, z, f4) : md.fixed_val[o + 1];
+            const PseqEntry& pe = md.pseq_e[i];
+            kind = pe.kind;
+            if (kind == 0) {
+                v = prior_term(pe.term, pt_prop_value(Xk, Xj, z, pe.fa));
+            } else if (kind == 1) {
+                const double s3 = pe.fa >= 0 ? pt_prop_value(Xk, Xj, z, pe.fa) : pe.va;
+                const double s4 = pe.fb >= 0 ? pt_prop_value(Xk, Xj, z, pe.fb) : pe.vb;
                 const double x = s3 * s3 + s4 * s4;
-                v = prior_term(md.ecc_prior[b], x);
+                v = prior_term(pe.term, x);
             }
         }
-        const unsigned long long chk = __ballot(i < md.npseq && q == PSEQ_CHECK);
+        const unsigned long long chk = __ballot(i < md.npseq && kind == 2);
         const int n = min(64, md.npseq - i0);
         for (int k = 0; k < n; ++k) {
             if ((chk >> k) & 1ull) {
