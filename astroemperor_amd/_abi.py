@@ -52,6 +52,7 @@ class EmpMoveDesc(C.Structure):
 
 
 EMP_MOVE_STRETCH, EMP_MOVE_GRADIENT = 0, 1
+EMP_GRAD_MIXED = 2
 
 
 class EmpPTDesc(C.Structure):

@@ -389,26 +389,37 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
     }
 }
 
-// Planet term at time t.  PREC 0: FP64 Markley; PREC 1: mixed, with `kt` the
-// planet's per-vector table (read only when pc.ktab is set, a per-vector,
-// hence wave-uniform, branch).
+// (sin E, cos E) of the planet at time t.  PREC 0: FP64 Markley; PREC 1:
+// mixed, with `kt` the planet's per-vector table (read only when pc.ktab is
+// set, a per-vector, hence wave-uniform, branch).
 template <int PREC>
-__device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t) {
-    double sE, cE;
-    if (PREC == 0) {
-        const double M = reduce_2pi(planet_M(pc, t));
-        kepler_fp64(M, pc, tb, sE, cE);
-        const double den = __builtin_fma(-pc.e, cE, 1.0);
-        return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64(den);
-    }
+__device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
+                                          double& sE, double& cE) {
     const double M = reduce_2pi(planet_M(pc, t));
+    if (PREC == 0) {
+        kepler_fp64(M, pc, tb, sE, cE);
+        return;
+    }
     const double Ma = __builtin_fabs(M);
     const float Mf = (float)Ma;
     if (pc.ktab) kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
     else kepler_fp64_step<true>(Ma, kepler_fp32_start(Mf, pc), pc.e, tb, sE, cE);
     if (M < 0.0) sE = -sE;
+}
+
+// Reciprocal of the path: two Newton steps (FP64) or one (mixed).
+template <int PREC>
+__device__ __forceinline__ double rcp_path(double x) {
+    return PREC == 0 ? rcp64(x) : rcp64_1(x);
+}
+
+// Planet term at time t (see planet_sc).
+template <int PREC>
+__device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t) {
+    double sE, cE;
+    planet_sc<PREC>(pc, tb, kt, t, sE, cE);
     const double den = __builtin_fma(-pc.e, cE, 1.0);
-    return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp64_1(den);
+    return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp_path<PREC>(den);
 }
 
 // ----------------------------------------------------------------------------

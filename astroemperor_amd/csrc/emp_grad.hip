@@ -51,6 +51,9 @@ struct EpochDeriv {
     double dM[KMAX], de[KMAX], dw[KMAX], dA[KMAX], tau[KMAX];
 };
 
+// PREC 0: FP64 Kepler solve; PREC 1: the mixed path's (FP32 start, one FP64
+// step; the per-vector Kepler tables must be in sl.ktab).
+template <int PREC>
 __device__ __forceinline__ void epoch_deriv(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
                                             int k, EpochDeriv& d) {
     d.mu = sl.gam[k];
@@ -65,11 +68,10 @@ __device__ __forceinline__ void epoch_deriv(const ModelDev& md, const SlotLDS& s
         d.dM[p] = d.de[p] = d.dw[p] = d.dA[p] = d.tau[p] = 0.0;
         if (p < md.K) {
             const PlanetC& pc = sl.planet[p];
-            const double M = reduce_2pi(planet_M(pc, t));
             double sE, cE;
-            kepler_fp64(M, pc, tab, sE, cE);
+            planet_sc<PREC>(pc, tab, sl.ktab[p], t, sE, cE);
             const double e = pc.e;
-            const double rD = rcp64(__builtin_fma(-e, cE, 1.0));
+            const double rD = rcp_path<PREC>(__builtin_fma(-e, cE, 1.0));
             const double bq = sqrt((1.0 - e) * (1.0 + e));
             const double cosf = (cE - e) * rD, sinf = bq * sE * rD;
             const double cfw = cosf * pc.cw - sinf * pc.sw;
@@ -115,7 +117,7 @@ __device__ __forceinline__ double ma_mult(const ModelDev& md, const SlotLDS& sl,
     return col < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-col / sl.ma_tau[k]));
 }
 
-template <int WPT>
+template <int PREC, int WPT>
 __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                     int64_t ld, double* __restrict__ out_lnl,
                                                     double* __restrict__ out_grad,
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     for (int j = lane; j < WN_N; j += 64) s_gw.nat[wave][j] = 0.0;
     tab_load(md, s_tile);
     __syncthreads();
-    slot_prepare<0, WPT>(md, sl, s_tile, live, wsub, lane);
+    slot_prepare<PREC, WPT>(md, sl, s_tile, live, wsub, lane);
 
     double acc_nat[KMAX][5];
 #pragma unroll
@@ -167,10 +169,10 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 const double t = s_tile.t[j];
                 const int k = s_tile.ins[j];
                 EpochDeriv d;
-                epoch_deriv(md, sl, s_tile.tab, t, k, d);
+                epoch_deriv<PREC>(md, sl, s_tile.tab, t, k, d);
                 const double err2 = s_tile.s2[j] + sl.jit2[k];
                 const double r = s_tile.y[j] - d.mu;
-                const double ie = rcp64(err2);
+                const double ie = rcp_path<PREC>(err2);
                 const double w = r * ie;
                 chi = __builtin_fma(r * r, ie, chi);
                 prod *= err2;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     } else {
         // Pass 1: the likelihood, storing eps.
         double* eps = eps_scratch + (live ? b : 0) * (int64_t)md.N;
-        lnl_v = block_lnl<0, WPT, false, true>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr, false, eps);
+        lnl_v = block_lnl<PREC, WPT, false, true>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr, false, eps);
         __syncthreads();   // eps rows written by every wave of the slot
         // Pass 2: epochs in reverse, lambda by suffix scans.
         double lam_carry = 0.0;
@@ -213,9 +215,9 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 if (valid) {
                     t = s_tile.t[j];
                     k = s_tile.ins[j];
-                    epoch_deriv(md, sl, s_tile.tab, t, k, d);
+                    epoch_deriv<PREC>(md, sl, s_tile.tab, t, k, d);
                     const double err2 = s_tile.s2[j] + sl.jit2[k];
-                    ie = rcp64(err2);
+                    ie = rcp_path<PREC>(err2);
                     e_i = eps[tb + j];
                     u = -e_i * ie;
                     const int n1 = tb + j + 1;     // the next epoch's multiplier
@@ -362,13 +364,13 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     }
 }
 
-template <int WPT>
+template <int PREC, int WPT>
 static void launch_grad_t(const ModelDev& md, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
                           unsigned long long* nf, double* scratch, hipStream_t s) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t grid = (B + SLOTS - 1) / SLOTS;
-    hipLaunchKernelGGL((k_lnl_grad<WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, lnl, grad, nf,
-                       scratch);
+    hipLaunchKernelGGL((k_lnl_grad<PREC, WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, lnl, grad,
+                       nf, scratch);
 }
 
 // The MA adjoint keeps one row of eps per vector: a model-owned scratch of
@@ -389,22 +391,24 @@ static int ensure_grad_scratch(EmpModel* m, int64_t B) {
 }
 
 int emp_launch_grad_ws(const EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
-                       double* eps_scratch, hipStream_t s) {
+                       double* eps_scratch, hipStream_t s, int precision) {
     if (B <= 0) return EMP_OK;
     if (m->dev.ma_active && !eps_scratch) { emp_set_error("moving-average gradient needs a scratch buffer"); return EMP_EINVAL; }
-    int wpt = emp_pick_wpt(m, B, EMP_FP64);
-    if (wpt == 1) launch_grad_t<1>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
-    else if (wpt == 2) launch_grad_t<2>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
-    else if (wpt == 4) launch_grad_t<4>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
-    else launch_grad_t<8>(m->dev, th, B, ld, lnl, grad, m->d_nonfinite, eps_scratch, s);
+    const int wpt = emp_pick_wpt(m, B, precision);
+    unsigned long long* nf = m->d_nonfinite;
+#define EMP_CASE(P, W) if (precision == P && wpt == W) launch_grad_t<P, W>(m->dev, th, B, ld, lnl, grad, nf, eps_scratch, s); else
+    EMP_CASE(0, 1) EMP_CASE(0, 2) EMP_CASE(0, 4) EMP_CASE(0, 8)
+    EMP_CASE(1, 1) EMP_CASE(1, 2) EMP_CASE(1, 4) EMP_CASE(1, 8) { emp_set_error("bad precision/wpt"); return EMP_EINVAL; }
+#undef EMP_CASE
     EMP_HIP(hipGetLastError());
     return EMP_OK;
 }
 
-int emp_launch_grad(EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad, hipStream_t s) {
+int emp_launch_grad(EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad, hipStream_t s,
+                    int precision) {
     int rc = ensure_grad_scratch(m, B);
     if (rc) return rc;
-    return emp_launch_grad_ws(m, th, B, ld, lnl, grad, m->d_grad_scratch, s);
+    return emp_launch_grad_ws(m, th, B, ld, lnl, grad, m->d_grad_scratch, s, precision);
 }
 
 extern "C" int emp_lnl_grad_batch(EmpModel* m, const double* theta, int64_t B, int64_t ld, double* lnl,
@@ -415,7 +419,8 @@ extern "C" int emp_lnl_grad_batch(EmpModel* m, const double* theta, int64_t B, i
     if (emp_fault_injected()) return EMP_EFAULT_INJECTED;
     if (B == 0) return EMP_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (!(flags & EMP_HOST_POINTERS)) return emp_launch_grad(m, theta, B, ld, lnl, grad, s);
+    const int prec = (flags & EMP_GRAD_MIXED) ? EMP_MIXED : EMP_FP64;
+    if (!(flags & EMP_HOST_POINTERS)) return emp_launch_grad(m, theta, B, ld, lnl, grad, s, prec);
     const size_t nin = (size_t)B * ld, D = (size_t)m->desc.ndim;
     double *din = nullptr, *dl = nullptr, *dg = nullptr;
     EMP_HIP(hipMalloc(&din, nin * 8));
@@ -423,7 +428,7 @@ extern "C" int emp_lnl_grad_batch(EmpModel* m, const double* theta, int64_t B, i
     EMP_HIP(hipMalloc(&dg, std::max<size_t>((size_t)B * D, 1) * 8));
     int rc = EMP_OK;
     if (hipMemcpyAsync(din, theta, nin * 8, hipMemcpyHostToDevice, s) != hipSuccess) rc = EMP_EHIP;
-    if (!rc) rc = emp_launch_grad(m, din, B, ld, dl, dg, s);
+    if (!rc) rc = emp_launch_grad(m, din, B, ld, dl, dg, s, prec);
     if (!rc && hipMemcpyAsync(lnl, dl, (size_t)B * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = EMP_EHIP;
     if (!rc && D && hipMemcpyAsync(grad, dg, (size_t)B * D * 8, hipMemcpyDeviceToHost, s) != hipSuccess) rc = EMP_EHIP;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = EMP_EHIP;

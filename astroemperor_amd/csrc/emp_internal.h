@@ -123,8 +123,8 @@ int emp_launch_prior(const EmpModel* m, const double* d_theta, int64_t B, int64_
                      double* d_lnp, hipStream_t s);
 // lnL and its gradient (emp_grad.hip); grows the model's MA scratch if needed.
 int emp_launch_grad(EmpModel* m, const double* d_theta, int64_t B, int64_t ld, double* d_lnl,
-                    double* d_grad, hipStream_t s);
+                    double* d_grad, hipStream_t s, int precision = EMP_FP64);
 // The same with a caller-owned scratch of B x N doubles (used only when the
 // moving average is active); allocates nothing, so it may be captured.
 int emp_launch_grad_ws(const EmpModel* m, const double* d_theta, int64_t B, int64_t ld, double* d_lnl,
-                       double* d_grad, double* eps_scratch, hipStream_t s);
+                       double* d_grad, double* eps_scratch, hipStream_t s, int precision = EMP_FP64);

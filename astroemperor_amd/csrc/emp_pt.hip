@@ -707,7 +707,7 @@ static int launch_grad_sweep(EmpPT* p) {
     const unsigned tgrid = (unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES);
     int rc = launch_record(p, 1);
     if (rc) return rc;
-    rc = emp_launch_grad_ws(p->model, d.X, TW, d.D, d.lg_l, d.lg_g, d.lg_scr, p->stream);
+    rc = emp_launch_grad_ws(p->model, d.X, TW, d.D, d.lg_l, d.lg_g, d.lg_scr, p->stream, p->desc.precision);
     if (rc) return rc;
     for (int st = 0; st < p->desc.nsteps; ++st)
         for (int h = 0; h < 2; ++h) {
@@ -715,7 +715,8 @@ static int launch_grad_sweep(EmpPT* p) {
             EMP_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_lg_propose, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, md, d, h, st);
             EMP_HIP(hipGetLastError());
-            rc = emp_launch_grad_ws(p->model, d.lg_Xp, tasks, d.D, d.lg_lp, d.lg_gp, d.lg_scr, p->stream);
+            rc = emp_launch_grad_ws(p->model, d.lg_Xp, tasks, d.D, d.lg_lp, d.lg_gp, d.lg_scr, p->stream,
+                                    p->desc.precision);
             if (rc) return rc;
             hipLaunchKernelGGL(k_lg_accept, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, d, h, st);
             EMP_HIP(hipGetLastError());

@@ -221,6 +221,7 @@ class ReddModel:
     def _batch(self, fn, theta, precision=None, grad=False):
         """theta: numpy [B, D] (host) or a torch CUDA tensor [B, D] (device)."""
         L = _abi.lib()
+        grad_flags = _abi.EMP_GRAD_MIXED if (grad and self._prec(precision) == _abi.EMP_MIXED) else 0
         h = self.device.handle
         is_t = type(theta).__module__.startswith("torch")
         if is_t:
@@ -234,7 +235,7 @@ class ReddModel:
             args = [h, C.c_void_p(th.data_ptr()), B, ld, C.c_void_p(out.data_ptr())]
             if grad:
                 g = torch.empty(B, self.ndim__, dtype=torch.float64, device=th.device)
-                _abi.check(fn(*args, C.c_void_p(g.data_ptr()), 0, stream))
+                _abi.check(fn(*args, C.c_void_p(g.data_ptr()), grad_flags, stream))
                 return out, g
             if precision is not None:
                 args.append(precision)
@@ -246,7 +247,7 @@ class ReddModel:
         args = [h, th.ctypes.data, B, ld, out.ctypes.data]
         if grad:
             g = np.empty((B, self.ndim__))
-            _abi.check(fn(*args, g.ctypes.data, _abi.EMP_HOST_POINTERS, None))
+            _abi.check(fn(*args, g.ctypes.data, _abi.EMP_HOST_POINTERS | grad_flags, None))
             return out, g
         if precision is not None:
             args.append(precision)
@@ -267,8 +268,10 @@ class ReddModel:
     def lnprior_batch(self, theta):
         return self._batch(_abi.lib().emp_lnprior_batch, theta)
 
-    def lnl_grad_batch(self, theta):
-        return self._batch(_abi.lib().emp_lnl_grad_batch, theta, grad=True)
+    def lnl_grad_batch(self, theta, precision="fp64"):
+        """lnL and d lnL / d theta (adjoint kernel); precision "mixed" uses the
+        mixed path's Kepler solve."""
+        return self._batch(_abi.lib().emp_lnl_grad_batch, theta, precision=precision, grad=True)
 
     def set_waves_per_theta(self, w):
         _abi.check(_abi.lib().emp_model_set_waves_per_theta(self.device.handle, int(w)))

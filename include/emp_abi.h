@@ -51,6 +51,7 @@ extern "C" {
 
 /* Flags. */
 #define EMP_HOST_POINTERS 1
+#define EMP_GRAD_MIXED 2    /* emp_lnl_grad_batch: the mixed path's Kepler solve (EMP_MIXED accuracy) */
 
 /* Precision of the likelihood path. */
 #define EMP_FP64 0          /* every operation FP64: the drop-in path */
@@ -126,7 +127,8 @@ int emp_lnprior_batch(EmpModel* model, const double* theta, int64_t B, int64_t l
 int emp_model_eval(EmpModel* model, const double* theta, double* model_out, double* err2_out);
 
 /* No reference counterpart: lnL and dlnL/dtheta[free]
- * for a batch, by a hand-written adjoint kernel (FP64). grad is [B, ndim].
+ * for a batch, by a hand-written adjoint kernel (FP64; with EMP_GRAD_MIXED the
+ * Kepler solve of the mixed path, FP32 start and one FP64 step). grad is [B, ndim].
  * With an active MA(1) term the model keeps a device scratch of B x n_epochs
  * doubles (grown on demand, freed with the model). */
 int emp_lnl_grad_batch(EmpModel* model, const double* theta, int64_t B, int64_t ld,

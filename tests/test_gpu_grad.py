@@ -156,3 +156,22 @@ def test_adjoint_ma1(case):
     assert worst_ad < 1e-8
     assert judged >= 0.4 * g.size
     assert worst_fd < 1e-6
+
+
+@pytest.mark.parametrize("ma", [False, True])
+def test_mixed_adjoint_matches_fp64(ma):
+    """The mixed path's adjoint (FP32 Kepler start, one FP64 step; table and
+    Markley starts both hit, e up to 0.8 at 752 epochs) against the FP64
+    adjoint: lnL within 1e-6 absolute, gradient within 1e-6 of max(|g|, 1)."""
+    kw = {"moav": {"order": 1, "global": False}, "semantics": "intended"} if ma else {}
+    sim = make_sim("GJ876", 2, parameterisation=1, **kw)
+    m = sim.model
+    th = _draws(m, 96, np.random.default_rng(11 + ma))
+    l64, g64 = m.lnl_grad_batch(th)
+    lm, gm = m.lnl_grad_batch(th, "mixed")
+    assert np.allclose(lm, m.lnl_batch(th, "mixed"), rtol=1e-12, atol=0)
+    dl = np.max(np.abs(lm - l64))
+    dg = np.max(np.abs(gm - g64) / np.maximum(np.abs(g64), 1.0))
+    print(f"ma={ma}: |dlnL| {dl:.2e}, gradient {dg:.2e}", flush=True)
+    assert dl < 1e-6
+    assert dg < 1e-6
