@@ -19,6 +19,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "emp_pt_common.h"
 
 using namespace emp;
@@ -636,31 +638,45 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_accept(PTDev pd, int half, i
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
+// With events, the launch itself records them (start and end of the
+// kernel's execution, as a kernel trace measures it).
 template <int PREC, int WPT>
-static void launch_half_t(const ModelDev& md, const PTDev& pd, int half, int step, hipStream_t s) {
+static void launch_half_t(const ModelDev& md, const PTDev& pd, int half, int step, hipStream_t s,
+                          hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t tasks = (int64_t)pd.T * pd.W2;
     const int64_t grid = (tasks + SLOTS - 1) / SLOTS;
-    hipLaunchKernelGGL((k_pt_half<PREC, WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, pd, half, step);
+    if (e0)
+        hipExtLaunchKernelGGL((k_pt_half<PREC, WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, e0, e1, 0, md, pd,
+                              half, step);
+    else
+        hipLaunchKernelGGL((k_pt_half<PREC, WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, pd, half, step);
 }
 
 // draw_swaps: this prior pass also draws the sweep's swap randoms (the last
 // half-step of a sweep).
-static int launch_half(EmpPT* p, int half, int step, int draw_swaps, int first = 0) {
+static int launch_prior(EmpPT* p, int half, int step, int draw_swaps, int first) {
+    const int64_t tasks = (int64_t)p->dev.T * p->dev.W2;
+    hipLaunchKernelGGL(k_pt_prior, dim3((unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES)), dim3(PRIOR_BLOCK), 0,
+                       p->stream, p->model->dev, p->dev, half, step, draw_swaps, first);
+    EMP_HIP(hipGetLastError());
+    return EMP_OK;
+}
+
+static int launch_lik(EmpPT* p, int half, int step, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const ModelDev& md = p->model->dev;
     const int prec = p->desc.precision, w = p->wpt;
-    {
-        const int64_t tasks = (int64_t)p->dev.T * p->dev.W2;
-        hipLaunchKernelGGL(k_pt_prior, dim3((unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES)), dim3(PRIOR_BLOCK), 0,
-                           p->stream, md, p->dev, half, step, draw_swaps, first);
-        EMP_HIP(hipGetLastError());
-    }
-#define EMP_CASE(P, W) if (prec == P && w == W) { launch_half_t<P, W>(md, p->dev, half, step, p->stream); EMP_HIP(hipGetLastError()); return EMP_OK; }
+#define EMP_CASE(P, W) if (prec == P && w == W) { launch_half_t<P, W>(md, p->dev, half, step, p->stream, e0, e1); EMP_HIP(hipGetLastError()); return EMP_OK; }
     EMP_CASE(0, 1) EMP_CASE(0, 2) EMP_CASE(0, 4) EMP_CASE(0, 8)
     EMP_CASE(1, 1) EMP_CASE(1, 2) EMP_CASE(1, 4) EMP_CASE(1, 8)
 #undef EMP_CASE
     emp_set_error("bad precision/wpt");
     return EMP_EINVAL;
+}
+
+static int launch_half(EmpPT* p, int half, int step, int draw_swaps, int first = 0) {
+    const int rc = launch_prior(p, half, step, draw_swaps, first);
+    return rc ? rc : launch_lik(p, half, step);
 }
 
 // Swap round (with ladder and sweep counter): the end of every sweep,
@@ -919,19 +935,35 @@ extern "C" int emp_pt_run(EmpPT* p, int64_t nsweeps, int32_t use_graph) {
     return EMP_OK;
 }
 
-// Times `n` half-step kernel launches individually with events (no graph).
-// The launches advance the chain like ordinary half-steps of sweep 0's step 0.
+// Runs ceil(n / 2) stretch sweeps one launch at a time (no graph) and times
+// the first n likelihood kernels with events recorded by their own launches:
+// the measured launches are those of ordinary sweeps, which advance the chain
+// and its history.
 extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     if (!p || n < 1 || !ms_avg) { emp_set_error("bad argument"); return EMP_EINVAL; }
+    const int64_t ns = ((int64_t)n + 2 * p->desc.nsteps - 1) / (2 * p->desc.nsteps);
+    if (p->host_sweeps + ns > p->dev.max_sweeps) {
+        emp_set_error("history capacity %lld sweeps exceeded (%lld done, %lld needed to time %d half-steps)",
+                      (long long)p->dev.max_sweeps, p->host_sweeps, (long long)ns, n);
+        return EMP_EINVAL;
+    }
     std::vector<hipEvent_t> ev(2 * (size_t)n);
     for (auto& e : ev) EMP_HIP(hipEventCreate(&e));
-    for (int i = 0; i < n; ++i) {
-        EMP_HIP(hipEventRecord(ev[2 * i], p->stream));
-        int rc = launch_half(p, i & 1, 0, 0);
-        if (rc) return rc;
-        EMP_HIP(hipEventRecord(ev[2 * i + 1], p->stream));
+    int k = 0, rc = EMP_OK;
+    for (int64_t sw = 0; sw < ns && !rc; ++sw) {
+        for (int st = 0; st < p->desc.nsteps && !rc; ++st)
+            for (int h = 0; h < 2 && !rc; ++h) {
+                rc = launch_prior(p, h, st, h == 1 && st == p->desc.nsteps - 1, h == 0 && st == 0);
+                if (rc) break;
+                rc = k < n ? launch_lik(p, h, st, ev[2 * k], ev[2 * k + 1]) : launch_lik(p, h, st);
+                ++k;
+            }
+        if (!rc) rc = launch_tail(p);
     }
+    if (!rc) rc = launch_record(p, 0);
+    if (rc) return rc;
     EMP_HIP(hipStreamSynchronize(p->stream));
+    p->host_sweeps += ns;
     double tot = 0.0;
     for (int i = 0; i < n; ++i) {
         float ms = 0.f;
@@ -944,16 +976,22 @@ extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     return EMP_OK;
 }
 
-// Times the half-step (both kernels) at 1, 2, 4 and 8 waves per walker on a
-// copy of the state, restores the state and keeps the fastest shape.
+// Times graph-replayed stretch sweeps at 1, 2, 4 and 8 waves per walker on a
+// copy of the state (positions, likelihoods, map, ladder, counters),
+// restores the state after each shape and keeps the fastest.  Each shape: an
+// 8-sweep graph, replayed once untimed and `trials` times timed.  History
+// rows the trials write lie at or after the current sweep and are written
+// again by the run that follows.
 extern "C" int emp_pt_autotune(EmpPT* p, int32_t trials, int32_t* best_wpt, double* best_ms) {
     if (!p || trials < 1) { emp_set_error("bad argument"); return EMP_EINVAL; }
     PTDev& v = p->dev;
     const size_t TW = (size_t)v.T * v.W;
-    const std::pair<void*, size_t> items[] = {{v.X, TW * v.D * 8}, {v.lnl, TW * 8}, {v.lnp, TW * 8},
-                                              {v.accepted, TW * 8}, {v.sweep, 16}};
+    const std::pair<void*, size_t> items[] = {{v.X, TW * v.D * 8}, {v.lnl, TW * 8},      {v.lnp, TW * 8},
+                                              {v.accepted, TW * 8}, {v.sweep, 16},         {v.map, TW * 4},
+                                              {v.betas, (size_t)v.T * 8}, {v.rec_owed, sizeof(int)}};
     std::vector<void*> copies;
     auto release = [&]() { for (void* q : copies) (void)hipFree(q); };
+    EMP_HIP(hipStreamSynchronize(p->stream));
     for (const auto& it : items) {
         void* q = nullptr;
         hipError_t e = hipMalloc(&q, it.second);
@@ -964,24 +1002,40 @@ extern "C" int emp_pt_autotune(EmpPT* p, int32_t trials, int32_t* best_wpt, doub
             return emp_hip_fail(e, "snapshot", __FILE__, __LINE__);
         }
     }
+    constexpr int NS = 8;
     const int keep = p->wpt;
     int best = keep;
     double best_t = -1.0;
     int rc = EMP_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "event", __FILE__, __LINE__);
     for (int w = 1; w <= 8 && !rc; w *= 2) {
         p->wpt = w;
-        double ms = 0.0;
-        rc = emp_pt_time_halfstep(p, 2 * trials, &ms);
+        drop_graphs(p);
+        hipGraphExec_t g = nullptr;
+        rc = capture_sweeps(p, 0, NS, &g);
+        float ms = 0.f;
+        if (!rc && hipGraphLaunch(g, p->stream) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "graph", __FILE__, __LINE__);
+        if (!rc && hipEventRecord(e0, p->stream) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "event", __FILE__, __LINE__);
+        for (int i = 0; i < trials && !rc; ++i)
+            if (hipGraphLaunch(g, p->stream) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "graph", __FILE__, __LINE__);
+        if (!rc && (hipEventRecord(e1, p->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = emp_hip_fail(hipGetLastError(), "timing", __FILE__, __LINE__);
+        if (g) (void)hipGraphExecDestroy(g);
         for (size_t i = 0; i < copies.size() && !rc; ++i)
             if (hipMemcpyAsync(items[i].first, copies[i], items[i].second, hipMemcpyDeviceToDevice, p->stream) != hipSuccess)
                 rc = emp_hip_fail(hipGetLastError(), "restore", __FILE__, __LINE__);
         if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "restore", __FILE__, __LINE__);
-        if (!rc && (best_t < 0 || ms < best_t)) { best_t = ms; best = w; }
+        const double per = (double)ms / (NS * trials);
+        if (!rc && (best_t < 0 || per < best_t)) { best_t = per; best = w; }
     }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     release();
     p->wpt = rc ? keep : best;
-    if (rc) return rc;
     drop_graphs(p);
+    if (rc) return rc;
     if (best_wpt) *best_wpt = best;
     if (best_ms) *best_ms = best_t;
     return EMP_OK;

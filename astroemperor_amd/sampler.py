@@ -287,8 +287,8 @@ class PTSampler:
         return a.value, b.value
 
     def autotune(self, trials=10):
-        """Pick the fastest waves per walker for the half-step (state left
-        unchanged); returns (waves_per_walker, ms per half-step)."""
+        """Pick the fastest waves per walker from timed graph-replayed sweeps
+        (state left unchanged); returns (waves_per_walker, ms per sweep)."""
         w, t = C.c_int32(), C.c_double()
         _abi.check(_abi.lib().emp_pt_autotune(self._pt, int(trials), C.byref(w), C.byref(t)))
         return w.value, t.value

@@ -40,11 +40,16 @@ OPS_KEP = {"fp64": 24 + 3 + 20 * 5, "mixed": 24 * 0.5 + 3 + 4 * 4 + 20 * 1}
 OPS_EPOCH = 6 + 20 * 2
 OPS_MA = 3 + 20 * 1
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
-# HBM bytes per launch of k_pt_half<mixed, 8 waves per walker> on the default
-# workload: rocprofv3 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE (784.0 + 115.2
-# KiB), the factors from the same counters on a 256 MiB copy (the summary is
-# profiles/k_pt_half_k_sys_sweep_hbm_traffic.md).
-TRAFFIC_BYTES_PER_LAUNCH = int((2.0 * 784.0 + 115.2) * 1024)
+# HBM bytes per launch of k_pt_half<mixed, W waves per walker> on the default
+# workload: rocprofv3 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE in KiB per
+# dispatch, the factors from the same counters on a 256 MiB copy (the summary
+# is profiles/k_pt_half_k_sys_sweep_hbm_traffic.md).
+TRAFFIC_KIB = {1: (476.7, 63.8), 2: (482.9, 63.8), 4: (508.3, 64.4), 8: (539.9, 115.2)}
+
+
+def traffic_bytes(wpt):
+    f, w = TRAFFIC_KIB[wpt]
+    return int((2.0 * f + 1.0 * w) * 1024)
 
 
 def flops_per_theta(model, precision):
@@ -57,13 +62,14 @@ def flops_per_theta(model, precision):
 
 
 def run_gpu(work, precision, steps, warmup, seed, half_launches):
+    half_launches += half_launches & 1          # whole sweeps: two likelihood launches each
     from astroemperor_amd import PTSampler
     from astroemperor_amd.simulation import test_init
     m = work["model"]
     T, W, _, nsteps = work["setup"]
     p0 = test_init(m, T, W, np.random.default_rng(seed))
     s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=precision, adapt_tau=1000, adapt_nu=1)
-    cap = warmup + steps + 4
+    cap = warmup + steps + (half_launches + 1) // 2 + 4
     s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=cap)
     wpt, _ = s.autotune(10)
     n0 = s.n_lnl_evaluated
@@ -298,7 +304,7 @@ def run_c5(seed, steps, warmup):
            "gradient_step": GradientMove().scale(m.ndim__)}
     last = None
     for prec in ("mixed", "fp64"):
-        extra = 2 * 6
+        extra = 2 * 6 + 3          # the per-move timings and the timed half-steps
         s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec, record=False, moves=mix)
         s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=warmup + steps + extra + 1)
         wpt, _ = s.autotune(3)
@@ -390,7 +396,8 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=20)
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--half-launches", type=int, default=40)
+    ap.add_argument("--half-launches", type=int, default=40,
+                    help="likelihood-kernel launches timed with events (an even number: whole sweeps)")
     ap.add_argument("--systems", type=int, default=4096, help="batched-systems leg (0 = skip)")
     ap.add_argument("--systems-steps", type=int, default=30)
     ap.add_argument("--systems-warmup", type=int, default=5)
@@ -454,7 +461,7 @@ def main():
     achieved = flops / (mixed["ms_halfstep"] * 1e-3) / 1e12
     roof = {"bound": "valu", "kernel": "k_pt_half<mixed>", "achieved": round(achieved, 3),
             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "traffic": traffic_bytes(mixed["waves_per_walker"]),
             "convention": "FP64-equivalent flops per lnL eval = 2 x (planet-epochs x ops_kep + epochs x 46"
                           " + MA epochs x 23); ops_kep = 127 (fp64) / 51 (mixed)",
             "flops_per_launch": flops, "launch_ms": mixed["ms_halfstep"],

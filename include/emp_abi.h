@@ -234,16 +234,19 @@ int emp_pt_restore(EmpPT* pt, const char* path);
  * (HIP events on the engine stream), and of its half-step kernels. */
 int emp_pt_last_timing(EmpPT* pt, double* ms_per_sweep, double* ms_halfstep_kernel);
 
-/* No reference counterpart (launch shape): times the half-step at 1, 2, 4
- * and 8 waves per walker for 2 x trials launches each on a copy of the
- * state, restores the state (the chain is unchanged) and keeps the fastest.
+/* No reference counterpart (launch shape): times graph-replayed stretch
+ * sweeps (8 x trials, after one untimed replay) at 1, 2, 4 and 8 waves per
+ * walker on a copy of the state, restores the state (the chain is unchanged)
+ * and keeps the fastest; best_ms_sweep receives its mean ms per sweep.
  * The waves per walker set the summation order of lnL, so chains run with
  * different shapes agree to rounding, not bit for bit. */
-int emp_pt_autotune(EmpPT* pt, int32_t trials, int32_t* best_waves_per_theta, double* best_ms_halfstep);
+int emp_pt_autotune(EmpPT* pt, int32_t trials, int32_t* best_waves_per_theta, double* best_ms_sweep);
 
-/* No reference counterpart (measurement): launches n fused stretch half-step
- * kernels one by one, bracketing each with HIP events on the engine stream,
- * and returns their mean duration.  The launches advance the chain. */
+/* No reference counterpart (measurement): runs ceil(n / (2 nsteps)) stretch
+ * sweeps launch by launch (no graph), timing the first n likelihood kernels
+ * with HIP events that their launches record (kernel start and end, on the
+ * engine stream), and returns their mean duration.  The sweeps are ordinary ones: they advance the chain and use
+ * history capacity. */
 int emp_pt_time_halfstep(EmpPT* pt, int32_t n, double* ms_avg);
 
 /* ---------------------------------------------------------------------------
