@@ -93,6 +93,7 @@ _SIGS = {
     "emp_pt_set_state": (C.c_int, [_P, _P, _P]),
     "emp_pt_set_adaptation": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
     "emp_pt_set_moves": (C.c_int, [_P, _P, C.c_int32]),
+    "emp_pt_lnl_moments": (C.c_int, [_P, C.c_int64, _P, _P, _P]),
     "emp_pt_checkpoint": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_restore": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -310,6 +310,66 @@ __global__ void k_identity(int* map, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// Evidence ingredients from the recorded lnL history, sweeps [s0, s1), all
+// walkers, one workgroup per temperature t: the mean lnL, and for t >= 1 the
+// stepping stone from t to the colder t - 1,
+//     log mean exp((beta_{t-1} - beta_t) lnL_t)
+// (the current ladder), by a max pass and a sum pass, with the delta-method
+// variance of that log (mean of exp(2 x) over the square of the mean of
+// exp(x), minus one, over n; autocorrelation ignored).  Every sum runs in a
+// fixed order (thread stride, then a fixed tree), so the result is
+// reproducible.
+// ---------------------------------------------------------------------------
+constexpr int MOM_BLOCK = 256;
+
+__device__ inline double block_tree(double v, double* s_tmp, bool take_max) {
+    const int tid = threadIdx.x;
+    s_tmp[tid] = v;
+    __syncthreads();
+    for (int o = MOM_BLOCK / 2; o > 0; o >>= 1) {
+        if (tid < o) s_tmp[tid] = take_max ? fmax(s_tmp[tid], s_tmp[tid + o]) : s_tmp[tid] + s_tmp[tid + o];
+        __syncthreads();
+    }
+    const double r = s_tmp[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(MOM_BLOCK) void k_pt_lnl_moments(PTDev pd, long long s0, long long s1, double* mean,
+                                                              double* lse, double* var) {
+    __shared__ double s_tmp[MOM_BLOCK];
+    const int t = blockIdx.x, T = pd.T, W = pd.W;
+    const long long n = (s1 - s0) * W;
+    const double db = t > 0 ? pd.betas[t - 1] - pd.betas[t] : 0.0;
+    double sum = 0.0, mx = -__builtin_inf();
+    for (long long i = threadIdx.x; i < n; i += MOM_BLOCK) {
+        const long long s = s0 + i / W;
+        const double l = pd.h_lnl[((size_t)s * T + t) * W + (size_t)(i % W)];
+        sum += l;
+        mx = fmax(mx, db * l);
+    }
+    sum = block_tree(sum, s_tmp, false);
+    mx = block_tree(mx, s_tmp, true);
+    const bool fin = mx > -__builtin_inf() && mx < __builtin_inf();
+    double se = 0.0, se2 = 0.0;
+    if (t > 0 && fin)
+        for (long long i = threadIdx.x; i < n; i += MOM_BLOCK) {
+            const long long s = s0 + i / W;
+            const double x = exp(db * pd.h_lnl[((size_t)s * T + t) * W + (size_t)(i % W)] - mx);
+            se += x;
+            se2 += x * x;
+        }
+    se = block_tree(se, s_tmp, false);
+    se2 = block_tree(se2, s_tmp, false);
+    if (threadIdx.x == 0) {
+        const double dn = (double)n;
+        mean[t] = sum / dn;
+        lse[t] = t == 0 ? 0.0 : (fin ? mx + log(se / dn) : mx);
+        var[t] = (t > 0 && fin) ? ((se2 / dn) / ((se / dn) * (se / dn)) - 1.0) / dn : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Gradient move: ensemble-preconditioned Metropolis-adjusted Langevin
 // (Roberts & Tweedie 1996; the preconditioner is the covariance of the
 // complementary half of the ensemble, Leimkuhler, Matthews & Weare 2018).
@@ -1123,6 +1183,29 @@ extern "C" int emp_pt_set_adaptation(EmpPT* p, int32_t mode, double tau, double 
     p->dev.adapt_tau = tau;
     p->dev.adapt_nu = nu;
     drop_graphs(p);        // kernel arguments changed: capture again on the next run
+    return EMP_OK;
+}
+
+extern "C" int emp_pt_lnl_moments(EmpPT* p, int64_t discard, double* mean, double* lse, double* var) {
+    if (!p || !mean || !lse || !var) { emp_set_error("null argument"); return EMP_EINVAL; }
+    const PTDev& v = p->dev;
+    if (!v.record) { emp_set_error("history was not recorded (record == 0)"); return EMP_EINVAL; }
+    EMP_HIP(hipStreamSynchronize(p->stream));
+    long long done = 0;
+    EMP_HIP(hipMemcpy(&done, v.sweep, sizeof(done), hipMemcpyDeviceToHost));
+    const long long S = std::min<long long>(done, v.max_sweeps);
+    if (discard < 0 || discard >= S) { emp_set_error("discard must lie in [0, %lld)", S); return EMP_EINVAL; }
+    double* d = nullptr;
+    EMP_HIP(hipMalloc(&d, 3 * (size_t)v.T * sizeof(double)));
+    hipLaunchKernelGGL(k_pt_lnl_moments, dim3((unsigned)v.T), dim3(MOM_BLOCK), 0, p->stream, v, (long long)discard, S,
+                       d, d + v.T, d + 2 * v.T);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(mean, d, (size_t)v.T * sizeof(double), hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(lse, d + v.T, (size_t)v.T * sizeof(double), hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(var, d + 2 * v.T, (size_t)v.T * sizeof(double), hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return emp_hip_fail(e, "lnl moments", __FILE__, __LINE__);
     return EMP_OK;
 }
 

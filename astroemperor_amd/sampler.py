@@ -273,6 +273,34 @@ class PTSampler:
         return np.array([integrated_time(chain[t], c=c, tol=tol, quiet=quiet) * thin
                          for t in range(self.ntemps)])
 
+    # -- evidence (reddemcee's get_evidence_*; emp/emp.py:1432-1447) -------------
+    def lnl_moments(self, discard=0):
+        """Per temperature (ladder order): mean lnL, stepping-stone log ratio
+        to the next colder temperature, and its variance, over the recorded
+        sweeps from `discard` on (device reduction)."""
+        T = self.ntemps
+        mean, lse, var = np.empty(T), np.empty(T), np.empty(T)
+        _abi.check(_abi.lib().emp_pt_lnl_moments(self._pt, int(discard), mean.ctypes.data, lse.ctypes.data,
+                                                  var.ctypes.data))
+        return mean, lse, var
+
+    def get_evidence_ti(self, discard=0, pchip=True):
+        """(log Z, error) by thermodynamic integration over the current ladder."""
+        from .evidence import ti_log_evidence
+        mean, _, _ = self.lnl_moments(discard)
+        return ti_log_evidence(self.betas, mean, pchip=pchip)
+
+    def get_evidence_ss(self, discard=0):
+        """(log Z, error) by stepping stones over the current ladder."""
+        from .evidence import ss_log_evidence
+        mean, lse, var = self.lnl_moments(discard)
+        return ss_log_evidence(self.betas, mean, lse, var)
+
+    def get_evidence_hybrid(self, discard=0):
+        """reddemcee's hybrid estimator is not available here; EMPEROR then
+        falls back to get_evidence_ti (emp/emp.py:1443-1446)."""
+        raise NotImplementedError("the hybrid evidence estimator is not available; use 'ti' or 'ss'")
+
     # -- checkpoint / resume ---------------------------------------------------
     def checkpoint(self, path):
         _abi.check(_abi.lib().emp_pt_checkpoint(self._pt, str(path).encode()))

@@ -225,6 +225,15 @@ typedef struct EmpMoveDesc {
  * a from EmpPTDesc). */
 int emp_pt_set_moves(EmpPT* pt, const EmpMoveDesc* moves, int32_t n);
 
+/* Evidence ingredients for reddemcee's get_evidence_ti / get_evidence_ss
+ * (called at emp/emp.py:1434-1446; the estimators run on the host): from the
+ * recorded lnL of sweeps [discard, sweeps done), per temperature t,
+ * mean_out[t] = mean lnL, and lse_out[t] (t >= 1) = log mean exp((beta_{t-1} -
+ * beta_t) lnL_t), the stepping stone to the next colder temperature under the
+ * current ladder (lse_out[0] = 0), with var_out[t] its delta-method variance
+ * (autocorrelation ignored).  Host arrays of T doubles.  Needs record == 1. */
+int emp_pt_lnl_moments(EmpPT* pt, int64_t discard, double* mean_out, double* lse_out, double* var_out);
+
 /* No reference counterpart (the reference cannot resume a run): binary
  * checkpoint of the full engine state, bitwise-resumable. */
 int emp_pt_checkpoint(EmpPT* pt, const char* path);
