@@ -12,6 +12,7 @@ from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MOAVBlock, 
                      OffsetBlock, SmartSetter)
 from .data import DataWrapper, RVData  # noqa: F401
 from .model import ReddModel  # noqa: F401
+from .backend import SavedRun  # noqa: F401
 from .moves import GradientMove, StretchMove  # noqa: F401
 from .sampler import PTSampler  # noqa: F401
 from .simulation import Simulation  # noqa: F401

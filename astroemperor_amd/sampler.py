@@ -273,6 +273,13 @@ class PTSampler:
         return np.array([integrated_time(chain[t], c=c, tol=tol, quiet=quiet) * thin
                          for t in range(self.ntemps)])
 
+    # -- chain storage (reddemcee's PTHDFBackend, emp/emp.py:722-762) ------------
+    def save(self, path):
+        """Writes the recorded history to a .npz archive (see
+        :mod:`astroemperor_amd.backend`); read it back with SavedRun."""
+        from .backend import save_run
+        save_run(self, path)
+
     # -- evidence (reddemcee's get_evidence_*; emp/emp.py:1432-1447) -------------
     def lnl_moments(self, discard=0):
         """Per temperature (ladder order): mean lnL, stepping-stone log ratio

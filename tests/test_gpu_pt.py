@@ -253,3 +253,28 @@ def test_history_survives_set_state_between_runs():
     assert chain.shape[1] == 60
     assert np.array_equal(chain[:, :40], first_run)
     assert np.array_equal(chain[:, 59], s.get_last_sample().coords)
+
+
+def test_saved_run_round_trip(tmp_path):
+    """save() writes the recorded history; SavedRun returns the same arrays
+    through the getters, with discard / thin / flat."""
+    from astroemperor_amd import PTSampler, SavedRun
+    from astroemperor_amd.simulation import test_init
+    sim = make_sim("51Peg", 1, parameterisation=1,
+                   conditions=[("Period 1", "limits", [4.0, 4.5]), ("Amplitude 1", "limits", [40, 70])])
+    m = sim.model
+    T, W = 3, 16
+    s = PTSampler(W, m.ndim__, m, ntemps=T, seed=9)
+    s.run_mcmc(test_init(m, T, W, np.random.default_rng(4)), nsweeps=30)
+    s.save(tmp_path / "run.npz")
+    r = SavedRun(tmp_path / "run.npz")
+    assert (r.ntemps, r.nsweeps, r.nwalkers, r.ndim) == (T, 30, W, m.ndim__)
+    assert r.iteration == s.iteration == 30
+    for kw in ({}, {"discard": 10, "thin": 2}, {"discard": 5, "flat": True}):
+        assert np.array_equal(r.get_chain(**kw), s.get_chain(**kw))
+        assert np.array_equal(r.get_log_like(**kw), s.get_log_like(**kw))
+        assert np.array_equal(r.get_log_prob(**kw), s.get_log_prob(**kw))
+    assert np.array_equal(r.get_betas(discard=3), s.get_betas(discard=3))
+    assert np.array_equal(r.get_tsw(), s.get_tsw()) and np.array_equal(r.get_smd(), s.get_smd())
+    assert np.array_equal(r.acceptance_fraction, s.acceptance_fraction)
+    assert np.array_equal(r.betas, s.betas)
