@@ -8,8 +8,8 @@ side: data staging, model layout, the reddemcee-style sampler and the
 __version__ = "0.1.0"
 
 from ._abi import EmpError, device_info  # noqa: F401
-from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MOAVBlock,  # noqa: F401
-                     OffsetBlock, SmartSetter)
+from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MagneticCycleBlock,  # noqa: F401
+                     MOAVBlock, OffsetBlock, SinusoidBlock, SmartSetter)
 from .data import DataWrapper, RVData  # noqa: F401
 from .model import ReddModel  # noqa: F401
 from .backend import SavedRun  # noqa: F401

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libemp_mi355x.so")
 
-EMP_ABI_VERSION = 1
+EMP_ABI_VERSION = 2
 EMP_HOST_POINTERS = 1
 EMP_FP64, EMP_MIXED = 0, 1
 PRIOR_KIND = {"Uniform": 0, "Normal": 1, "Jeffreys": 2, "Beta": 3, "Fixed": 4}
@@ -41,6 +41,7 @@ class EmpModelDesc(C.Structure):
                 ("ndim", C.c_int32), ("n_blocks", C.c_int32),
                 ("off_kep", C.c_int32), ("off_acc", C.c_int32), ("off_offset", C.c_int32),
                 ("off_jitter", C.c_int32), ("off_ma", C.c_int32),
+                ("off_sinusoid", C.c_int32), ("off_magnetic", C.c_int32),
                 ("t", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
                 ("yerr", C.POINTER(C.c_double)), ("ins", C.POINTER(C.c_int32)),
                 ("free_index", C.POINTER(C.c_int32)), ("fixed_value", C.POINTER(C.c_double)),

@@ -5,7 +5,7 @@ Host-side mirror of the reference's model-spec layer: parameter metadata
 (emp/emp_model.py:16-160), the block factories (emp/block_repo.py:22-240)
 and ``SmartSetter`` (emp/block_repo.py:505-867).  Only the blocks on the RV
 hot path are provided: Keplerian (all parameterisations the RV templates
-support), Acceleration, Offset, Jitter and MOAV.
+support), Acceleration, Offset, Jitter, MOAV, Sinusoid and MagneticCycle.
 
 These objects carry names, limits, priors and fixed values; the numerical
 work happens in the HIP library, which receives a flat descriptor built by
@@ -220,6 +220,23 @@ def MOAVBlock(nins, moav):
                                     "number_": nins, "moav": order, "is_global": is_global})
 
 
+def SinusoidBlock(number):
+    """emp/block_repo.py:75-100: A cos(2 pi t / P + phase)."""
+    suffix = f" {number}" if number else ""
+    params = [_named(k, suffix) for k in ("dPeriod", "dAmplitude", "dPhase")]
+    return Parameter_Block(params, {"name_": "SinusoidBlock", "type_": "Sinusoid", "class_": "Model",
+                                    "number_": number})
+
+
+def MagneticCycleBlock(number):
+    """emp/block_repo.py:275-314: A1 cos(pi t / P + phase1) + A2 cos(2 pi t / P + phase2),
+    parameters (Period S1, Amplitude S1, Amplitude S2, Phase S1, Phase S2)."""
+    keys = ("dPeriod", "dAmplitude", "dAmplitude", "dPhase", "dPhase")
+    params = [_named(k, f" S{s}") for k, s in zip(keys, (1, 1, 2, 1, 2))]
+    return Parameter_Block(params, {"name_": "MagneticCycleBlock", "type_": "MagneticCycle",
+                                    "class_": "Model", "number_": number})
+
+
 class SmartSetter:
     """Data-driven default limits and priors (emp/block_repo.py:505-867).
 
@@ -316,6 +333,16 @@ class SmartSetter:
     # block_repo.py:815-826
     def set_Acceleration(self, b, **kw):
         return [[-1, 1]] * b.number_, [self.uni] * b.number_, [None] * b.number_
+
+    # block_repo.py:805-830
+    def set_Sinusoid(self, b, **kw):
+        lims = [[1.5, self._span()], [1e-6, self._std() * np.sqrt(4)], self.angular]
+        return [list(map(float, l)) for l in lims], [self.uni] * 3, [None] * 3
+
+    def set_MagneticCycle(self, b, **kw):
+        amp = [1e-6, self._std() * np.sqrt(4)]
+        lims = [[1.5, self._span()], amp, amp, self.angular, self.angular]
+        return [list(map(float, l)) for l in lims], [self.uni] * 5, [None] * 5
 
     def __call__(self, block, *args, **kw):
         lims, pri, pra = getattr(self, f"set_{block.type_}")(block, *args, **kw)

@@ -23,6 +23,11 @@
 // recomputed and lambda comes from a suffix scan with the carry running
 // backwards.
 //
+// Cosine terms (sinusoid, magnetic cycle) S_i = sum_c A_c cos(f_c t_i + ph_c)
+// are subtracted after the moving average, so dlnL/dS_i is the final
+// residual over err2_i and does not pass through the MA adjoint:
+//   dS/dA = cos M, dS/dph = -A sin M, dS/df = -A sin M t, df/dP = -f/P.
+//
 // The natural-parameter gradients are then mapped to the block's
 // parameterisation (block_repo.py:23-31) once per vector.
 #include <cmath>
@@ -34,14 +39,16 @@ using namespace emp;
 struct GradLDS {
     double g[MAXD];            // gradient w.r.t. the full vector
     double nat[KMAX][5];       // per planet: (M*tau, M, e, w, A) accumulators
+    double cosg[CMAX][3];      // per cosine term: (A, phase, frequency) accumulators
 };
 
 // Per-wave partial sums, added over a slot's waves in wave order so that the
 // gradient does not depend on how the waves were scheduled.
-constexpr int WA_OFF = 0, WA_JIT = IMAX, WA_MAC = 2 * IMAX, WA_MAT = 3 * IMAX, WA_N = 4 * IMAX;
+constexpr int WA_OFF = 0, WA_JIT = IMAX, WA_MAC = 2 * IMAX, WA_MAT = 3 * IMAX, WA_COS = 4 * IMAX;
+constexpr int WA_N = 4 * IMAX + 3 * CMAX;
 constexpr int WN_ACC = KMAX * 5, WN_N = KMAX * 5 + AMAX;
 struct GradWaveLDS {
-    double acc[NWAVES][WA_N];   // offsets, jitters, MA coefficients, MA timescales (per instrument / set)
+    double acc[NWAVES][WA_N];   // offsets, jitters, MA coefficients, MA timescales (per instrument / set), cosine terms
     double nat[NWAVES][WN_N];   // natural-parameter sums per planet, then the acceleration terms
 };
 
@@ -110,6 +117,20 @@ __device__ __forceinline__ void accumulate(const ModelDev& md, const EpochDeriv&
     }
 }
 
+// Adds w = dlnL/dS_i times the cosine terms' derivatives at time t.
+__device__ __forceinline__ void cos_accumulate(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
+                                               double w, double* acc) {
+    for (int c = 0; c < md.n_cos; ++c) {
+        const double r = reduce_2pi(cos_phase(sl, c, t));
+        double sn, cs;
+        sincos_tab(tab, __builtin_fabs(r), sn, cs);
+        const double dph = -sl.cosw[c][0] * (r < 0.0 ? -sn : sn);
+        atomicAdd(&acc[3 * c + 0], w * cs);
+        atomicAdd(&acc[3 * c + 1], w * dph);
+        atomicAdd(&acc[3 * c + 2], w * dph * t);
+    }
+}
+
 // MA(1) multiplier of epoch n (device order) from its instrument and its MA
 // column (the fixed-timescale weight, or dt with -1 at a segment start).
 __device__ __forceinline__ double ma_mult(const ModelDev& md, const SlotLDS& sl, int k, double col) {
@@ -171,7 +192,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 EpochDeriv d;
                 epoch_deriv<PREC>(md, sl, s_tile.tab, t, k, d);
                 const double err2 = s_tile.s2[j] + sl.jit2[k];
-                const double r = s_tile.y[j] - d.mu;
+                const double cs = md.n_cos ? epoch_cos(md, sl, s_tile.tab, t) : 0.0;
+                const double r = s_tile.y[j] - d.mu - cs;
                 const double ie = rcp_path<PREC>(err2);
                 const double w = r * ie;
                 chi = __builtin_fma(r * r, ie, chi);
@@ -181,6 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
                 atomicAdd(&wacc[WA_OFF + k], w);
                 if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (w * w - ie));
+                if (md.n_cos) cos_accumulate(md, sl, s_tile.tab, t, w, wacc + WA_COS);
             }
         }
         double ldet = live ? log(prod) + (double)pexp * LN2 : 0.0;
@@ -210,7 +233,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 const int j = cb + g;
                 const bool valid = live && j < tn;
                 EpochDeriv d;
-                double u = 0.0, amul = 1.0, e_i = 0.0, ie = 0.0, t = 0.0;
+                double u = 0.0, amul = 1.0, e_i = 0.0, ie = 0.0, t = 0.0, ef = 0.0;
                 int k = 0;
                 if (valid) {
                     t = s_tile.t[j];
@@ -219,7 +242,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     const double err2 = s_tile.s2[j] + sl.jit2[k];
                     ie = rcp_path<PREC>(err2);
                     e_i = eps[tb + j];
-                    u = -e_i * ie;
+                    ef = md.n_cos ? e_i - epoch_cos(md, sl, s_tile.tab, t) : e_i;
+                    u = -ef * ie;
                     const int n1 = tb + j + 1;     // the next epoch's multiplier
                     amul = 0.0;
                     if (n1 < md.N) {
@@ -253,7 +277,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     const double w = -lam;
                     accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
                     atomicAdd(&wacc[WA_OFF + k], w);
-                    if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (e_i * e_i * ie * ie - ie));
+                    if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (ef * ef * ie * ie - ie));
+                    if (md.n_cos) cos_accumulate(md, sl, s_tile.tab, t, -u, wacc + WA_COS);
                     // da_i/dc = a_i / c, da_i/dtau = a_i dt / tau^2 (a_i = 0 at a segment start)
                     const double col = s_tile.dt[j];
                     const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
@@ -297,7 +322,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
             double v = 0.0;
             for (int w = 0; w < WPT; ++w) v += s_gw.acc[w0 + w][e];
             const int grp = e / IMAX, k = e - grp * IMAX;
-            if (grp == 0 && k < md.I) gl.g[md.off_off + k] = v;
+            if (e >= WA_COS) (&gl.cosg[0][0])[e - WA_COS] = v;
+            else if (grp == 0 && k < md.I) gl.g[md.off_off + k] = v;
             else if (grp == 1 && md.has_jit && k < md.I) gl.g[md.off_jit + k] = v;
             else if (grp >= 2 && md.ma_active && k < (md.ma_global ? 1 : md.I))
                 gl.g[md.off_ma + 2 * k + (grp - 2)] = v;
@@ -345,6 +371,22 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
             } else {
                 gp[3] = ge;
                 gp[4] = gw;
+            }
+        }
+        if (lane == 63) {
+            for (int c = 0; c < md.n_cos; ++c) {
+                const int m = md.off_sin >= 0 ? c - 1 : c;    // -1: the sinusoid
+                const int o = m < 0 ? md.off_sin : md.off_mag;
+                const double gP = gl.cosg[c][2] * -(sl.cosw[c][1] / sl.theta[o]);
+                if (m < 0) {
+                    gl.g[o] = gP;
+                    gl.g[o + 1] = gl.cosg[c][0];
+                    gl.g[o + 2] = gl.cosg[c][1];
+                } else {
+                    gl.g[o] += gP;
+                    gl.g[o + 1 + m] = gl.cosg[c][0];
+                    gl.g[o + 3 + m] = gl.cosg[c][1];
+                }
             }
         }
     }

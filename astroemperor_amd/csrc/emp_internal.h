@@ -13,6 +13,7 @@ constexpr int MAXD = 128;        // full parameter vector length supported by th
 constexpr int KMAX = 8;          // Keplerians per model
 constexpr int IMAX = 16;         // instruments per model
 constexpr int AMAX = 4;          // acceleration order
+constexpr int CMAX = 3;          // cosine terms per model (a sinusoid, a magnetic cycle's two)
 constexpr int BLOCK = 512;       // threads per workgroup (8 waves)
 constexpr int NWAVES = BLOCK / 64;
 constexpr int MAX_WAVES = 16;    // largest workgroup any kernel launches (1024 threads)
@@ -42,6 +43,8 @@ struct ModelDev {
     int ma_wfixed;        // every MA timescale is fixed: dt holds the weights exp(-dt / tau)
     int Dfull, D, nblocks;
     int off_kep, off_acc, off_off, off_jit, off_ma;
+    int off_sin, off_mag; // Sinusoid / MagneticCycle block starts, -1 when absent
+    int n_cos;            // cosine terms: 1 per sinusoid, 2 per magnetic cycle
     const double* t;      // [N] device order
     const double* y;      // [N]
     const double* s2;     // [N] yerr^2
@@ -68,6 +71,7 @@ struct SlotLDS {
     PlanetC planet[KMAX];
     double gam[IMAX], jit2[IMAX], ma_c[IMAX], ma_tau[IMAX], jit[IMAX];
     double acc[AMAX];
+    double cosw[CMAX][3];      // per cosine term: (amplitude, angular frequency, phase)
     float2 ktab[KMAX][KT_N];   // mixed path: (E_j, h dE/dM_j) per planet
 };
 

@@ -78,6 +78,23 @@ __device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
         }
     }
     if (lane < md.acc) sl.acc[lane] = sl.theta[md.off_acc + lane];
+    if (lane < md.n_cos) {
+        // Term 0 is the sinusoid when there is one (sinusoid00.model:2-5:
+        // freq = 2 pi / P); the magnetic cycle's terms are (A1, pi / P, phase1)
+        // and (A2, 2 pi / P, phase2) (magneticcycle00.model:2-8).
+        const int c = md.off_sin >= 0 ? lane - 1 : lane;
+        double A, f, ph;
+        if (c < 0) {
+            const double* th = sl.theta + md.off_sin;
+            A = th[1]; f = (2.0 * PI_D) / th[0]; ph = th[2];
+        } else {
+            const double* th = sl.theta + md.off_mag;
+            A = th[1 + c]; f = (c == 0 ? PI_D : 2.0 * PI_D) / th[0]; ph = th[3 + c];
+        }
+        sl.cosw[lane][0] = A;
+        sl.cosw[lane][1] = f;
+        sl.cosw[lane][2] = ph;
+    }
 }
 
 struct TileLDS {
@@ -161,6 +178,29 @@ __device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl
     return mu;
 }
 
+// Phase freq * t + phase of cosine term c, rounded as numpy rounds
+// `freq * X_ + phase` (no contraction).
+__device__ __forceinline__ double cos_phase(const SlotLDS& sl, int c, double t) {
+#pragma clang fp contract(off)
+    return sl.cosw[c][1] * t + sl.cosw[c][2];
+}
+
+// Sum of the cosine terms at time t.  They enter the model after the moving
+// average: in the reference's block order Sinusoid and MagneticCycle follow
+// MOAV (emp/emp.py:2628-2650), so the MA residuals do not contain them.  The
+// angle is reduced in FP64 and its cosine taken from the FP64 table on both
+// precision paths.
+__device__ __forceinline__ double epoch_cos(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t) {
+    double v = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < md.n_cos; ++c) {
+        double sn, cs;
+        sincos_tab(tb, __builtin_fabs(reduce_2pi(cos_phase(sl, c, t))), sn, cs);
+        v = __builtin_fma(sl.cosw[c][0], cs, v);
+    }
+    return v;
+}
+
 // Affine map composition for the MA(1) recurrence eps_i = a_i eps_{i-1} + b_i:
 // applying (a1, b1) then (a2, b2) gives (a2 a1, a2 b1 + b2).  Inclusive scan
 // over the wave by DPP: within rows of 16 lanes, then across rows; lanes
@@ -208,13 +248,14 @@ struct ScanLDS {
 // a slot skip the arithmetic while still taking part in the barriers.  The
 // slot must be prepared (slot_prepare) and the sin/cos table loaded.
 // With WRITE the per-epoch model and err2 are stored (device order); with
-// WEPS the per-epoch MA-filtered residual eps (= r without MA).
+// WEPS the per-epoch MA-filtered residual eps (= r without MA; before the
+// cosine terms are taken off).
 // `resident`: the caller has already staged all epochs (md.N <= TILE) in tl.
 // `first`: the first tile's epochs, already loaded by the caller (else loaded
 // here).  With PF the next tile's loads are issued a tile ahead (launches
 // have at least TILE threads); without, each tile is loaded when its turn
-// comes (fewer registers).
-template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true>
+// comes (fewer registers).  COS = false compiles the cosine terms out.
+template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true, bool COS = true>
 __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
                             bool live, int wsub, double* out_model, double* out_err2, bool resident = false,
                             double* out_eps = nullptr, const EpochRegs* first = nullptr) {
@@ -290,12 +331,14 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
                 }
             }
             if (valid) {
-                chi = __builtin_fma(eps * eps, PREC == 1 ? rcp64_1(err2) : rcp64(err2), chi);
+                const double cs = (COS && md.n_cos) ? epoch_cos(md, sl, tl.tab, tl.t[j]) : 0.0;
+                const double res = eps - cs;
+                chi = __builtin_fma(res * res, PREC == 1 ? rcp64_1(err2) : rcp64(err2), chi);
                 prod *= err2;
                 pexp += __builtin_amdgcn_frexp_exp(prod);
                 prod = __builtin_amdgcn_frexp_mant(prod);
                 if (WRITE) {
-                    out_model[tb + j] = mu + (r - eps);
+                    out_model[tb + j] = mu + (r - eps) + cs;
                     out_err2[tb + j] = err2;
                 }
                 if (WEPS) out_eps[tb + j] = eps;

@@ -223,6 +223,8 @@ static int validate(const EmpModelDesc* d) {
         const int sets = d->ma_global ? 1 : d->n_ins;
         if (d->off_ma < 0 || d->off_ma + 2 * d->ma_order * sets > d->ndim_full) { emp_set_error("MA block out of range"); return EMP_EINVAL; }
     }
+    if (d->off_sinusoid < -1 || (d->off_sinusoid >= 0 && d->off_sinusoid + 3 > d->ndim_full)) { emp_set_error("sinusoid block out of range"); return EMP_EINVAL; }
+    if (d->off_magnetic < -1 || (d->off_magnetic >= 0 && d->off_magnetic + 5 > d->ndim_full)) { emp_set_error("magnetic-cycle block out of range"); return EMP_EINVAL; }
     for (int i = 0; i < d->n_epochs; ++i) {
         if (d->ins[i] < 0 || d->ins[i] >= d->n_ins) { emp_set_error("instrument tag out of range at epoch %d", i); return EMP_EINVAL; }
     }
@@ -324,6 +326,8 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
     md.Dfull = d->ndim_full; md.D = d->ndim; md.nblocks = d->n_blocks;
     md.off_kep = d->off_kep; md.off_acc = d->off_acc; md.off_off = d->off_offset;
     md.off_jit = d->off_jitter; md.off_ma = d->off_ma;
+    md.off_sin = d->off_sinusoid; md.off_mag = d->off_magnetic;
+    md.n_cos = (d->off_sinusoid >= 0 ? 1 : 0) + (d->off_magnetic >= 0 ? 2 : 0);
     md.lnl_const = -0.5 * std::log(2.0 * M_PI) * (double)N;
     md.npseq = (int)h->pseq.size();
     md.t0 = d->t[0];

@@ -6,7 +6,7 @@ model is flattened into an ``EmpModelDesc`` and handed to the HIP library,
 which evaluates ``my_model``/``my_likelihood``/``my_prior`` on the device.
 
 Block order follows the reference (emp/emp.py:1235, 2628-2650):
-``[Keplerian 1..k, Acceleration, Offset, Jitter, MOAV]``.
+``[Keplerian 1..k, Acceleration, Offset, Jitter, MOAV, Sinusoid, MagneticCycle]``.
 """
 from __future__ import annotations
 
@@ -18,7 +18,8 @@ from . import _abi
 from .blocks import SC_PARAMETERISATIONS
 from .data import RVData
 
-_ORDER = {"Keplerian": 0, "Acceleration": 1, "Offset": 2, "Jitter": 3, "MOAV": 4}
+_ORDER = {"Keplerian": 0, "Acceleration": 1, "Offset": 2, "Jitter": 3, "MOAV": 4, "Sinusoid": 5,
+          "MagneticCycle": 6}
 
 
 def _as_f64(a):
@@ -179,6 +180,9 @@ class ReddModel:
         desc.off_offset = off.slice.start
         desc.off_jitter = jit.slice.start if jit is not None else 0
         desc.off_ma = ma.slice.start if ma is not None else 0
+        sn, mc = self.block("Sinusoid"), self.block("MagneticCycle")
+        desc.off_sinusoid = sn.slice.start if sn is not None else -1
+        desc.off_magnetic = mc.slice.start if mc is not None else -1
         t, y, e = _as_f64(d.X), _as_f64(d.Y), _as_f64(d.YERR)
         ins = np.ascontiguousarray(d.FLAG - 1, dtype=np.int32)
         free = np.ascontiguousarray(self.C_, dtype=np.int32)

@@ -4,7 +4,8 @@ Mirrors the parts of ``astroemperor.Simulation`` (emp/emp.py:2240-2916) that
 drive the RV hot path: ``load_data``, ``set_engine('reddemcee')`` with its
 ``engine_config``/``run_config`` dictionaries, ``add_condition``, the model
 switches (``keplerian_parameterisation``, ``use_c``, ``moav``,
-``acceleration``, ``switch_jitter``, ``eccentricity_*``, ``jitter_*``),
+``acceleration``, ``sinusoid``, ``magnetic_cycle``, ``switch_jitter``,
+``eccentricity_*``, ``jitter_*``),
 ``run()`` and ``autorun(k_start, k_end)`` with the BIC stopping rule.  Instead
 of writing and executing a script in a subprocess (emp/emp.py:137-175,
 2559-2582), the model is built in-process and sampled on the GPU.
@@ -20,8 +21,8 @@ import time
 
 import numpy as np
 
-from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MOAVBlock, OffsetBlock,
-                     SmartSetter)
+from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MagneticCycleBlock, MOAVBlock,
+                     OffsetBlock, SinusoidBlock, SmartSetter)
 from .data import DataWrapper, RVData
 from .model import ReddModel
 from .sampler import PTSampler
@@ -70,6 +71,8 @@ class Simulation:
         self.use_c = False
         self.moav = {"order": 0, "global": False}
         self.acceleration = 0
+        self.sinusoid = 0
+        self.magnetic_cycle = 0
         self.switch_jitter = True
         self.eccentricity_limits = [0, 1]
         self.eccentricity_prargs = [0, 0.1]
@@ -143,6 +146,12 @@ class Simulation:
     def add_acceleration(self):
         self._append(AccelerationBlock(self.acceleration))
 
+    def add_sinusoid(self):
+        self._append(SinusoidBlock(self.sinusoid))
+
+    def add_magnetic_cycle(self):
+        self._append(MagneticCycleBlock(self.magnetic_cycle))
+
     def _autorun_add_RV_ins(self):                               # emp.py:2628-2650
         if self.acceleration:
             self.add_acceleration()
@@ -151,6 +160,10 @@ class Simulation:
             self.add_jitter()
         if self.moav["order"]:
             self.add_moav()
+        if self.sinusoid:
+            self.add_sinusoid()
+        if self.magnetic_cycle:
+            self.add_magnetic_cycle()
 
     # -- conditions (emp/emp.py:2790-2810) -----------------------------------
     def add_condition(self, cond):

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define EMP_ABI_VERSION 1
+#define EMP_ABI_VERSION 2
 
 /* Error codes. */
 #define EMP_OK 0
@@ -98,6 +98,10 @@ typedef struct EmpModelDesc {
     int32_t ndim;               /* free parameters */
     int32_t n_blocks;           /* number of parameter blocks */
     int32_t off_kep, off_acc, off_offset, off_jitter, off_ma;  /* block starts in the full vector */
+    int32_t off_sinusoid;       /* Sinusoid block (P, A, phase) start, -1 = none (sinusoid00.model) */
+    int32_t off_magnetic;       /* MagneticCycle block (P, A1, A2, phase1, phase2) start, -1 = none
+                                   (magneticcycle00.model).  Both terms are added to the model
+                                   after the moving average, as their blocks follow MOAV. */
     const double* t;            /* [N] X_ = BJD - common_t, time-sorted (host) */
     const double* y;            /* [N] Y_ */
     const double* yerr;         /* [N] YERR_ */

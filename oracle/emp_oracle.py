@@ -24,6 +24,11 @@ Pinning (see the tests that exercise each function):
 * ``OracleModel.my_likelihood`` -- same mpmath vectors, the 51 Peg KATs.
 * ``OracleModel.my_prior``     -- scipy.stats closed forms.
 * ``ma_residuals``             -- a literal scalar loop of moav01 on tiny N.
+* Sinusoid / MagneticCycle terms of ``my_model`` -- a sinusoid equals the
+                                  circular (e = 0, omega = 0) Keplerian; a
+                                  magnetic cycle equals two sinusoids at P and
+                                  2P; spot values; unfiltered by the MA
+                                  (tests/test_oracle_cosine_terms.py).
 * ``stretch_step`` / ``swap_step`` -- injected-random unit tests and the
                                   tempered-Gaussian moment tests.
 * ``langevin_factor`` / ``langevin_logq`` / ``langevin_halfstep`` -- the
@@ -295,7 +300,8 @@ class OracleModel:
     produces (``astroemperor_amd.model.ReddModel.spec()``):
 
     * ``blocks``: list of dicts in block order with keys ``type``
-      ('Keplerian', 'Acceleration', 'Offset', 'Jitter', 'MOAV'), ``start``,
+      ('Keplerian', 'Acceleration', 'Offset', 'Jitter', 'MOAV', 'Sinusoid',
+      'MagneticCycle'), ``start``,
       ``len`` (full-vector slice), and for Keplerians ``parameterisation``,
       ``use_c`` and ``additional`` (derived-eccentricity priors);
       MOAV blocks carry ``order`` and ``global``.
@@ -366,6 +372,14 @@ class OracleModel:
             elif typ == "Jitter":                                      # jitter00.model:3
                 for nin in range(self.nins):
                     err20 = err20 + self.masks[nin] * sl[nin] ** 2
+            elif typ == "Sinusoid":                                    # sinusoid00.model:2-7
+                per, A, phase = sl
+                model0 = model0 + A * np.cos(2. * np.pi / per * X + phase)
+            elif typ == "MagneticCycle":                               # magneticcycle00.model:2-10
+                per, A1, A2, phase1, phase2 = sl
+                M1 = np.pi / per * X + phase1
+                M2 = 2. * np.pi / per * X + phase2
+                model0 = model0 + (A1 * np.cos(M1) + A2 * np.cos(M2))
             else:
                 raise ValueError(f"unsupported block {typ}")
         return model0, err20

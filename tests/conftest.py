@@ -31,7 +31,7 @@ def peg51():
 
 
 def make_sim(folder, k, parameterisation=1, moav=None, jitter=True, acceleration=0,
-             conditions=(), semantics="reference"):
+             conditions=(), semantics="reference", sinusoid=0, magnetic_cycle=0):
     """A Simulation with the blocks autorun(k, k) would add, conditions applied."""
     from astroemperor_amd import Simulation
     sim = Simulation()
@@ -42,6 +42,8 @@ def make_sim(folder, k, parameterisation=1, moav=None, jitter=True, acceleration
     sim.acceleration = acceleration
     sim.switch_jitter = jitter
     sim.semantics = semantics
+    sim.sinusoid = sinusoid
+    sim.magnetic_cycle = magnetic_cycle
     if moav:
         sim.moav = moav
     sim._autorun_add_RV_ins()

@@ -107,3 +107,18 @@ def test_philox_known_answers():
         o = np.zeros(4, dtype=np.uint32)
         _abi.check(L.emp_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data))
         assert list(o) == want
+
+
+def test_descriptor_cosine_block_offsets():
+    """off_sinusoid / off_magnetic: the block starts, -1 when absent; a model
+    smaller than a magnetic-cycle block without one creates."""
+    from tests.conftest import make_sim
+    m = make_sim("51Peg", 0).model
+    desc, _ = m.descriptor()
+    assert desc.off_sinusoid == -1 and desc.off_magnetic == -1
+    assert desc.ndim_full < 5
+    m2 = make_sim("51Peg", 1, sinusoid=1, magnetic_cycle=1).model
+    desc, _ = m2.descriptor()
+    assert desc.off_sinusoid == m2.block("Sinusoid").slice.start
+    assert desc.off_magnetic == m2.block("MagneticCycle").slice.start
+    assert desc.off_magnetic + 5 == desc.ndim_full

@@ -57,3 +57,20 @@ def test_torch_reference_ma1_equals_oracle(glob):
         ref = o.my_likelihood(th)
         got = lnl_torch(m, torch.tensor(th)).item()
         assert abs(got - ref) <= 1e-10 * abs(ref), (got, ref)
+
+
+@pytest.mark.parametrize("moav", [None, {"order": 1, "global": True}, {"order": 1, "global": False}])
+def test_torch_reference_cosine_terms_equal_oracle(moav):
+    """Sinusoid and magnetic-cycle terms (after the MA) in the torch reference."""
+    rng = np.random.default_rng(21)
+    sim = make_sim("GJ876", 1, parameterisation=1, moav=moav, semantics="intended", sinusoid=1,
+                   magnetic_cycle=1)
+    m = sim.model
+    o = oracle_for(m)
+    lim = m.limits_free()
+    for _ in range(3):
+        th = lim[:, 0] + (lim[:, 1] - lim[:, 0]) * rng.uniform(0.2, 0.8, size=len(lim))
+        th[3:5] *= 0.5
+        ref = o.my_likelihood(th)
+        got = lnl_torch(m, torch.tensor(th)).item()
+        assert abs(got - ref) <= 1e-10 * abs(ref), (got, ref)

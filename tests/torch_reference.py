@@ -94,4 +94,13 @@ def lnl_torch(model, theta):
                 eps[i] = e_i
                 prev = i
         r = torch.stack(eps)
+    # Sinusoid / MagneticCycle follow MOAV in block order: added after the MA.
+    sn = model.block("Sinusoid")
+    if sn is not None:
+        per, A, ph = full[sn.slice.start:sn.slice.start + 3]
+        r = r - A * torch.cos(2 * np.pi / per * X + ph)
+    mc = model.block("MagneticCycle")
+    if mc is not None:
+        per, A1, A2, ph1, ph2 = full[mc.slice.start:mc.slice.start + 5]
+        r = r - (A1 * torch.cos(np.pi / per * X + ph1) + A2 * torch.cos(2 * np.pi / per * X + ph2))
     return -0.5 * torch.sum(r ** 2 / err2 + torch.log(err2)) - 0.5 * np.log(2 * np.pi) * len(X)
