@@ -9,7 +9,7 @@ __version__ = "0.1.0"
 
 from ._abi import EmpError, device_info  # noqa: F401
 from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MagneticCycleBlock,  # noqa: F401
-                     MOAVBlock, OffsetBlock, SinusoidBlock, SmartSetter)
+                     MOAVBlock, OffsetBlock, SAIBlock, SinusoidBlock, SmartSetter)
 from .data import DataWrapper, RVData  # noqa: F401
 from .model import ReddModel  # noqa: F401
 from .backend import SavedRun  # noqa: F401

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libemp_mi355x.so")
 
-EMP_ABI_VERSION = 2
+EMP_ABI_VERSION = 3
 EMP_HOST_POINTERS = 1
 EMP_FP64, EMP_MIXED = 0, 1
 PRIOR_KIND = {"Uniform": 0, "Normal": 1, "Jeffreys": 2, "Beta": 3, "Fixed": 4}
@@ -42,10 +42,12 @@ class EmpModelDesc(C.Structure):
                 ("off_kep", C.c_int32), ("off_acc", C.c_int32), ("off_offset", C.c_int32),
                 ("off_jitter", C.c_int32), ("off_ma", C.c_int32),
                 ("off_sinusoid", C.c_int32), ("off_magnetic", C.c_int32),
+                ("off_sai", C.c_int32), ("n_sai", C.c_int32),
                 ("t", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
                 ("yerr", C.POINTER(C.c_double)), ("ins", C.POINTER(C.c_int32)),
                 ("free_index", C.POINTER(C.c_int32)), ("fixed_value", C.POINTER(C.c_double)),
-                ("prior", C.POINTER(EmpPriorTerm)), ("ecc_prior", C.POINTER(EmpPriorTerm))]
+                ("prior", C.POINTER(EmpPriorTerm)), ("ecc_prior", C.POINTER(EmpPriorTerm)),
+                ("sai", C.POINTER(C.c_double)), ("sai_counts", C.POINTER(C.c_int32))]
 
 
 class EmpMoveDesc(C.Structure):

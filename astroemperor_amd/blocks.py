@@ -5,7 +5,8 @@ Host-side mirror of the reference's model-spec layer: parameter metadata
 (emp/emp_model.py:16-160), the block factories (emp/block_repo.py:22-240)
 and ``SmartSetter`` (emp/block_repo.py:505-867).  Only the blocks on the RV
 hot path are provided: Keplerian (all parameterisations the RV templates
-support), Acceleration, Offset, Jitter, MOAV, Sinusoid and MagneticCycle.
+support), Acceleration, Offset, Jitter, MOAV, StellarActivity (SAI),
+Sinusoid and MagneticCycle.
 
 These objects carry names, limits, priors and fixed values; the numerical
 work happens in the HIP library, which receives a flat descriptor built by
@@ -46,6 +47,7 @@ _META = {
     "dMACoefficient": ("MACoefficient", "Phi", False, False),
     "dMATimescale": ("MATimescale", "tau", False, False),
     "dAcceleration": ("Acceleration", "gamma", False, False),
+    "dStaract": ("Staract", "SA", False, False),
     "dSMA": ("Semi-Major Axis", "SMA", False, False),
     "dMinM": ("Minimum Mass", "MinM", False, False),
 }
@@ -220,6 +222,22 @@ def MOAVBlock(nins, moav):
                                     "number_": nins, "moav": order, "is_global": is_global})
 
 
+def SAIBlock(nins, cornums):
+    """emp/block_repo.py:243-272: one coefficient per activity column,
+    'Staract i j' for column j of instrument i, instrument-major."""
+    params = []
+    for i in range(nins):
+        for j in range(int(cornums[i])):
+            d = make_parameter("dStaract")
+            d["name"] += f" {i + 1} {j + 1}"
+            d["mininame"] += f"_{i + 1} {j + 1}"
+            params.append(Parameter(d))
+    b = Parameter_Block(params, {"name_": "StellarActivityBlock", "type_": "StellarActivity",
+                                 "class_": "Data", "number_": nins})
+    b.cornums = [int(c) for c in cornums]
+    return b
+
+
 def SinusoidBlock(number):
     """emp/block_repo.py:75-100: A cos(2 pi t / P + phase)."""
     suffix = f" {number}" if number else ""
@@ -333,6 +351,11 @@ class SmartSetter:
     # block_repo.py:815-826
     def set_Acceleration(self, b, **kw):
         return [[-1, 1]] * b.number_, [self.uni] * b.number_, [None] * b.number_
+
+    # block_repo.py:742-749
+    def set_StellarActivity(self, b, **kw):
+        n = sum(b.cornums)
+        return [[-1.0, 1.0]] * n, [self.uni] * n, [None] * n
 
     # block_repo.py:805-830
     def set_Sinusoid(self, b, **kw):

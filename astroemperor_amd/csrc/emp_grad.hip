@@ -23,10 +23,11 @@
 // recomputed and lambda comes from a suffix scan with the carry running
 // backwards.
 //
-// Cosine terms (sinusoid, magnetic cycle) S_i = sum_c A_c cos(f_c t_i + ph_c)
-// are subtracted after the moving average, so dlnL/dS_i is the final
-// residual over err2_i and does not pass through the MA adjoint:
-//   dS/dA = cos M, dS/dph = -A sin M, dS/df = -A sin M t, df/dP = -f/P.
+// Terms added after the moving average, S_i = sum_j zeta_j a_ij (activity
+// columns) + sum_c A_c cos(f_c t_i + ph_c) (sinusoid, magnetic cycle): dlnL/dS_i
+// is the final residual over err2_i and does not pass through the MA adjoint:
+//   dS/dzeta_j = a_ij, dS/dA = cos M, dS/dph = -A sin M, dS/df = -A sin M t,
+//   df/dP = -f/P.
 //
 // The natural-parameter gradients are then mapped to the block's
 // parameterisation (block_repo.py:23-31) once per vector.
@@ -45,10 +46,11 @@ struct GradLDS {
 // Per-wave partial sums, added over a slot's waves in wave order so that the
 // gradient does not depend on how the waves were scheduled.
 constexpr int WA_OFF = 0, WA_JIT = IMAX, WA_MAC = 2 * IMAX, WA_MAT = 3 * IMAX, WA_COS = 4 * IMAX;
-constexpr int WA_N = 4 * IMAX + 3 * CMAX;
+constexpr int WA_SAI = WA_COS + 3 * CMAX, WA_N = WA_SAI + SAIMAX;
 constexpr int WN_ACC = KMAX * 5, WN_N = KMAX * 5 + AMAX;
 struct GradWaveLDS {
-    double acc[NWAVES][WA_N];   // offsets, jitters, MA coefficients, MA timescales (per instrument / set), cosine terms
+    double acc[NWAVES][WA_N];   // offsets, jitters, MA coefficients, MA timescales (per instrument / set),
+                                // cosine terms, activity coefficients
     double nat[NWAVES][WN_N];   // natural-parameter sums per planet, then the acceleration terms
 };
 
@@ -117,9 +119,15 @@ __device__ __forceinline__ void accumulate(const ModelDev& md, const EpochDeriv&
     }
 }
 
-// Adds w = dlnL/dS_i times the cosine terms' derivatives at time t.
-__device__ __forceinline__ void cos_accumulate(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
-                                               double w, double* acc) {
+// Adds w = dlnL/dS_n times the derivatives of the terms added after the MA
+// at epoch n (time t, instrument k).
+__device__ __forceinline__ void post_accumulate(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
+                                                int n, int k, double w, double* wacc) {
+    if (md.sai_w) {
+        const int b0 = md.sai_base[k], nc = md.sai_base[k + 1] - b0;
+        for (int c = 0; c < nc; ++c) atomicAdd(&wacc[WA_SAI + b0 + c], w * md.sai[(size_t)c * md.N + n]);
+    }
+    double* acc = wacc + WA_COS;
     for (int c = 0; c < md.n_cos; ++c) {
         const double r = reduce_2pi(cos_phase(sl, c, t));
         double sn, cs;
@@ -138,7 +146,7 @@ __device__ __forceinline__ double ma_mult(const ModelDev& md, const SlotLDS& sl,
     return col < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-col / sl.ma_tau[k]));
 }
 
-template <int PREC, int WPT>
+template <int PREC, int WPT, bool POST>
 __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                     int64_t ld, double* __restrict__ out_lnl,
                                                     double* __restrict__ out_grad,
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 EpochDeriv d;
                 epoch_deriv<PREC>(md, sl, s_tile.tab, t, k, d);
                 const double err2 = s_tile.s2[j] + sl.jit2[k];
-                const double cs = md.n_cos ? epoch_cos(md, sl, s_tile.tab, t) : 0.0;
+                const double cs = (POST && has_post(md)) ? epoch_post(md, sl, s_tile.tab, t, tb + j, k) : 0.0;
                 const double r = s_tile.y[j] - d.mu - cs;
                 const double ie = rcp_path<PREC>(err2);
                 const double w = r * ie;
@@ -203,7 +211,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
                 atomicAdd(&wacc[WA_OFF + k], w);
                 if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (w * w - ie));
-                if (md.n_cos) cos_accumulate(md, sl, s_tile.tab, t, w, wacc + WA_COS);
+                if (POST && has_post(md)) post_accumulate(md, sl, s_tile.tab, t, tb + j, k, w, wacc);
             }
         }
         double ldet = live ? log(prod) + (double)pexp * LN2 : 0.0;
@@ -218,7 +226,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     } else {
         // Pass 1: the likelihood, storing eps.
         double* eps = eps_scratch + (live ? b : 0) * (int64_t)md.N;
-        lnl_v = block_lnl<PREC, WPT, false, true>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr, false, eps);
+        lnl_v = block_lnl<PREC, WPT, false, true, true, POST>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr,
+                                                             false, eps);
         __syncthreads();   // eps rows written by every wave of the slot
         // Pass 2: epochs in reverse, lambda by suffix scans.
         double lam_carry = 0.0;
@@ -242,7 +251,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     const double err2 = s_tile.s2[j] + sl.jit2[k];
                     ie = rcp_path<PREC>(err2);
                     e_i = eps[tb + j];
-                    ef = md.n_cos ? e_i - epoch_cos(md, sl, s_tile.tab, t) : e_i;
+                    ef = (POST && has_post(md)) ? e_i - epoch_post(md, sl, s_tile.tab, t, tb + j, k) : e_i;
                     u = -ef * ie;
                     const int n1 = tb + j + 1;     // the next epoch's multiplier
                     amul = 0.0;
@@ -278,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
                     atomicAdd(&wacc[WA_OFF + k], w);
                     if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (ef * ef * ie * ie - ie));
-                    if (md.n_cos) cos_accumulate(md, sl, s_tile.tab, t, -u, wacc + WA_COS);
+                    if (POST && has_post(md)) post_accumulate(md, sl, s_tile.tab, t, tb + j, k, -u, wacc);
                     // da_i/dc = a_i / c, da_i/dtau = a_i dt / tau^2 (a_i = 0 at a segment start)
                     const double col = s_tile.dt[j];
                     const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
@@ -318,12 +327,15 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     __syncthreads();
     if (live && wsub == 0) {
         const int w0 = wave;
-        for (int e = lane; e < WA_N; e += 64) {
+        for (int e = lane; e < (POST ? WA_N : WA_COS); e += 64) {
             double v = 0.0;
             for (int w = 0; w < WPT; ++w) v += s_gw.acc[w0 + w][e];
             const int grp = e / IMAX, k = e - grp * IMAX;
-            if (e >= WA_COS) (&gl.cosg[0][0])[e - WA_COS] = v;
-            else if (grp == 0 && k < md.I) gl.g[md.off_off + k] = v;
+            if (e >= WA_SAI) {
+                if (md.sai_w && e - WA_SAI < md.sai_base[md.I]) gl.g[md.off_sai + (e - WA_SAI)] = v;
+            } else if (e >= WA_COS) {
+                (&gl.cosg[0][0])[e - WA_COS] = v;
+            } else if (grp == 0 && k < md.I) gl.g[md.off_off + k] = v;
             else if (grp == 1 && md.has_jit && k < md.I) gl.g[md.off_jit + k] = v;
             else if (grp >= 2 && md.ma_active && k < (md.ma_global ? 1 : md.I))
                 gl.g[md.off_ma + 2 * k + (grp - 2)] = v;
@@ -373,7 +385,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                 gp[4] = gw;
             }
         }
-        if (lane == 63) {
+        if (POST && lane == 63) {
             for (int c = 0; c < md.n_cos; ++c) {
                 const int m = md.off_sin >= 0 ? c - 1 : c;    // -1: the sinusoid
                 const int o = m < 0 ? md.off_sin : md.off_mag;
@@ -406,13 +418,19 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     }
 }
 
+// Models without terms added after the MA run the variant that compiles
+// them out (fewer scalar registers spilled).
 template <int PREC, int WPT>
 static void launch_grad_t(const ModelDev& md, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
                           unsigned long long* nf, double* scratch, hipStream_t s) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t grid = (B + SLOTS - 1) / SLOTS;
-    hipLaunchKernelGGL((k_lnl_grad<PREC, WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, lnl, grad,
-                       nf, scratch);
+    if (has_post(md))
+        hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, true>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
+                           lnl, grad, nf, scratch);
+    else
+        hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, false>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
+                           lnl, grad, nf, scratch);
 }
 
 // The MA adjoint keeps one row of eps per vector: a model-owned scratch of

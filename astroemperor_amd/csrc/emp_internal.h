@@ -14,6 +14,7 @@ constexpr int KMAX = 8;          // Keplerians per model
 constexpr int IMAX = 16;         // instruments per model
 constexpr int AMAX = 4;          // acceleration order
 constexpr int CMAX = 3;          // cosine terms per model (a sinusoid, a magnetic cycle's two)
+constexpr int SAIMAX = 32;       // stellar-activity columns per model
 constexpr int BLOCK = 512;       // threads per workgroup (8 waves)
 constexpr int NWAVES = BLOCK / 64;
 constexpr int MAX_WAVES = 16;    // largest workgroup any kernel launches (1024 threads)
@@ -45,6 +46,10 @@ struct ModelDev {
     int off_kep, off_acc, off_off, off_jit, off_ma;
     int off_sin, off_mag; // Sinusoid / MagneticCycle block starts, -1 when absent
     int n_cos;            // cosine terms: 1 per sinusoid, 2 per magnetic cycle
+    int off_sai;          // StellarActivity block start, -1 when absent
+    int sai_w;            // most activity columns of one instrument (0: no activity terms)
+    const double* sai;    // [sai_w][N] column c of each epoch's own instrument, device order
+    const int* sai_base;  // [I + 1] first coefficient (within the block) of each instrument
     const double* t;      // [N] device order
     const double* y;      // [N]
     const double* s2;     // [N] yerr^2
@@ -100,12 +105,18 @@ struct EmpModel {
 struct ModelHost {
     emp::ModelDev dev{};
     std::vector<int> order;   // device position -> caller position
-    std::vector<double> t, y, s2, dt, fixed;
-    std::vector<int> ins, inv, free_idx, bstart, pseq;
+    std::vector<double> t, y, s2, dt, fixed, sai;
+    std::vector<int> ins, inv, free_idx, bstart, pseq, sai_base;
     std::vector<emp::PseqEntry> pseq_e;
     std::vector<EmpPriorTerm> prior, ecc;
 };
 int emp_model_host(const EmpModelDesc* d, ModelHost* h);
+// Every device array a likelihood kernel may read is set (pseq_e is only
+// used by the single-model engines and may be null).
+inline bool emp_model_dev_complete(const emp::ModelDev& m) {
+    return m.t && m.y && m.s2 && m.dt && m.ins && m.inv_free && m.free_idx && m.fixed_val && m.prior &&
+           m.block_start && m.ecc_prior && m.pseq && m.sctab && m.sai && m.sai_base;
+}
 // (sin, cos) of k pi / SC_N, k = 0..SC_N, rounded from extended precision.
 void emp_sincos_table(std::vector<double2>& tab);
 

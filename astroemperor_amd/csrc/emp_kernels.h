@@ -185,13 +185,24 @@ __device__ __forceinline__ double cos_phase(const SlotLDS& sl, int c, double t) 
     return sl.cosw[c][1] * t + sl.cosw[c][2];
 }
 
-// Sum of the cosine terms at time t.  They enter the model after the moving
-// average: in the reference's block order Sinusoid and MagneticCycle follow
-// MOAV (emp/emp.py:2628-2650), so the MA residuals do not contain them.  The
-// angle is reduced in FP64 and its cosine taken from the FP64 table on both
-// precision paths.
-__device__ __forceinline__ double epoch_cos(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t) {
+// Terms added after the moving average: in the reference's block order
+// StellarActivity, Sinusoid and MagneticCycle follow MOAV (emp/emp.py:
+// 2628-2650), so the MA residuals do not contain them.
+__host__ __device__ inline bool has_post(const ModelDev& md) { return md.n_cos > 0 || md.sai_w > 0; }
+
+// Their sum at epoch n (device order; time t, instrument k): the activity
+// columns of the epoch's instrument times their coefficients (sai00.model),
+// then the cosine terms, whose angle is reduced in FP64 and whose cosine
+// comes from the FP64 table on both precision paths.
+__device__ __forceinline__ double epoch_post(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t,
+                                             int n, int k) {
     double v = 0.0;
+    if (md.sai_w) {
+        const int b0 = md.sai_base[k], nc = md.sai_base[k + 1] - b0;
+        const double* zeta = sl.theta + md.off_sai + b0;
+#pragma unroll 1
+        for (int c = 0; c < nc; ++c) v = __builtin_fma(zeta[c], md.sai[(size_t)c * md.N + n], v);
+    }
 #pragma unroll 1
     for (int c = 0; c < md.n_cos; ++c) {
         double sn, cs;
@@ -249,13 +260,14 @@ struct ScanLDS {
 // slot must be prepared (slot_prepare) and the sin/cos table loaded.
 // With WRITE the per-epoch model and err2 are stored (device order); with
 // WEPS the per-epoch MA-filtered residual eps (= r without MA; before the
-// cosine terms are taken off).
+// terms added after the MA are taken off).
 // `resident`: the caller has already staged all epochs (md.N <= TILE) in tl.
 // `first`: the first tile's epochs, already loaded by the caller (else loaded
 // here).  With PF the next tile's loads are issued a tile ahead (launches
 // have at least TILE threads); without, each tile is loaded when its turn
-// comes (fewer registers).  COS = false compiles the cosine terms out.
-template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true, bool COS = true>
+// comes (fewer registers).  POST = false compiles the terms added after the
+// moving average (activity and cosine terms) out.
+template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true, bool POST = true>
 __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
                             bool live, int wsub, double* out_model, double* out_err2, bool resident = false,
                             double* out_eps = nullptr, const EpochRegs* first = nullptr) {
@@ -331,7 +343,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
                 }
             }
             if (valid) {
-                const double cs = (COS && md.n_cos) ? epoch_cos(md, sl, tl.tab, tl.t[j]) : 0.0;
+                const double cs = (POST && has_post(md)) ? epoch_post(md, sl, tl.tab, tl.t[j], tb + j, k) : 0.0;
                 const double res = eps - cs;
                 chi = __builtin_fma(res * res, PREC == 1 ? rcp64_1(err2) : rcp64(err2), chi);
                 prod *= err2;

@@ -77,7 +77,7 @@ extern "C" int emp_device_info(int32_t* n_cu, int32_t* clock_khz, char* arch_nam
 // ---------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------
-template <int PREC, int WPT>
+template <int PREC, int WPT, bool POST>
 __global__ __launch_bounds__(BLOCK) void k_lnl(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                int64_t ld, double* __restrict__ out,
                                                unsigned long long* nonfinite) {
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl(ModelDev md, const double* __rest
     tab_load(md, s_tile);
     __syncthreads();
     slot_prepare<PREC, WPT>(md, sl, s_tile, live, wsub, lane);
-    double v = block_lnl<PREC, WPT, false>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
+    double v = block_lnl<PREC, WPT, false, false, true, POST>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
     if (live && wsub == 0 && lane == 0) {
         if (!(v == v) || v == __builtin_inf()) {
             v = -__builtin_inf();
@@ -134,12 +134,18 @@ __global__ __launch_bounds__(BLOCK) void k_model_eval(ModelDev md, const double*
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
+// Models without terms added after the MA (activity, cosine) run the
+// variant that compiles them out (fewer registers).
 template <int PREC, int WPT>
 static void launch_lnl_t(const ModelDev& md, const double* th, int64_t B, int64_t ld, double* out,
                          unsigned long long* nf, hipStream_t s) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t grid = (B + SLOTS - 1) / SLOTS;
-    hipLaunchKernelGGL((k_lnl<PREC, WPT>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, out, nf);
+    if (has_post(md))
+        hipLaunchKernelGGL((k_lnl<PREC, WPT, true>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, out, nf);
+    else
+        hipLaunchKernelGGL((k_lnl<PREC, WPT, false>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, out,
+                           nf);
 }
 
 int emp_pick_wpt(const EmpModel* m, int64_t B, int precision) {
@@ -225,6 +231,20 @@ static int validate(const EmpModelDesc* d) {
     }
     if (d->off_sinusoid < -1 || (d->off_sinusoid >= 0 && d->off_sinusoid + 3 > d->ndim_full)) { emp_set_error("sinusoid block out of range"); return EMP_EINVAL; }
     if (d->off_magnetic < -1 || (d->off_magnetic >= 0 && d->off_magnetic + 5 > d->ndim_full)) { emp_set_error("magnetic-cycle block out of range"); return EMP_EINVAL; }
+    if (d->n_sai < 0 || d->n_sai > SAIMAX) { emp_set_error("n_sai must be in [0, %d]", SAIMAX); return EMP_EUNSUP; }
+    if ((d->n_sai > 0) != (d->off_sai >= 0) || d->off_sai < -1 || (d->off_sai >= 0 && d->off_sai + d->n_sai > d->ndim_full)) {
+        emp_set_error("stellar-activity block out of range");
+        return EMP_EINVAL;
+    }
+    if (d->n_sai > 0) {
+        if (!d->sai || !d->sai_counts) { emp_set_error("null activity arrays"); return EMP_EINVAL; }
+        int tot = 0;
+        for (int k = 0; k < d->n_ins; ++k) {
+            if (d->sai_counts[k] < 0) { emp_set_error("negative activity column count"); return EMP_EINVAL; }
+            tot += d->sai_counts[k];
+        }
+        if (tot != d->n_sai) { emp_set_error("sai_counts sum to %d, n_sai is %d", tot, d->n_sai); return EMP_EINVAL; }
+    }
     for (int i = 0; i < d->n_epochs; ++i) {
         if (d->ins[i] < 0 || d->ins[i] >= d->n_ins) { emp_set_error("instrument tag out of range at epoch %d", i); return EMP_EINVAL; }
     }
@@ -328,6 +348,31 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
     md.off_jit = d->off_jitter; md.off_ma = d->off_ma;
     md.off_sin = d->off_sinusoid; md.off_mag = d->off_magnetic;
     md.n_cos = (d->off_sinusoid >= 0 ? 1 : 0) + (d->off_magnetic >= 0 ? 2 : 0);
+    // Activity columns: each epoch keeps only its own instrument's columns
+    // (the others are zero, which the caller must honour), column-major so
+    // that lanes over consecutive epochs read consecutive addresses.
+    md.off_sai = d->n_sai > 0 ? d->off_sai : -1;
+    md.sai_w = 0;
+    h->sai_base.assign(d->n_ins + 1, 0);
+    if (d->n_sai > 0) {
+        for (int k = 0; k < d->n_ins; ++k) {
+            h->sai_base[k + 1] = h->sai_base[k] + d->sai_counts[k];
+            md.sai_w = std::max(md.sai_w, (int)d->sai_counts[k]);
+        }
+        h->sai.assign((size_t)md.sai_w * N, 0.0);
+        for (int i = 0; i < N; ++i) {
+            const int o = h->order[i], k = h->ins[i];
+            const double* row = d->sai + (size_t)o * d->n_sai;
+            for (int c = 0; c < d->n_sai; ++c) {
+                const int own = c - h->sai_base[k];
+                if (own >= 0 && own < d->sai_counts[k]) h->sai[(size_t)own * N + i] = row[c];
+                else if (row[c] != 0.0) {
+                    emp_set_error("activity column %d is non-zero at epoch %d of instrument %d", c, o, k + 1);
+                    return EMP_EINVAL;
+                }
+            }
+        }
+    }
     md.lnl_const = -0.5 * std::log(2.0 * M_PI) * (double)N;
     md.npseq = (int)h->pseq.size();
     md.t0 = d->t[0];
@@ -395,12 +440,19 @@ extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
     EMP_UP(h.ecc, md.ecc_prior);
     EMP_UP(h.pseq, md.pseq);
     EMP_UP(h.pseq_e, md.pseq_e);
+    EMP_UP(h.sai, md.sai);
+    EMP_UP(h.sai_base, md.sai_base);
     {
         std::vector<double2> tab;
         emp_sincos_table(tab);
         EMP_UP(tab, md.sctab);
     }
 #undef EMP_UP
+    if (!emp_model_dev_complete(md) || !md.pseq_e) {
+        emp_model_destroy(m);
+        emp_set_error("a device array of the model is missing");
+        return EMP_EINVAL;
+    }
     void* nf = nullptr;
     if (hipMalloc(&nf, sizeof(unsigned long long)) != hipSuccess) {
         emp_model_destroy(m);
@@ -413,6 +465,7 @@ extern "C" int emp_model_create(const EmpModelDesc* d, EmpModel** out) {
     m->desc.t = m->desc.y = m->desc.yerr = nullptr;
     m->desc.ins = nullptr; m->desc.free_index = nullptr; m->desc.fixed_value = nullptr;
     m->desc.prior = nullptr; m->desc.ecc_prior = nullptr;
+    m->desc.sai = nullptr; m->desc.sai_counts = nullptr;
     *out = m;
     return EMP_OK;
 }

@@ -172,7 +172,7 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd,
 // prior support, one slot of the workgroup per task; a workgroup whose tasks
 // all lie outside leaves at once.
 // ---------------------------------------------------------------------------
-template <int PREC, int WPT, bool COS>
+template <int PREC, int WPT, bool POST>
 __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int half, int step) {
     constexpr int SLOTS = NWAVES / WPT;
     __shared__ SlotLDS s_slot[SLOTS];
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
             sl.theta[j] = fi >= 0 ? row[fi] : md.fixed_val[j];
         }
     }
-    pt_eval_accept<PREC, WPT, true, COS>(md, pv, s_ops, live, sl, s_tile, s_scan, s_acc, slot, wsub, false,
+    pt_eval_accept<PREC, WPT, true, POST>(md, pv, s_ops, live, sl, s_tile, s_scan, s_acc, slot, wsub, false,
                               (unsigned long long*)(pd.sweep + 1));
 }
 
@@ -700,26 +700,27 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_accept(PTDev pd, int half, i
 // ---------------------------------------------------------------------------
 // With events, the launch itself records them (start and end of the
 // kernel's execution, as a kernel trace measures it).  Models without
-// cosine terms run the variant that compiles them out (it keeps the
-// one-slot variants within the scalar-register budget of 8 waves per SIMD).
-template <int PREC, int WPT, bool COS>
+// activity or cosine terms run the variant that compiles them out (it keeps
+// the one-slot variants within the scalar-register budget of 8 waves per
+// SIMD).
+template <int PREC, int WPT, bool POST>
 static void launch_half_c(const ModelDev& md, const PTDev& pd, int half, int step, hipStream_t s, hipEvent_t e0,
                           hipEvent_t e1) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t tasks = (int64_t)pd.T * pd.W2;
     const int64_t grid = (tasks + SLOTS - 1) / SLOTS;
     if (e0)
-        hipExtLaunchKernelGGL((k_pt_half<PREC, WPT, COS>), dim3((unsigned)grid), dim3(BLOCK), 0, s, e0, e1, 0, md,
+        hipExtLaunchKernelGGL((k_pt_half<PREC, WPT, POST>), dim3((unsigned)grid), dim3(BLOCK), 0, s, e0, e1, 0, md,
                               pd, half, step);
     else
-        hipLaunchKernelGGL((k_pt_half<PREC, WPT, COS>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, pd, half,
+        hipLaunchKernelGGL((k_pt_half<PREC, WPT, POST>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, pd, half,
                            step);
 }
 
 template <int PREC, int WPT>
 static void launch_half_t(const ModelDev& md, const PTDev& pd, int half, int step, hipStream_t s,
                           hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    if (md.n_cos) launch_half_c<PREC, WPT, true>(md, pd, half, step, s, e0, e1);
+    if (has_post(md)) launch_half_c<PREC, WPT, true>(md, pd, half, step, s, e0, e1);
     else launch_half_c<PREC, WPT, false>(md, pd, half, step, s, e0, e1);
 }
 

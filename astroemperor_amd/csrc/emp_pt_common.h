@@ -173,9 +173,9 @@ __device__ __forceinline__ TaskOps pt_task_ops(const PTView& pv, const Proposal&
 // Kepler tables, lnL, acceptance, and the accepted position written back.
 // Every evaluated likelihood adds one to *n_lnl.  The sin/cos table must be
 // in `tile`; with `resident` all epochs are too.  All threads of the
-// workgroup call this together (it contains barriers).  COS = false
-// compiles the cosine terms out (models without them).
-template <int PREC, int WPT, bool PF = true, bool COS = true>
+// workgroup call this together (it contains barriers).  POST = false
+// compiles the terms added after the MA out (models without them).
+template <int PREC, int WPT, bool PF = true, bool POST = true>
 __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskOps* s_ops, bool live, SlotLDS& sl,
                                TileLDS& tile, ScanLDS& scan, int* s_acc, int slot, int wsub, bool resident,
                                unsigned long long* n_lnl) {
@@ -192,7 +192,7 @@ __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskO
             if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
     }
     __syncthreads();
-    double ll = block_lnl<PREC, WPT, false, false, PF, COS>(md, sl, tile, scan, live, wsub, nullptr, nullptr,
+    double ll = block_lnl<PREC, WPT, false, false, PF, POST>(md, sl, tile, scan, live, wsub, nullptr, nullptr,
                                                             resident, nullptr, (PF && !staged) ? &e0 : nullptr);
     if (live && wsub == 0 && lane == 0) {
         const TaskOps& ops = s_ops[slot];

@@ -394,7 +394,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
 
     // Host preparation and packing of every model.
     std::vector<ModelHost> hosts(S);
-    size_t nep = 0, npar = 0, nfree = 0, nblk = 0, necc = 0, nseq = 0;
+    size_t nep = 0, npar = 0, nfree = 0, nblk = 0, necc = 0, nseq = 0, nsai = 0, nsaib = 0;
     for (int s = 0; s < S; ++s) {
         int rc = emp_model_host(&models[s], &hosts[s]);
         if (rc) {
@@ -410,6 +410,8 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
         nblk += hosts[s].bstart.size();
         necc += hosts[s].ecc.size();
         nseq += hosts[s].pseq.size();
+        nsai += hosts[s].sai.size();
+        nsaib += hosts[s].sai_base.size();
     }
     EmpSystems* p = new EmpSystems();
     p->desc = d;
@@ -423,20 +425,24 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
         }
         p->n_cu = prop.multiProcessorCount;
     }
-    std::vector<double> t(nep), y(nep), s2(nep), dt(nep), fixed(npar);
-    std::vector<int> ins(nep), inv(npar), fidx(nfree), bst(nblk), pseq(nseq);
+    std::vector<double> t(nep), y(nep), s2(nep), dt(nep), fixed(npar), sai(nsai);
+    std::vector<int> ins(nep), inv(npar), fidx(nfree), bst(nblk), pseq(nseq), saib(nsaib);
     std::vector<EmpPriorTerm> pri(npar), ecc(necc);
     std::vector<ModelDev> mds(S);
-    std::vector<size_t> o_ep(S), o_par(S), o_free(S), o_blk(S), o_ecc(S), o_seq(S);
+    std::vector<size_t> o_ep(S), o_par(S), o_free(S), o_blk(S), o_ecc(S), o_seq(S), o_sai(S), o_saib(S);
     p->D.resize(S);
     p->xoff.resize(S);
     p->rxoff.resize(S);
     {
-        size_t e = 0, q = 0, f = 0, b = 0, c = 0, g = 0;
+        size_t e = 0, q = 0, f = 0, b = 0, c = 0, g = 0, a = 0, ab = 0;
         int64_t xo = 0, ro = 0;
         for (int s = 0; s < S; ++s) {
             const ModelHost& h = hosts[s];
             o_ep[s] = e; o_par[s] = q; o_free[s] = f; o_blk[s] = b; o_ecc[s] = c; o_seq[s] = g;
+            o_sai[s] = a; o_saib[s] = ab;
+            std::copy(h.sai.begin(), h.sai.end(), sai.begin() + a);
+            std::copy(h.sai_base.begin(), h.sai_base.end(), saib.begin() + ab);
+            a += h.sai.size(); ab += h.sai_base.size();
             std::copy(h.t.begin(), h.t.end(), t.begin() + e);
             std::copy(h.y.begin(), h.y.end(), y.begin() + e);
             std::copy(h.s2.begin(), h.s2.end(), s2.begin() + e);
@@ -468,8 +474,8 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
     v.max_rec = d.record_every > 0 ? d.max_records : 0;
     const size_t STW = (size_t)S * T * W;
     int rc = 0;
-    double *dt_, *dy, *ds2, *ddt, *dfix;
-    int *dins, *dinv, *dfidx, *dbst, *dseq;
+    double *dt_, *dy, *ds2, *ddt, *dfix, *dsai;
+    int *dins, *dinv, *dfidx, *dbst, *dseq, *dsaib;
     EmpPriorTerm *dpri, *decc;
     double2* dtab;
     std::vector<double2> tab;
@@ -478,7 +484,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
         (rc = upload(p, dt, &ddt)) || (rc = upload(p, ins, &dins)) || (rc = upload(p, inv, &dinv)) ||
         (rc = upload(p, fixed, &dfix)) || (rc = upload(p, pri, &dpri)) || (rc = upload(p, fidx, &dfidx)) ||
         (rc = upload(p, bst, &dbst)) || (rc = upload(p, ecc, &decc)) || (rc = upload(p, tab, &dtab)) ||
-        (rc = upload(p, pseq, &dseq))) {
+        (rc = upload(p, pseq, &dseq)) || (rc = upload(p, sai, &dsai)) || (rc = upload(p, saib, &dsaib))) {
         emp_systems_destroy(p);
         return rc;
     }
@@ -489,6 +495,13 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
         m.free_idx = dfidx + o_free[s]; m.block_start = dbst + o_blk[s]; m.ecc_prior = decc + o_ecc[s];
         m.sctab = dtab;
         m.pseq = dseq + o_seq[s];
+        m.sai = dsai + o_sai[s];
+        m.sai_base = dsaib + o_saib[s];
+        if (!emp_model_dev_complete(m)) {
+            emp_systems_destroy(p);
+            emp_set_error("system %d: a device array of the model is missing", s);
+            return EMP_EINVAL;
+        }
         mds[s] = m;
     }
     // Queue orders: longest first (stable), and the input order.

@@ -6,7 +6,8 @@ model is flattened into an ``EmpModelDesc`` and handed to the HIP library,
 which evaluates ``my_model``/``my_likelihood``/``my_prior`` on the device.
 
 Block order follows the reference (emp/emp.py:1235, 2628-2650):
-``[Keplerian 1..k, Acceleration, Offset, Jitter, MOAV, Sinusoid, MagneticCycle]``.
+``[Keplerian 1..k, Acceleration, Offset, Jitter, MOAV, StellarActivity, Sinusoid,
+MagneticCycle]``.
 """
 from __future__ import annotations
 
@@ -18,8 +19,8 @@ from . import _abi
 from .blocks import SC_PARAMETERISATIONS
 from .data import RVData
 
-_ORDER = {"Keplerian": 0, "Acceleration": 1, "Offset": 2, "Jitter": 3, "MOAV": 4, "Sinusoid": 5,
-          "MagneticCycle": 6}
+_ORDER = {"Keplerian": 0, "Acceleration": 1, "Offset": 2, "Jitter": 3, "MOAV": 4, "StellarActivity": 5,
+          "Sinusoid": 6, "MagneticCycle": 7}
 
 
 def _as_f64(a):
@@ -90,6 +91,9 @@ class ReddModel:
                 self.kplan__ += 1
             elif b.type_ in ("Offset", "Jitter") and len(b) != self.nins__:
                 raise ValueError(f"{b.type_} block has {len(b)} parameters for {self.nins__} instruments")
+            elif b.type_ == "StellarActivity" and list(b.cornums) != list(self.data.nsai):
+                raise ValueError(f"StellarActivity block columns {list(b.cornums)} differ from the data's "
+                                 f"{list(self.data.nsai)}")
         self.ndim_full = blen
         params = [p for b in self.blocks for p in b]
         self.mod_fixed = [p.fixed for p in params]
@@ -122,6 +126,8 @@ class ReddModel:
             if b.type_ == "MOAV":
                 d["order"] = b.moav
                 d["global"] = bool(b.is_global)
+            if b.type_ == "StellarActivity":
+                d["cornums"] = list(b.cornums)
             blocks.append(d)
         params = [{"name": p.name, "prior": p.prior, "limits": list(p.limits), "prargs": p.prargs,
                    "fixed": p.fixed} for p in self.params()]
@@ -183,6 +189,9 @@ class ReddModel:
         sn, mc = self.block("Sinusoid"), self.block("MagneticCycle")
         desc.off_sinusoid = sn.slice.start if sn is not None else -1
         desc.off_magnetic = mc.slice.start if mc is not None else -1
+        sa = self.block("StellarActivity")
+        desc.off_sai = sa.slice.start if sa is not None and len(sa) else -1
+        desc.n_sai = len(sa) if sa is not None else 0
         t, y, e = _as_f64(d.X), _as_f64(d.Y), _as_f64(d.YERR)
         ins = np.ascontiguousarray(d.FLAG - 1, dtype=np.int32)
         free = np.ascontiguousarray(self.C_, dtype=np.int32)
@@ -212,7 +221,12 @@ class ReddModel:
         desc.fixed_value = fixed.ctypes.data_as(P(C.c_double))
         desc.prior = terms
         desc.ecc_prior = ecc
-        return desc, (t, y, e, ins, free, fixed, terms, ecc)
+        sai = _as_f64(d.SA)
+        counts = np.ascontiguousarray(d.nsai, dtype=np.int32)
+        if desc.n_sai:
+            desc.sai = sai.ctypes.data_as(P(C.c_double))
+            desc.sai_counts = counts.ctypes.data_as(P(C.c_int32))
+        return desc, (t, y, e, ins, free, fixed, terms, ecc, sai, counts)
 
     @property
     def device(self):

@@ -4,7 +4,8 @@ Mirrors the parts of ``astroemperor.Simulation`` (emp/emp.py:2240-2916) that
 drive the RV hot path: ``load_data``, ``set_engine('reddemcee')`` with its
 ``engine_config``/``run_config`` dictionaries, ``add_condition``, the model
 switches (``keplerian_parameterisation``, ``use_c``, ``moav``,
-``acceleration``, ``sinusoid``, ``magnetic_cycle``, ``switch_jitter``,
+``acceleration``, ``sinusoid``, ``magnetic_cycle``, ``switch_SA`` (set before
+``load_data``), ``switch_jitter``,
 ``eccentricity_*``, ``jitter_*``),
 ``run()`` and ``autorun(k_start, k_end)`` with the BIC stopping rule.  Instead
 of writing and executing a script in a subprocess (emp/emp.py:137-175,
@@ -22,7 +23,7 @@ import time
 import numpy as np
 
 from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MagneticCycleBlock, MOAVBlock,
-                     OffsetBlock, SinusoidBlock, SmartSetter)
+                     OffsetBlock, SAIBlock, SinusoidBlock, SmartSetter)
 from .data import DataWrapper, RVData
 from .model import ReddModel
 from .sampler import PTSampler
@@ -73,6 +74,8 @@ class Simulation:
         self.acceleration = 0
         self.sinusoid = 0
         self.magnetic_cycle = 0
+        self.switch_SA = False
+        self.cornums = []
         self.switch_jitter = True
         self.eccentricity_limits = [0, 1]
         self.eccentricity_prargs = [0, 0.1]
@@ -100,12 +103,18 @@ class Simulation:
         self.data_wrapper = DataWrapper(folder_name, read_loc=self.read_loc)
         self._set_data(self.data_wrapper.get_data__())
 
-    def load_arrays(self, bjd_list, rv_list, erv_list, names=None, starname="arrays"):
-        """Extension: per-instrument arrays instead of a datafiles folder."""
+    def load_arrays(self, bjd_list, rv_list, erv_list, names=None, starname="arrays", sa_list=None):
+        """Extension: per-instrument arrays instead of a datafiles folder
+        (sa_list: raw activity-index columns per instrument, or None)."""
         self.starname = starname
-        self._set_data(RVData.from_arrays(bjd_list, rv_list, erv_list, labels=names))
+        self._set_data(RVData.from_arrays(bjd_list, rv_list, erv_list, labels=names, sa_list=sa_list))
 
     def _set_data(self, data):
+        # emp.py:2305-2312: activity columns are kept only with switch_SA
+        self.cornums = list(data.nsai)
+        if not self.switch_SA:
+            data = data.without_activity()
+            self.cornums = [0] * data.nins
         self.my_data = data
         self.nins__ = data.nins
         if self.instrument_names_RV is None:
@@ -146,6 +155,9 @@ class Simulation:
     def add_acceleration(self):
         self._append(AccelerationBlock(self.acceleration))
 
+    def add_sai(self):
+        self._append(SAIBlock(self.nins__, self.cornums))
+
     def add_sinusoid(self):
         self._append(SinusoidBlock(self.sinusoid))
 
@@ -160,6 +172,8 @@ class Simulation:
             self.add_jitter()
         if self.moav["order"]:
             self.add_moav()
+        if self.switch_SA and sum(self.cornums) > 0:
+            self.add_sai()
         if self.sinusoid:
             self.add_sinusoid()
         if self.magnetic_cycle:

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define EMP_ABI_VERSION 2
+#define EMP_ABI_VERSION 3
 
 /* Error codes. */
 #define EMP_OK 0
@@ -102,6 +102,9 @@ typedef struct EmpModelDesc {
     int32_t off_magnetic;       /* MagneticCycle block (P, A1, A2, phase1, phase2) start, -1 = none
                                    (magneticcycle00.model).  Both terms are added to the model
                                    after the moving average, as their blocks follow MOAV. */
+    int32_t off_sai;            /* StellarActivity block start (one coefficient per activity
+                                   column, sai00.model), -1 = none; also added after the MA */
+    int32_t n_sai;              /* activity columns, sum of sai_counts (<= 32) */
     const double* t;            /* [N] X_ = BJD - common_t, time-sorted (host) */
     const double* y;            /* [N] Y_ */
     const double* yerr;         /* [N] YERR_ */
@@ -110,6 +113,9 @@ typedef struct EmpModelDesc {
     const double* fixed_value;  /* [ndim_full] values of fixed parameters (mod_fixed_) */
     const EmpPriorTerm* prior;  /* [ndim_full] in block order (kind ignored for fixed) */
     const EmpPriorTerm* ecc_prior; /* [n_planets] prior on e = S^2 + C^2 (EMP_PRIOR_NONE if absent) */
+    const double* sai;          /* [N][n_sai] activity columns in the epochs' order, instrument-major
+                                   columns; zero at other instruments' epochs (null if n_sai = 0) */
+    const int32_t* sai_counts;  /* [n_ins] activity columns per instrument (null if n_sai = 0) */
 } EmpModelDesc;
 
 /* Replaces: the generated module top level (data, masks, constants).

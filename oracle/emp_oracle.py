@@ -300,8 +300,8 @@ class OracleModel:
     produces (``astroemperor_amd.model.ReddModel.spec()``):
 
     * ``blocks``: list of dicts in block order with keys ``type``
-      ('Keplerian', 'Acceleration', 'Offset', 'Jitter', 'MOAV', 'Sinusoid',
-      'MagneticCycle'), ``start``,
+      ('Keplerian', 'Acceleration', 'Offset', 'Jitter', 'MOAV',
+      'StellarActivity', 'Sinusoid', 'MagneticCycle'), ``start``,
       ``len`` (full-vector slice), and for Keplerians ``parameterisation``,
       ``use_c`` and ``additional`` (derived-eccentricity priors);
       MOAV blocks carry ``order`` and ``global``.
@@ -310,8 +310,10 @@ class OracleModel:
     * ``nins``, ``semantics`` ('reference' or 'intended').
     """
 
-    def __init__(self, spec, X, Y, YERR, FLAG):
+    def __init__(self, spec, X, Y, YERR, FLAG, SAI=None):
         self.spec = spec
+        # activity columns SAI{n}_ = my_data.iloc[:, 3 + n] (emp_model.py:425-432)
+        self.SAI = None if SAI is None else np.asarray(SAI, dtype=np.float64)
         self.X = np.asarray(X, dtype=np.float64)
         self.Y = np.asarray(Y, dtype=np.float64)
         self.YERR = np.asarray(YERR, dtype=np.float64)
@@ -372,6 +374,9 @@ class OracleModel:
             elif typ == "Jitter":                                      # jitter00.model:3
                 for nin in range(self.nins):
                     err20 = err20 + self.masks[nin] * sl[nin] ** 2
+            elif typ == "StellarActivity":                             # emp_model.py:736-745
+                for n in range(b["len"]):                              # sai00.model:3
+                    model0 = model0 + sl[n] * self.SAI[:, n]
             elif typ == "Sinusoid":                                    # sinusoid00.model:2-7
                 per, A, phase = sl
                 model0 = model0 + A * np.cos(2. * np.pi / per * X + phase)

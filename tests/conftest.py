@@ -31,10 +31,11 @@ def peg51():
 
 
 def make_sim(folder, k, parameterisation=1, moav=None, jitter=True, acceleration=0,
-             conditions=(), semantics="reference", sinusoid=0, magnetic_cycle=0):
+             conditions=(), semantics="reference", sinusoid=0, magnetic_cycle=0, switch_SA=False):
     """A Simulation with the blocks autorun(k, k) would add, conditions applied."""
     from astroemperor_amd import Simulation
     sim = Simulation()
+    sim.switch_SA = switch_SA
     sim.read_loc = GOLDEN + "/"
     sim.load_data(folder)
     sim.set_engine("reddemcee")
@@ -59,4 +60,4 @@ def make_sim(folder, k, parameterisation=1, moav=None, jitter=True, acceleration
 def oracle_for(model):
     from oracle.emp_oracle import OracleModel
     d = model.data
-    return OracleModel(model.spec(), d.X, d.Y, d.YERR, d.FLAG)
+    return OracleModel(model.spec(), d.X, d.Y, d.YERR, d.FLAG, SAI=d.SA)

@@ -1,5 +1,6 @@
-"""Sinusoid and MagneticCycle terms (sinusoid00.model, magneticcycle00.model)
-on the GPU, against the oracle:
+"""Terms added after the moving average -- stellar-activity columns
+(sai00.model), Sinusoid and MagneticCycle (sinusoid00.model,
+magneticcycle00.model) -- on the GPU, against the oracle:
 
 * lnL of the batch kernel, FP64 within 1e-10 relative, the mixed path within
   1e-6 absolute of FP64, at every launch shape, with and without the moving
@@ -29,6 +30,10 @@ CASES = {
                            moav={"order": 1, "global": True}),
     "both_ins_ma": dict(folder="GJ876", k=1, sinusoid=1, magnetic_cycle=1,
                         moav={"order": 1, "global": False}, semantics="intended"),
+    "sai": dict(folder="GJ876", k=1, switch_SA=True),
+    "sai_cos_ins_ma": dict(folder="GJ876", k=2, switch_SA=True, sinusoid=1,
+                           moav={"order": 1, "global": False}, semantics="intended"),
+    "sai_global_ma": dict(folder="GJ876", k=1, switch_SA=True, moav={"order": 1, "global": True}),
 }
 
 
@@ -93,7 +98,7 @@ def test_adjoint(case):
 def test_pt_engine_stored_lnl(precision):
     from astroemperor_amd import PTSampler
     from astroemperor_amd.simulation import test_init
-    m = _sim("both_ins_ma").model
+    m = _sim("sai_cos_ins_ma").model
     T, W = 4, 32
     p0 = test_init(m, T, W, np.random.default_rng(2))
     s = PTSampler(W, m.ndim__, m, ntemps=T, betas=np.array([1.0, 0.5, 0.2, 0.05]), seed=77,
@@ -112,7 +117,8 @@ def test_pt_engine_stored_lnl(precision):
 def test_systems_stored_lnl(precision):
     from astroemperor_amd import SystemsPT
     from astroemperor_amd.simulation import test_init
-    models = [_sim("sin").model, _sim("mag").model, make_sim("51Peg", 1, parameterisation=1).model]
+    models = [_sim("sin").model, _sim("mag").model, make_sim("51Peg", 1, parameterisation=1).model,
+              _sim("sai").model]
     T, W = 4, 16
     rng = np.random.default_rng(4)
     p0 = [test_init(m, T, W, rng) for m in models]

@@ -122,3 +122,16 @@ def test_descriptor_cosine_block_offsets():
     assert desc.off_sinusoid == m2.block("Sinusoid").slice.start
     assert desc.off_magnetic == m2.block("MagneticCycle").slice.start
     assert desc.off_magnetic + 5 == desc.ndim_full
+
+
+def test_descriptor_activity_columns():
+    """off_sai / n_sai and the [N, n_sai] columns with their per-instrument counts."""
+    from tests.conftest import make_sim
+    m = make_sim("GJ876", 1, switch_SA=True).model
+    desc, keep = m.descriptor()
+    assert desc.off_sai == m.block("StellarActivity").slice.start and desc.n_sai == 10
+    assert [desc.sai_counts[k] for k in range(8)] == [0, 0, 0, 0, 6, 0, 0, 4]
+    assert desc.sai[3 * 10 + 4] == m.data.SA[3, 4]
+    m0 = make_sim("GJ876", 1).model
+    desc, _ = m0.descriptor()
+    assert desc.off_sai == -1 and desc.n_sai == 0 and not desc.sai

@@ -94,7 +94,12 @@ def lnl_torch(model, theta):
                 eps[i] = e_i
                 prev = i
         r = torch.stack(eps)
-    # Sinusoid / MagneticCycle follow MOAV in block order: added after the MA.
+    # StellarActivity, Sinusoid and MagneticCycle follow MOAV in block order:
+    # added after the MA.
+    sa = model.block("StellarActivity")
+    if sa is not None and len(sa):
+        zeta = torch.stack(full[sa.slice.start:sa.slice.start + len(sa)])
+        r = r - torch.as_tensor(d.SA) @ zeta
     sn = model.block("Sinusoid")
     if sn is not None:
         per, A, ph = full[sn.slice.start:sn.slice.start + 3]
