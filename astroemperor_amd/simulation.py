@@ -26,6 +26,7 @@ from .blocks import (AccelerationBlock, JitterBlock, KeplerianBlock, MagneticCyc
                      OffsetBlock, SAIBlock, SinusoidBlock, SmartSetter)
 from .data import DataWrapper, RVData
 from .model import ReddModel
+from .postprocess import StatsMixin
 from .sampler import PTSampler
 
 
@@ -63,7 +64,7 @@ def test_init(model, ntemps, nwalkers, rng, max_repeats=100):
     return p0
 
 
-class Simulation:
+class Simulation(StatsMixin):
     def __init__(self):
         self.time_init = time.time()
         self.read_loc = ""
@@ -92,10 +93,8 @@ class Simulation:
         self.model = None
         self.sampler = None
         self.switch_first_run = True
-        self.BIC = np.inf
-        self.like_max = -np.inf
-        self.post_max = -np.inf
         self.results = []
+        self._init_stats()
 
     # -- data (emp/emp.py:2263-2312) ----------------------------------------
     def load_data(self, folder_name):
@@ -234,39 +233,32 @@ class Simulation:
         self._postprocess()
         self.time_run = time.time() - t0
 
-    def _postprocess(self):
-        """emp.py:1401-1475: max-posterior sample of the cold chain and the
-        statistics the autorun loop compares."""
-        kw = dict(discard=self.reddemcee_discard, thin=self.reddemcee_thin, flat=True)
-        self.chain = self.sampler.get_chain(**kw)
-        self.likes = self.sampler.get_log_like(**kw)
-        self.posts = self.sampler.get_log_prob(**kw)
-        best = int(np.argmax(self.posts[0]))
-        self.post_max = float(self.posts[0][best])
-        self.like_max = float(self.likes[0][best])
-        self.fit_max = self.chain[0][best]
-        n = self.my_data.ndat
-        self.BIC = float(np.log(n) * self.model.ndim__ - 2 * self.like_max)
-        self.AIC = float(2 * self.model.ndim__ - 2 * self.like_max)
-        self.results.append(dict(k=self.kplanets__, like_max=self.like_max, post_max=self.post_max,
-                                 BIC=self.BIC, fit=self.fit_max.tolist()))
+    def _autorun_add_blocks(self):                                # emp.py:2614-2625
+        if self.switch_first_run:
+            self._autorun_add_RV_ins()
+            for _ in range(self.k_start):
+                self.add_keplerian()
+
+    def _load_next_model(self):                                   # emp.py:2773-2778
+        self.add_keplerian()
+        self.update_model()
 
     def autorun(self, k_start=0, k_end=10):
-        """emp.py:2414-2456: grow the number of Keplerians until BIC stops
-        improving by more than 5."""
-        assert k_start <= k_end, f"Invalid keplerian starting point: ({k_start}, {k_end})"
-        self._autorun_add_RV_ins()
-        for _ in range(k_start):
-            self.add_keplerian()
-        old_bic = np.inf
-        k = k_start
-        while k <= k_end:
+        """emp.py:2414-2456: run with k_start Keplerians, then add one at a time
+        while the comparison criterion (BIC by default) improves by more than
+        the tolerance (5), starting each new run from the previous run's
+        planet ranges and constraints (method 'range' by default)."""
+        self.k_start, self.k_end = k_start, k_end
+        assert self.k_start <= self.k_end, f"Invalid keplerian starting point: ({k_start}, {k_end})"
+        while self.k_start <= self.k_end:
+            self._autorun_add_blocks()
+            self._stat_holder_update()
             self.run()
-            if not (old_bic - self.BIC > 5):
+            self.switch_first_run = False
+            self.k_start += 1
+            if self._check_continue_run():
                 break
-            old_bic = self.BIC
-            k += 1
-            if k > k_end:
-                break
-            self.add_keplerian()
+            self._set_init_pos()
+            self._constrain_next_run()
+            self._load_next_model()
         return self.results

@@ -1,0 +1,81 @@
+"""End-to-end ``Simulation.autorun`` on the device engine (emp/emp.py:2414-2456):
+51 Peg from 0 Keplerians, BIC comparison, constrain-next-run by range.
+
+Checks: the loop adds the planet and goes on exactly while BIC improves by
+more than 5 (up to k_end); the statistics of each run agree with the oracle evaluated at
+the run's own point estimates (like_max = lnL at the max-posterior sample,
+chi2 and RMSE from my_model there, BIC/AIC/HQIC from like_max, DIC from lnL
+at the posterior mean); the planet's next-run limits are its 95% HDI where
+that is wider than the tolerance.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import GOLDEN, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim():
+    from astroemperor_amd import Simulation
+    sim = Simulation()
+    sim.read_loc = GOLDEN + "/"
+    sim.load_data("51Peg")
+    sim.set_engine("reddemcee")
+    sim.keplerian_parameterisation = 1
+    sim.engine_config["setup"] = np.array([5, 64, 600, 1])
+    sim.engine_config["seed"] = 1234
+    sim.run_config["burnin"] = 0.5
+    sim.add_condition(["Period 1", "limits", [3.5, 5.0]])
+    return sim
+
+
+def test_autorun_51peg():
+    sim = _sim()
+    res = sim.autorun(0, 2)
+    ks = [r["k"] for r in res]
+    print("autorun:", [(r["k"], round(r["BIC"], 2), round(r["like_max"], 3)) for r in res], flush=True)
+    assert ks == list(range(len(res))) and len(res) >= 2
+    assert res[0]["BIC"] - res[1]["BIC"] > 5             # the planet is detected
+    # the loop went on exactly while BIC improved by more than 5
+    for a, b in zip(res[:-2], res[1:-1]):
+        assert a["BIC"] - b["BIC"] > 5
+    if len(res) < 3:
+        assert res[-2]["BIC"] - res[-1]["BIC"] <= 5
+    assert res[1]["like_max"] > -875.0                    # near the recorded KAT (-869.48)
+    # for the run after k = 1 the planet's amplitude is limited to its 95% HDI
+    # from that run; the period's HDI is narrower than tol = 1e-4 of its
+    # value, so its limits stay (emp.py:2765-2767); both start from the
+    # 1-sigma ranges
+    per, amp = sim.blocks__[0][0], sim.blocks__[0][1]
+    assert (per.name, amp.name) == ("Period 1", "Amplitude 1")
+    if len(res) == 3:
+        assert per.limits == [3.5, 5.0]
+        assert 45.0 < amp.limits[0] < 55.9 < amp.limits[1] < 65.0
+        assert per.init_pos[0] <= res[1]["fit"][0] <= per.init_pos[1]
+
+
+def test_run_statistics_match_oracle():
+    sim = _sim()
+    sim._autorun_add_RV_ins()
+    sim.add_keplerian()
+    sim.run()
+    m = sim.model
+    o = oracle_for(m)
+    assert abs(o.my_likelihood(sim.fit_max) - sim.like_max) <= 1e-10 * abs(sim.like_max)
+    assert abs(o.my_prior(sim.fit_max) + sim.like_max - sim.post_max) <= 1e-9 * abs(sim.post_max)
+    ymod, err2 = o.my_model(sim.ajuste)
+    r = m.data.Y - ymod
+    assert np.isclose(sim.chi2, np.sum(r ** 2 / err2), rtol=1e-10)
+    assert np.isclose(sim.RMSE, np.sqrt(np.mean(r ** 2)), rtol=1e-10)
+    n, k = m.data.ndat, m.ndim__
+    assert np.isclose(sim.BIC, np.log(n) * k - 2 * sim.like_max, rtol=1e-14)
+    assert np.isclose(sim.AIC, 2 * k - 2 * sim.like_max, rtol=1e-14)
+    assert np.isclose(sim.HQIC, 2 * k * np.log(np.log(n)) - 2 * sim.like_max, rtol=1e-14)
+    dic = -2 * o.my_likelihood(sim.fit_mean) + np.var(-2 * sim.likes[0])
+    assert np.isclose(sim.DIC, dic, rtol=1e-10)
+    z, dz = sim.evidence
+    assert np.isfinite(z) and z < sim.like_max
+    for j, p in enumerate(q for b in sim.blocks__ for q in b if q.fixed is None):
+        assert p.value == sim.fit_max[j]
+        assert p.limits[0] <= p.value_range[0] <= p.value_range[1] <= p.limits[1]
