@@ -452,6 +452,10 @@ static int ensure_grad_scratch(EmpModel* m, int64_t B) {
 
 int emp_launch_grad_ws(const EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
                        double* eps_scratch, hipStream_t s, int precision) {
+    if (m->dev.ma_active && m->dev.ma_q > 1) {
+        emp_set_error("moving-average order %d: the adjoint implements order 1", m->dev.ma_q);
+        return EMP_EUNSUP;
+    }
     if (B <= 0) return EMP_OK;
     if (m->dev.ma_active && !eps_scratch) { emp_set_error("moving-average gradient needs a scratch buffer"); return EMP_EINVAL; }
     const int wpt = emp_pick_wpt(m, B, precision);

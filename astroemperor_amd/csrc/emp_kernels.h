@@ -190,6 +190,13 @@ __device__ __forceinline__ double cos_phase(const SlotLDS& sl, int c, double t) 
 // 2628-2650), so the MA residuals do not contain them.
 __host__ __device__ inline bool has_post(const ModelDev& md) { return md.n_cos > 0 || md.sai_w > 0; }
 
+// Model features the common models do not use: the terms added after the MA
+// and MA orders above 1.  Kernels come in two instances, with and without
+// them, so that the common instance keeps its register budget.
+__host__ __device__ inline bool needs_ext(const ModelDev& md) {
+    return has_post(md) || (md.ma_active && md.ma_q > 1);
+}
+
 // Their sum at epoch n (device order; time t, instrument k): the activity
 // columns of the epoch's instrument times their coefficients (sai00.model),
 // then the cosine terms, whose angle is reduced in FP64 and whose cosine
@@ -234,6 +241,146 @@ __device__ __forceinline__ void wave_affine_scan(double& a, double& b, int lane)
     affine_step<0x143, 0xc>(a, b);
 }
 
+// MA(q), q >= 2: the order-q recurrence (moav01.model:7-15)
+//   eps_i = r_i + sum_{c<q} a_{i,c} eps_{i-1-c},
+//   a_{i,c} = -phi_c exp(-|t_i - t_{i-1-c}| / tau_c)  (0 when epoch i-1-c is
+//   not in the segment),
+// as an affine map on the state (eps_i, ..., eps_{i-Q+1}): a companion matrix
+// whose first row is (a_{i,0}, ..., a_{i,Q-1}) plus (r_i, 0, ...).  Maps are
+// composed over the wave by DPP exactly as the order-1 scan composes its
+// (a, b) pairs; lanes without a source compose with the identity.
+template <int Q, int CTRL, int ROW_MASK>
+__device__ __forceinline__ void affq_step(double (&m)[Q][Q], double (&v)[Q]) {
+    double pm[Q][Q], pv[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+#pragma unroll
+        for (int b = 0; b < Q; ++b) pm[a][b] = dpp_f64<CTRL, ROW_MASK>(a == b ? 1.0 : 0.0, m[a][b]);
+        pv[a] = dpp_f64<CTRL, ROW_MASK>(0.0, v[a]);
+    }
+    double nm[Q][Q], nv[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        double acc = v[a];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) acc = __builtin_fma(m[a][b], pv[b], acc);
+        nv[a] = acc;
+#pragma unroll
+        for (int c = 0; c < Q; ++c) {
+            double e = 0.0;
+#pragma unroll
+            for (int b = 0; b < Q; ++b) e = __builtin_fma(m[a][b], pm[b][c], e);
+            nm[a][c] = e;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        v[a] = nv[a];
+#pragma unroll
+        for (int c = 0; c < Q; ++c) m[a][c] = nm[a][c];
+    }
+}
+
+// One chunk of the order-q MA: the lane's epoch (valid: j of the tile at tb,
+// time tj, instrument k, residual r) and the state carried in from the
+// previous chunk (`carry`, updated).  Reads the tile only before the first
+// barrier.  Returns eps of the lane's epoch.  Whole wave active; with WPT > 1
+// all waves of the workgroup call this together (one barrier).
+template <int Q, int WPT>
+__device__ __forceinline__ double ma_chunk_q(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, bool valid,
+                                             int j, int tb, double tj, int k, double r, int lane, int wave,
+                                             int wsub, int& parity, double (&carry)[4]) {
+    __shared__ double qtot[2][MAX_WAVES][Q * Q + Q];
+    double m[Q][Q], v[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        v[a] = 0.0;
+#pragma unroll
+        for (int b = 0; b < Q; ++b) m[a][b] = a == b ? 1.0 : 0.0;
+    }
+    if (valid) {
+        const int q = md.ma_q;
+        const double* ph = sl.theta + md.off_ma + 2 * q * (md.ma_global ? 0 : k);
+        const int n = tb + j;
+#pragma unroll
+        for (int c = 0; c < Q; ++c) {
+            double a = 0.0;
+            const int n1 = n - 1 - c, j1 = j - 1 - c;
+            if (c < q && n1 >= 0) {
+                const int k1 = j1 >= 0 ? tl.ins[j1] : md.ins[n1];
+                if (md.ma_global || k1 == k) {
+                    const double t1 = j1 >= 0 ? tl.t[j1] : md.t[n1];
+                    a = -(ph[2 * c] * exp(-__builtin_fabs(tj - t1) / ph[2 * c + 1]));
+                }
+            }
+            m[0][c] = a;
+        }
+#pragma unroll
+        for (int a = 1; a < Q; ++a)
+#pragma unroll
+            for (int b = 0; b < Q; ++b) m[a][b] = b == a - 1 ? 1.0 : 0.0;
+        v[0] = r;
+    }
+    affq_step<Q, 0x111, 0xf>(m, v);
+    affq_step<Q, 0x112, 0xf>(m, v);
+    affq_step<Q, 0x114, 0xf>(m, v);
+    affq_step<Q, 0x118, 0xf>(m, v);
+    affq_step<Q, 0x142, 0xa>(m, v);
+    affq_step<Q, 0x143, 0xc>(m, v);
+    double sin_[Q];
+    if (WPT == 1) {
+#pragma unroll
+        for (int a = 0; a < Q; ++a) sin_[a] = carry[a];
+    } else {
+        if (lane == 63) {
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+#pragma unroll
+                for (int b = 0; b < Q; ++b) qtot[parity][wave][a * Q + b] = m[a][b];
+                qtot[parity][wave][Q * Q + a] = v[a];
+            }
+        }
+        __syncthreads();
+        const int w0 = wave - wsub;
+        double s_all[Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a) s_all[a] = carry[a];
+        for (int w = 0; w < WPT; ++w) {
+            if (w == wsub) {
+#pragma unroll
+                for (int a = 0; a < Q; ++a) sin_[a] = s_all[a];
+            }
+            const double* T = qtot[parity][w0 + w];
+            double ns[Q];
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                double acc = T[Q * Q + a];
+#pragma unroll
+                for (int b = 0; b < Q; ++b) acc = __builtin_fma(T[a * Q + b], s_all[b], acc);
+                ns[a] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a < Q; ++a) s_all[a] = ns[a];
+        }
+#pragma unroll
+        for (int a = 0; a < Q; ++a) carry[a] = s_all[a];
+        parity ^= 1;
+    }
+    double st[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        double acc = v[a];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) acc = __builtin_fma(m[a][b], sin_[b], acc);
+        st[a] = acc;
+    }
+    if (WPT == 1) {
+#pragma unroll
+        for (int a = 0; a < Q; ++a) carry[a] = readlane_f64(st[a], 63);
+    }
+    return st[0];
+}
+
 // Suffix composition for the adjoint of that recurrence, lambda_i = b_i +
 // a_i lambda_{i+1}: afterwards lane i holds the map of lanes i..63, so that
 // lambda_i = b + a lambda_in with lambda_in the value after lane 63.
@@ -265,8 +412,9 @@ struct ScanLDS {
 // `first`: the first tile's epochs, already loaded by the caller (else loaded
 // here).  With PF the next tile's loads are issued a tile ahead (launches
 // have at least TILE threads); without, each tile is loaded when its turn
-// comes (fewer registers).  POST = false compiles the terms added after the
-// moving average (activity and cosine terms) out.
+// comes (fewer registers).  POST = false compiles out the features of
+// needs_ext: the terms added after the moving average (activity and cosine
+// terms) and MA orders above 1.
 template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true, bool POST = true>
 __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
                             bool live, int wsub, double* out_model, double* out_err2, bool resident = false,
@@ -278,6 +426,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
     double chi = 0.0, prod = 1.0;
     int pexp = 0;
     double carry = 0.0;
+    double qcarry[4] = {0.0, 0.0, 0.0, 0.0};   // MA(q >= 2) state carried between chunks
     int parity = 0;
     const bool staged = resident && md.N <= TILE;
     // With the moving average over several waves, every chunk ends with the
@@ -304,16 +453,24 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
         for (int cb = 0; cb < tn; cb += G) {
             const int j = cb + g;
             const bool valid = live && (j < tn);
-            double r = 0.0, err2 = 1.0, mu = 0.0;
+            // Everything this chunk needs from the tile is read before the MA
+            // scan: with the MA over several waves the scan's barrier is the
+            // last one before the next tile overwrites this one.
+            double r = 0.0, err2 = 1.0, mu = 0.0, tj = 0.0;
             int k = 0;
             if (valid) {
                 k = tl.ins[j];
-                mu = epoch_mu<PREC>(md, sl, tl.tab, tl.t[j], k);
+                tj = tl.t[j];
+                mu = epoch_mu<PREC>(md, sl, tl.tab, tj, k);
                 err2 = tl.s2[j] + sl.jit2[k];
                 r = tl.y[j] - mu;
             }
             double eps = r;
-            if (md.ma_active) {
+            if (POST && md.ma_active && md.ma_q > 1) {
+                eps = md.ma_q == 2
+                          ? ma_chunk_q<2, WPT>(md, sl, tl, valid, j, tb, tj, k, r, lane, wave, wsub, parity, qcarry)
+                          : ma_chunk_q<4, WPT>(md, sl, tl, valid, j, tb, tj, k, r, lane, wave, wsub, parity, qcarry);
+            } else if (md.ma_active) {
                 double a = 1.0, b = 0.0;
                 if (valid) {
                     const double dtj = tl.dt[j];
@@ -343,7 +500,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
                 }
             }
             if (valid) {
-                const double cs = (POST && has_post(md)) ? epoch_post(md, sl, tl.tab, tl.t[j], tb + j, k) : 0.0;
+                const double cs = (POST && has_post(md)) ? epoch_post(md, sl, tl.tab, tj, tb + j, k) : 0.0;
                 const double res = eps - cs;
                 chi = __builtin_fma(res * res, PREC == 1 ? rcp64_1(err2) : rcp64(err2), chi);
                 prod *= err2;

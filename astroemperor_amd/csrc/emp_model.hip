@@ -134,14 +134,14 @@ __global__ __launch_bounds__(BLOCK) void k_model_eval(ModelDev md, const double*
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-// Models without terms added after the MA (activity, cosine) run the
-// variant that compiles them out (fewer registers).
+// Models without the extended features (terms added after the MA, MA orders
+// above 1) run the variant that compiles them out (fewer registers).
 template <int PREC, int WPT>
 static void launch_lnl_t(const ModelDev& md, const double* th, int64_t B, int64_t ld, double* out,
                          unsigned long long* nf, hipStream_t s) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t grid = (B + SLOTS - 1) / SLOTS;
-    if (has_post(md))
+    if (needs_ext(md))
         hipLaunchKernelGGL((k_lnl<PREC, WPT, true>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, out, nf);
     else
         hipLaunchKernelGGL((k_lnl<PREC, WPT, false>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld, out,
@@ -268,8 +268,8 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
     int rc = validate(d);
     if (rc) return rc;
     const bool ma_active = d->ma_order > 0 && (d->ma_global || d->semantics == EMP_SEM_INTENDED);
-    if (ma_active && d->ma_order != 1) {
-        emp_set_error("moving-average order %d: the kernels implement order 1", d->ma_order);
+    if (ma_active && d->ma_order > 4) {
+        emp_set_error("moving-average order %d: the kernels implement orders 1 to 4", d->ma_order);
         return EMP_EUNSUP;
     }
     const int N = d->n_epochs;
@@ -379,7 +379,7 @@ int emp_model_host(const EmpModelDesc* d, ModelHost* h) {
     // Fixed MA timescales (the usual setup): the weight exp(-dt / tau) of
     // every epoch is the same for all parameter vectors; compute it once.
     md.ma_wfixed = 0;
-    if (ma_active) {
+    if (ma_active && d->ma_order == 1) {
         bool fixed = true;
         for (int k = 0; k < d->n_ins && fixed; ++k) {
             const int j = d->off_ma + 2 * d->ma_order * (d->ma_global ? 0 : k) + 1;

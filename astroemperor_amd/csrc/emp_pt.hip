@@ -699,10 +699,10 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_accept(PTDev pd, int half, i
 // Host side.
 // ---------------------------------------------------------------------------
 // With events, the launch itself records them (start and end of the
-// kernel's execution, as a kernel trace measures it).  Models without
-// activity or cosine terms run the variant that compiles them out (it keeps
-// the one-slot variants within the scalar-register budget of 8 waves per
-// SIMD).
+// kernel's execution, as a kernel trace measures it).  Models without the
+// extended features (activity or cosine terms, MA orders above 1) run the
+// variant that compiles them out (it keeps the one-slot variants within the
+// scalar-register budget of 8 waves per SIMD).
 template <int PREC, int WPT, bool POST>
 static void launch_half_c(const ModelDev& md, const PTDev& pd, int half, int step, hipStream_t s, hipEvent_t e0,
                           hipEvent_t e1) {
@@ -720,7 +720,7 @@ static void launch_half_c(const ModelDev& md, const PTDev& pd, int half, int ste
 template <int PREC, int WPT>
 static void launch_half_t(const ModelDev& md, const PTDev& pd, int half, int step, hipStream_t s,
                           hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    if (has_post(md)) launch_half_c<PREC, WPT, true>(md, pd, half, step, s, e0, e1);
+    if (needs_ext(md)) launch_half_c<PREC, WPT, true>(md, pd, half, step, s, e0, e1);
     else launch_half_c<PREC, WPT, false>(md, pd, half, step, s, e0, e1);
 }
 
@@ -1242,6 +1242,10 @@ extern "C" int emp_pt_set_moves(EmpPT* p, const EmpMoveDesc* moves, int32_t n) {
         }
     }
     if (!(ws + wg > 0.0)) { emp_set_error("move weights sum to zero"); return EMP_EINVAL; }
+    if (wg > 0.0 && p->model->dev.ma_active && p->model->dev.ma_q > 1) {
+        emp_set_error("the gradient move needs the adjoint, which implements MA order 1 only");
+        return EMP_EUNSUP;
+    }
     PTDev& v = p->dev;
     EMP_HIP(hipStreamSynchronize(p->stream));
     if (wg > 0.0 && !v.lg_g) {

@@ -220,7 +220,7 @@ __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskO
 // prior lp is already known (b < 0: no task for this slot in this pass):
 // proposal into LDS, then pt_eval_accept (without the tile prefetch: the
 // batched-systems kernel that calls this is register-bound).
-template <int PREC, int WPT>
+template <int PREC, int WPT, bool EXT = true>
 __device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long sweep, int half, int step, int b,
                              double lp, SlotLDS& sl, TileLDS& tile, ScanLDS& scan, int* s_acc, TaskOps* s_ops,
                              int slot, int wsub, bool resident, unsigned long long* n_lnl) {
@@ -236,7 +236,7 @@ __device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long swe
             if (md.inv_free[q] < 0) sl.theta[q] = md.fixed_val[q];
         if (wsub == 0 && lane == 0) s_ops[slot] = pt_task_ops(pv, pr, lp, md.D);
     }
-    pt_eval_accept<PREC, WPT, false>(md, pv, s_ops, live, sl, tile, scan, s_acc, slot, wsub, resident, n_lnl);
+    pt_eval_accept<PREC, WPT, false, EXT>(md, pv, s_ops, live, sl, tile, scan, s_acc, slot, wsub, resident, n_lnl);
 }
 
 __device__ __forceinline__ uint32_t swap_partner(const PTView& pv, long long sweep, int h, int i) {

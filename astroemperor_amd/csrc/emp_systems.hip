@@ -132,8 +132,10 @@ __device__ inline void sys_priors(const ModelDev& md, const PTView& pv, long lon
     }
 }
 
-// One sweep of every system.  NW waves per workgroup, WPT waves per walker.
-template <int PREC, int WPT, int NW>
+// One sweep of every system.  NW waves per workgroup, WPT waves per walker;
+// EXT: the instance with the extended model features (needs_ext), used when
+// any system of the batch has them.
+template <int PREC, int WPT, int NW, bool EXT>
 __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
     constexpr int NT = NW * 64, SLOTS = NW / WPT;
     __shared__ SlotLDS s_slot[SLOTS];
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
                 for (int q0 = 0; q0 < nlive; q0 += SLOTS) {
                     const int q = q0 + slot;
                     const int b = q < nlive ? s_list[q] : -1;
-                    pt_half_task<PREC, WPT>(md, pv, sweep, half, st, b, b >= 0 ? s_tlp[b] : -__builtin_inf(),
+                    pt_half_task<PREC, WPT, EXT>(md, pv, sweep, half, st, b, b >= 0 ? s_tlp[b] : -__builtin_inf(),
                                             s_slot[slot], s_tile, s_scan, s_acc, s_ops, slot, wsub, res, &s_cnt);
                 }
                 __syncthreads();     // this half's writes before the other half reads them
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
 // Initial prior and likelihood of every walker (the state loaded into slot =
 // position order), one workgroup per system; the same arithmetic as the
 // batched likelihood kernel with the same waves per walker.
-template <int PREC, int WPT, int NW>
+template <int PREC, int WPT, int NW, bool EXT>
 __global__ __launch_bounds__(NW * 64) void k_sys_init(SysDev sd) {
     constexpr int NT = NW * 64, SLOTS = NW / WPT;
     __shared__ SlotLDS s_slot[SLOTS];
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_init(SysDev sd) {
         __syncthreads();
         if (live && wsub == 0 && lane == 0) s_lp[slot] = prior_seq(md, sl.theta);
         slot_prepare<PREC, WPT>(md, sl, s_tile, live, wsub, lane);
-        double v = block_lnl<PREC, WPT, false, false, false>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
+        double v = block_lnl<PREC, WPT, false, false, false, EXT>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr);
         if (live && wsub == 0 && lane == 0) {
             if (!(v == v) || v == __builtin_inf()) v = -__builtin_inf();
             const double lp = s_lp[slot];
@@ -284,6 +286,7 @@ struct EmpSystems {
     hipGraphExec_t graph_exec = nullptr;
     EmpSystemsTuning tune{};
     int n_cu = 256;
+    bool ext = false;          // some system needs the extended instance (needs_ext)
     int64_t x_total = 0;
     double last_ms = -1.0;
 };
@@ -296,7 +299,9 @@ static size_t swap_bytes(int T, int W) { return (size_t)(2 * T - 1) * W * (sizeo
 #define EMP_SYS_CONFIGS(X) X(0, 1, 8) X(0, 1, 16) X(0, 2, 8) X(0, 2, 16) X(1, 1, 8) X(1, 1, 16) X(1, 2, 8) X(1, 2, 16)
 
 template <int PREC, int WPT, int NW>
-static const void* sweep_fn() { return (const void*)&k_sys_sweep<PREC, WPT, NW>; }
+static const void* sweep_fn(bool ext) {
+    return ext ? (const void*)&k_sys_sweep<PREC, WPT, NW, true> : (const void*)&k_sys_sweep<PREC, WPT, NW, false>;
+}
 
 static int launch_sweep(EmpSystems* p) {
     const EmpSystemsTuning& tu = p->tune;
@@ -305,7 +310,8 @@ static int launch_sweep(EmpSystems* p) {
     p->dev.order = tu.order == 0 ? p->d_order_lpt : p->d_order_input;
 #define EMP_CASE(P, WP, NW)                                                                                \
     if (prec == P && tu.waves_per_theta == WP && tu.waves_per_wg == NW) {                                 \
-        hipLaunchKernelGGL((k_sys_sweep<P, WP, NW>), dim3(grid), dim3(NW * 64), 0, p->stream, p->dev);   \
+        if (p->ext) hipLaunchKernelGGL((k_sys_sweep<P, WP, NW, true>), dim3(grid), dim3(NW * 64), 0, p->stream, p->dev); \
+        else hipLaunchKernelGGL((k_sys_sweep<P, WP, NW, false>), dim3(grid), dim3(NW * 64), 0, p->stream, p->dev); \
         EMP_HIP(hipGetLastError());                                                                       \
         return EMP_OK;                                                                                    \
     }
@@ -321,7 +327,8 @@ static int launch_init(EmpSystems* p) {
     const int prec = p->desc.precision;
 #define EMP_CASE(P, WP, NW)                                                                                           \
     if (prec == P && tu.waves_per_theta == WP && tu.waves_per_wg == NW) {                                            \
-        hipLaunchKernelGGL((k_sys_init<P, WP, NW>), dim3(p->desc.n_systems), dim3(NW * 64), 0, p->stream, p->dev);  \
+        if (p->ext) hipLaunchKernelGGL((k_sys_init<P, WP, NW, true>), dim3(p->desc.n_systems), dim3(NW * 64), 0, p->stream, p->dev); \
+        else hipLaunchKernelGGL((k_sys_init<P, WP, NW, false>), dim3(p->desc.n_systems), dim3(NW * 64), 0, p->stream, p->dev); \
         EMP_HIP(hipGetLastError());                                                                                  \
         return EMP_OK;                                                                                               \
     }
@@ -502,6 +509,7 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
             emp_set_error("system %d: a device array of the model is missing", s);
             return EMP_EINVAL;
         }
+        p->ext = p->ext || needs_ext(m);
         mds[s] = m;
     }
     // Queue orders: longest first (stable), and the input order.
@@ -688,7 +696,7 @@ extern "C" int emp_systems_autotune(EmpSystems* p, int32_t trial_sweeps, EmpSyst
                     int fit = 0;
                     const void* fn = nullptr;
                     const int prec = p->desc.precision;
-#define EMP_CASE(P, WP, NW) if (prec == P && c.waves_per_theta == WP && c.waves_per_wg == NW) fn = sweep_fn<P, WP, NW>();
+#define EMP_CASE(P, WP, NW) if (prec == P && c.waves_per_theta == WP && c.waves_per_wg == NW) fn = sweep_fn<P, WP, NW>(p->ext);
                     EMP_SYS_CONFIGS(EMP_CASE)
 #undef EMP_CASE
                     if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, fn, c.waves_per_wg * 64, 0) != hipSuccess)
