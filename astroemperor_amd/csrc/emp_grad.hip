@@ -45,7 +45,9 @@ struct GradLDS {
 
 // Per-wave partial sums, added over a slot's waves in wave order so that the
 // gradient does not depend on how the waves were scheduled.
-constexpr int WA_OFF = 0, WA_JIT = IMAX, WA_MAC = 2 * IMAX, WA_MAT = 3 * IMAX, WA_COS = 4 * IMAX;
+// MA coefficients and timescales: one entry per (set, order), order-minor.
+constexpr int QM = 4;
+constexpr int WA_OFF = 0, WA_JIT = IMAX, WA_MAC = 2 * IMAX, WA_MAT = WA_MAC + IMAX * QM, WA_COS = WA_MAT + IMAX * QM;
 constexpr int WA_SAI = WA_COS + 3 * CMAX, WA_N = WA_SAI + SAIMAX;
 constexpr int WN_ACC = KMAX * 5, WN_N = KMAX * 5 + AMAX;
 struct GradWaveLDS {
@@ -146,12 +148,143 @@ __device__ __forceinline__ double ma_mult(const ModelDev& md, const SlotLDS& sl,
     return col < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-col / sl.ma_tau[k]));
 }
 
-template <int PREC, int WPT, bool POST>
+// Adjoint of MA(q), q >= 2, for one chunk (epochs in reverse): with
+// a_{n,c} the forward multipliers, lambda_n = u_n + sum_{c<q} a_{n+1+c,c}
+// lambda_{n+1+c}, an affine map on the state (lambda_n, ..., lambda_{n+Q-1})
+// whose companion matrix has first row (a_{n+1,0}, ..., a_{n+Q,Q-1}).  Maps
+// are composed from the wave's last lane down (suffix), then across the
+// slot's waves from the last one.  `carry` is the state after the chunk
+// (updated to the state before it).  Reads the tile before the barrier.
+template <int Q, int WPT>
+__device__ __forceinline__ double lam_chunk_q(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, bool valid,
+                                              int j, int tb, int tn, double t, int k, double u, int lane, int wave,
+                                              int wsub, int& parity, double (&carry)[4]) {
+    __shared__ double qtot[2][MAX_WAVES][Q * Q + Q];
+    double m[Q][Q], v[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        v[a] = 0.0;
+#pragma unroll
+        for (int b = 0; b < Q; ++b) m[a][b] = a == b ? 1.0 : 0.0;
+    }
+    if (valid) {
+        const int q = md.ma_q, n = tb + j;
+        const double* ph = sl.theta + md.off_ma + 2 * q * (md.ma_global ? 0 : k);
+#pragma unroll
+        for (int c = 0; c < Q; ++c) {
+            double a = 0.0;
+            const int n2 = n + 1 + c, j2 = j + 1 + c;
+            if (c < q && n2 < md.N) {
+                const int k2 = j2 < tn ? tl.ins[j2] : md.ins[n2];
+                if (md.ma_global || k2 == k) {
+                    const double t2 = j2 < tn ? tl.t[j2] : md.t[n2];
+                    a = -(ph[2 * c] * exp(-__builtin_fabs(t2 - t) / ph[2 * c + 1]));
+                }
+            }
+            m[0][c] = a;
+        }
+#pragma unroll
+        for (int a = 1; a < Q; ++a)
+#pragma unroll
+            for (int b = 0; b < Q; ++b) m[a][b] = b == a - 1 ? 1.0 : 0.0;
+        v[0] = u;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        double pm[Q][Q], pv[Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+#pragma unroll
+            for (int b = 0; b < Q; ++b) pm[a][b] = __shfl_down(m[a][b], o, 64);
+            pv[a] = __shfl_down(v[a], o, 64);
+        }
+        if (lane + o < 64) {
+            double nm[Q][Q], nv[Q];
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                double acc = v[a];
+#pragma unroll
+                for (int b = 0; b < Q; ++b) acc = __builtin_fma(m[a][b], pv[b], acc);
+                nv[a] = acc;
+#pragma unroll
+                for (int c = 0; c < Q; ++c) {
+                    double e = 0.0;
+#pragma unroll
+                    for (int b = 0; b < Q; ++b) e = __builtin_fma(m[a][b], pm[b][c], e);
+                    nm[a][c] = e;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                v[a] = nv[a];
+#pragma unroll
+                for (int c = 0; c < Q; ++c) m[a][c] = nm[a][c];
+            }
+        }
+    }
+    double sin_[Q];
+    if (WPT == 1) {
+#pragma unroll
+        for (int a = 0; a < Q; ++a) sin_[a] = carry[a];
+    } else {
+        if (lane == 0) {
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+#pragma unroll
+                for (int b = 0; b < Q; ++b) qtot[parity][wave][a * Q + b] = m[a][b];
+                qtot[parity][wave][Q * Q + a] = v[a];
+            }
+        }
+        __syncthreads();
+        const int w0 = wave - wsub;
+        double s_all[Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a) s_all[a] = carry[a];
+        for (int w = WPT - 1; w >= 0; --w) {
+            if (w == wsub) {
+#pragma unroll
+                for (int a = 0; a < Q; ++a) sin_[a] = s_all[a];
+            }
+            const double* T = qtot[parity][w0 + w];
+            double ns[Q];
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                double acc = T[Q * Q + a];
+#pragma unroll
+                for (int b = 0; b < Q; ++b) acc = __builtin_fma(T[a * Q + b], s_all[b], acc);
+                ns[a] = acc;
+            }
+#pragma unroll
+            for (int a = 0; a < Q; ++a) s_all[a] = ns[a];
+        }
+#pragma unroll
+        for (int a = 0; a < Q; ++a) carry[a] = s_all[a];
+        parity ^= 1;
+    }
+    double st[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        double acc = v[a];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) acc = __builtin_fma(m[a][b], sin_[b], acc);
+        st[a] = acc;
+    }
+    if (WPT == 1) {
+#pragma unroll
+        for (int a = 0; a < Q; ++a) carry[a] = __shfl(st[a], 0, 64);
+    }
+    return st[0];
+}
+
+// EXT: 0 plain models, 1 with terms added after the MA, 2 also MA orders
+// above 1 (its order-q scans hold the most registers).
+template <int PREC, int WPT, int EXT>
 __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                     int64_t ld, double* __restrict__ out_lnl,
                                                     double* __restrict__ out_grad,
                                                     unsigned long long* nonfinite, double* eps_scratch) {
     constexpr int SLOTS = NWAVES / WPT;
+    constexpr bool POST = EXT >= 1;
     __shared__ SlotLDS s_slot[SLOTS];
     __shared__ GradLDS s_grad[SLOTS];
     __shared__ GradWaveLDS s_gw;
@@ -231,6 +364,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
         __syncthreads();   // eps rows written by every wave of the slot
         // Pass 2: epochs in reverse, lambda by suffix scans.
         double lam_carry = 0.0;
+        double lq_carry[4] = {0.0, 0.0, 0.0, 0.0};     // MA(q >= 2) adjoint state
+        const bool maq = EXT == 2 && md.ma_q > 1;
         int parity = 0;
         const int ntile = (md.N + TILE - 1) / TILE;
         for (int it = ntile - 1; it >= 0; --it) {
@@ -255,7 +390,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     u = -ef * ie;
                     const int n1 = tb + j + 1;     // the next epoch's multiplier
                     amul = 0.0;
-                    if (n1 < md.N) {
+                    if (!maq && n1 < md.N) {
                         const bool in_tile = j + 1 < tn;
                         const int k1 = in_tile ? s_tile.ins[j + 1] : md.ins[n1];
                         const double c1 = in_tile ? s_tile.dt[j + 1] : md.dt[n1];
@@ -263,12 +398,19 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     }
                 }
                 double sa = amul, sb = u;
-                wave_affine_suffix_scan(sa, sb, lane);
                 double lam;
-                if (WPT == 1) {
+                if (maq) {
+                    lam = md.ma_q == 2
+                              ? lam_chunk_q<2, WPT>(md, sl, s_tile, valid, j, tb, tn, t, k, u, lane, wave, wsub,
+                                                    parity, lq_carry)
+                              : lam_chunk_q<4, WPT>(md, sl, s_tile, valid, j, tb, tn, t, k, u, lane, wave, wsub,
+                                                    parity, lq_carry);
+                } else if (WPT == 1) {
+                    wave_affine_suffix_scan(sa, sb, lane);
                     lam = __builtin_fma(sa, lam_carry, sb);
                     lam_carry = __shfl(lam, 0, 64);
                 } else {
+                    wave_affine_suffix_scan(sa, sb, lane);
                     if (lane == 0) { s_scan.tot[parity][wave][0] = sa; s_scan.tot[parity][wave][1] = sb; }
                     __syncthreads();
                     const int w0 = wave - wsub;
@@ -288,18 +430,35 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                     atomicAdd(&wacc[WA_OFF + k], w);
                     if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (ef * ef * ie * ie - ie));
                     if (POST && has_post(md)) post_accumulate(md, sl, s_tile.tab, t, tb + j, k, -u, wacc);
+                    if (maq) {
+                        // d a_{n,c} / d phi_c = -exp(-dt / tau_c), d a_{n,c} / d tau_c = a_{n,c} dt / tau_c^2
+                        const int n = tb + j, set = md.ma_global ? 0 : k;
+                        const double* ph = sl.theta + md.off_ma + 2 * md.ma_q * set;
+                        for (int c = 0; c < md.ma_q; ++c) {
+                            const int n1 = n - 1 - c, j1 = j - 1 - c;
+                            if (n1 < 0) break;
+                            const int k1 = j1 >= 0 ? s_tile.ins[j1] : md.ins[n1];
+                            if (!(md.ma_global || k1 == k)) break;
+                            const double dt = __builtin_fabs(t - (j1 >= 0 ? s_tile.t[j1] : md.t[n1]));
+                            const double tau = ph[2 * c + 1];
+                            const double wgt = exp(-dt / tau);
+                            const double le = lam * eps[n1];
+                            atomicAdd(&wacc[WA_MAC + set * QM + c], le * -wgt);
+                            atomicAdd(&wacc[WA_MAT + set * QM + c], le * -(ph[2 * c] * wgt * dt / (tau * tau)));
+                        }
+                    }
                     // da_i/dc = a_i / c, da_i/dtau = a_i dt / tau^2 (a_i = 0 at a segment start)
                     const double col = s_tile.dt[j];
                     const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
-                    if (!start) {
+                    if (!maq && !start) {
                         const int n = tb + j;
                         const double e_prev = eps[n - 1];
                         const int set = md.ma_global ? 0 : k;
                         const double wgt = md.ma_wfixed ? col : exp(-col / sl.ma_tau[k]);
-                        atomicAdd(&wacc[WA_MAC + set], lam * e_prev * -wgt);
+                        atomicAdd(&wacc[WA_MAC + set * QM], lam * e_prev * -wgt);
                         if (!md.ma_wfixed) {
                             const double tau = sl.ma_tau[k];
-                            atomicAdd(&wacc[WA_MAT + set],
+                            atomicAdd(&wacc[WA_MAT + set * QM],
                                       lam * e_prev * -(sl.ma_c[k] * wgt * col / (tau * tau)));
                         }
                     }
@@ -330,15 +489,20 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
         for (int e = lane; e < (POST ? WA_N : WA_COS); e += 64) {
             double v = 0.0;
             for (int w = 0; w < WPT; ++w) v += s_gw.acc[w0 + w][e];
-            const int grp = e / IMAX, k = e - grp * IMAX;
             if (e >= WA_SAI) {
                 if (md.sai_w && e - WA_SAI < md.sai_base[md.I]) gl.g[md.off_sai + (e - WA_SAI)] = v;
             } else if (e >= WA_COS) {
                 (&gl.cosg[0][0])[e - WA_COS] = v;
-            } else if (grp == 0 && k < md.I) gl.g[md.off_off + k] = v;
-            else if (grp == 1 && md.has_jit && k < md.I) gl.g[md.off_jit + k] = v;
-            else if (grp >= 2 && md.ma_active && k < (md.ma_global ? 1 : md.I))
-                gl.g[md.off_ma + 2 * k + (grp - 2)] = v;
+            } else if (e >= WA_MAC) {
+                const bool tau = e >= WA_MAT;
+                const int r = e - (tau ? WA_MAT : WA_MAC), set = r / QM, c = r - set * QM;
+                if (md.ma_active && set < (md.ma_global ? 1 : md.I) && c < md.ma_q)
+                    gl.g[md.off_ma + 2 * md.ma_q * set + 2 * c + (tau ? 1 : 0)] = v;
+            } else if (e >= WA_JIT) {
+                if (md.has_jit && e - WA_JIT < md.I) gl.g[md.off_jit + (e - WA_JIT)] = v;
+            } else if (e < md.I) {
+                gl.g[md.off_off + e] = v;
+            }
         }
         for (int e = lane; e < WN_N; e += 64) {
             double v = 0.0;
@@ -418,18 +582,22 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     }
 }
 
-// Models without terms added after the MA run the variant that compiles
-// them out (fewer scalar registers spilled).
+// Each model runs the instance with only the features it uses: the terms
+// added after the MA cost scalar registers, MA orders above 1 vector
+// registers (the order-4 scans spill).
 template <int PREC, int WPT>
 static void launch_grad_t(const ModelDev& md, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
                           unsigned long long* nf, double* scratch, hipStream_t s) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t grid = (B + SLOTS - 1) / SLOTS;
-    if (has_post(md))
-        hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, true>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
+    if (md.ma_active && md.ma_q > 1)
+        hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, 2>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
+                           lnl, grad, nf, scratch);
+    else if (has_post(md))
+        hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, 1>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
                            lnl, grad, nf, scratch);
     else
-        hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, false>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
+        hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, 0>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
                            lnl, grad, nf, scratch);
 }
 
@@ -452,10 +620,6 @@ static int ensure_grad_scratch(EmpModel* m, int64_t B) {
 
 int emp_launch_grad_ws(const EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
                        double* eps_scratch, hipStream_t s, int precision) {
-    if (m->dev.ma_active && m->dev.ma_q > 1) {
-        emp_set_error("moving-average order %d: the adjoint implements order 1", m->dev.ma_q);
-        return EMP_EUNSUP;
-    }
     if (B <= 0) return EMP_OK;
     if (m->dev.ma_active && !eps_scratch) { emp_set_error("moving-average gradient needs a scratch buffer"); return EMP_EINVAL; }
     const int wpt = emp_pick_wpt(m, B, precision);

@@ -1242,10 +1242,6 @@ extern "C" int emp_pt_set_moves(EmpPT* p, const EmpMoveDesc* moves, int32_t n) {
         }
     }
     if (!(ws + wg > 0.0)) { emp_set_error("move weights sum to zero"); return EMP_EINVAL; }
-    if (wg > 0.0 && p->model->dev.ma_active && p->model->dev.ma_q > 1) {
-        emp_set_error("the gradient move needs the adjoint, which implements MA order 1 only");
-        return EMP_EUNSUP;
-    }
     PTDev& v = p->dev;
     EMP_HIP(hipStreamSynchronize(p->stream));
     if (wg > 0.0 && !v.lg_g) {

@@ -74,3 +74,19 @@ def test_torch_reference_cosine_terms_equal_oracle(moav):
         ref = o.my_likelihood(th)
         got = lnl_torch(m, torch.tensor(th)).item()
         assert abs(got - ref) <= 1e-10 * abs(ref), (got, ref)
+
+
+@pytest.mark.parametrize("q,glob", [(2, True), (3, False)])
+def test_torch_reference_maq_equals_oracle(q, glob):
+    """MA(q) of the torch reference against the oracle's literal loop."""
+    rng = np.random.default_rng(30 + q)
+    sim = make_sim("GJ876", 1, parameterisation=1, moav={"order": q, "global": glob}, semantics="intended")
+    m = sim.model
+    o = oracle_for(m)
+    lim = m.limits_free()
+    for _ in range(2):
+        th = lim[:, 0] + (lim[:, 1] - lim[:, 0]) * rng.uniform(0.2, 0.8, size=len(lim))
+        th[3:5] *= 0.5
+        ref = o.my_likelihood(th)
+        got = lnl_torch(m, torch.tensor(th)).item()
+        assert abs(got - ref) <= 1e-10 * abs(ref), (got, ref)

@@ -77,22 +77,22 @@ def lnl_torch(model, theta):
     r = Y - mu
     ma = model.block("MOAV")
     if ma is not None and (model.semantics == "intended" or ma.is_global):
-        # MA(1) (moav01.model:7-15): eps_i = r_i - c exp(-|t_i - t_prev|/tau) eps_prev
+        # MA(q) (moav01.model:7-15): eps_i = r_i - sum_c c_c exp(-|t_i - t_{i-1-c}|/tau_c) eps_{i-1-c}
         # along the time-ordered series (global) or each instrument's series.
-        if ma.moav != 1:
-            raise NotImplementedError("MA order 1 only")
+        q = ma.moav
         s0 = ma.slice.start
         eps = [None] * len(X)
         groups = [np.arange(len(X))] if ma.is_global else \
             [np.where(d.FLAG == i + 1)[0] for i in range(model.nins__)]
         for gi, idx in enumerate(groups):
-            c, tau = full[s0 + 2 * gi], full[s0 + 2 * gi + 1]
-            prev = None
-            for i in idx:
-                e_i = r[i] if prev is None else \
-                    r[i] - c * torch.exp(-torch.abs(X[i] - X[prev]) / tau) * eps[prev]
+            th = full[s0 + 2 * q * gi:s0 + 2 * q * (gi + 1)]
+            for pos, i in enumerate(idx):
+                e_i = r[i]
+                for c in range(q):
+                    if pos > c:
+                        p = idx[pos - 1 - c]
+                        e_i = e_i - th[2 * c] * torch.exp(-torch.abs(X[i] - X[p]) / th[2 * c + 1]) * eps[p]
                 eps[i] = e_i
-                prev = i
         r = torch.stack(eps)
     # StellarActivity, Sinusoid and MagneticCycle follow MOAV in block order:
     # added after the MA.
