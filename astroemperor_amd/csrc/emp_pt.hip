@@ -215,6 +215,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
 // otherwise the round works on them in global memory.
 // ---------------------------------------------------------------------------
 constexpr int SWAP_BLOCK = 1024;
+constexpr int SWAP_UNROLL = 4;
 
 template <bool USE_LDS>
 __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
@@ -232,14 +233,36 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     int* JP = (int*)(s_dyn + n + np) + n;            // [T-1][W] partners (LDS path)
     for (int e = tid; e < T; e += SWAP_BLOCK) s_bet[e] = pd.betas[e];
     if (USE_LDS) {
-        for (int e = tid; e < n; e += SWAP_BLOCK) {
-            const int m = pd.map[e];
-            Mp[e] = m;
-            L[e] = pd.lnl[m];
-        }
-        for (int e = tid; e < np; e += SWAP_BLOCK) {
-            LU[e] = pd.swap_lu[e];
-            JP[e] = pd.swap_jp[e];
+        // Up to SWAP_UNROLL entries per thread with all independent loads in
+        // flight together (map and the swap randoms, then the lnL through the
+        // map), instead of one dependent pair per loop trip.
+        for (int e0 = 0; e0 < n; e0 += SWAP_UNROLL * SWAP_BLOCK) {     // np < n
+            int m[SWAP_UNROLL], jp[SWAP_UNROLL];
+            double lu[SWAP_UNROLL], l[SWAP_UNROLL];
+#pragma unroll
+            for (int r = 0; r < SWAP_UNROLL; ++r) {
+                const int e = e0 + r * SWAP_BLOCK + tid;
+                m[r] = e < n ? pd.map[e] : 0;
+                lu[r] = e < np ? pd.swap_lu[e] : 0.0;
+                jp[r] = e < np ? pd.swap_jp[e] : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < SWAP_UNROLL; ++r) {
+                const int e = e0 + r * SWAP_BLOCK + tid;
+                l[r] = e < n ? pd.lnl[m[r]] : 0.0;
+            }
+#pragma unroll
+            for (int r = 0; r < SWAP_UNROLL; ++r) {
+                const int e = e0 + r * SWAP_BLOCK + tid;
+                if (e < n) {
+                    Mp[e] = m[r];
+                    L[e] = l[r];
+                }
+                if (e < np) {
+                    LU[e] = lu[r];
+                    JP[e] = jp[r];
+                }
+            }
         }
     } else {
         for (int e = tid; e < n; e += SWAP_BLOCK) L[e] = pd.lnl[pd.map[e]];
