@@ -32,6 +32,8 @@ struct PTDev {
     int adapt_mode;
     double adapt_tau, adapt_nu;
     int64_t max_sweeps;
+    int nsteps;         // stretch steps per sweep
+    int64_t max_rows;   // history rows: one per step (max_sweeps x nsteps)
     int record;
     double* X;          // [T*W slots][D]
     double* lnl;        // [slots]
@@ -47,9 +49,10 @@ struct PTDev {
     double* prop_lp;      // [T W/2] prior of each task's proposal (gradient move)
     TaskOps* rec;         // [T W/2] acceptance operands of each task's proposal (lp = -inf: outside the support)
     double* prow;         // [T W/2][D] the proposals inside the prior support
-    double* h_chain;    // [S][T][W][D]
-    double* h_lnl;      // [S][T][W]
-    double* h_lnp;      // [S][T][W]
+    double* h_chain;    // [rows][T][W][D], one row per step (the state after it; a sweep's last
+                        // row is taken after its swap round)
+    double* h_lnl;      // [rows][T][W]
+    double* h_lnp;      // [rows][T][W]
     double* h_betas;    // [S][T]
     double* h_tsw;      // [S][T-1]
     double* h_smd;      // [S][T-1]
@@ -116,11 +119,12 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.x * PRIOR_WAVES + wave;
     const PTView pv = view_of(pd);
-    if (first && pd.record && *pd.rec_owed && b < pd.T * pd.W2) {
-        // The sweep's first pass records the previous sweep's final state
-        // (nothing has moved since): walker k of half 0 and its half-1 twin.
-        const long long s = *pd.sweep - 1;
-        if (s >= 0 && s < pd.max_sweeps) {
+    if (first && pd.record && (step > 0 || *pd.rec_owed) && b < pd.T * pd.W2) {
+        // A step's first pass records the state before it (nothing has moved
+        // since): the previous step's, or the previous sweep's after its
+        // swap round.  Walker k of half 0 and its half-1 twin.
+        const long long s = *pd.sweep * pd.nsteps + step - 1;
+        if (s >= 0 && s < pd.max_rows) {
             const int t = b / pd.W2, k = b % pd.W2;
             record_walker(pd, s, t * pd.W + k, lane);
             record_walker(pd, s, t * pd.W + k + pd.W2, lane);
@@ -303,13 +307,15 @@ __global__ void k_pt_swap_draw(PTDev pd) {
     pd.swap_lu[e] = swap_logu(pv, sweep, h, i);
 }
 
-// Records the previous sweep's state if it is still owed (only_owed), or
-// unconditionally (the end of a run, which also clears the debt).
-__global__ void k_pt_record(PTDev pd, int only_owed) {
-    const long long s = *pd.sweep - 1;
-    if (only_owed && !*pd.rec_owed) return;
-    if (!only_owed && blockIdx.x == 0 && threadIdx.x == 0) *pd.rec_owed = 0;
-    if (s < 0 || s >= pd.max_sweeps) return;
+// History row of the state before step `step` of the current sweep (step 0:
+// the previous sweep's final state).  mode 0: the end of a run, which also
+// clears the debt; 1: the start of a gradient sweep, only if still owed;
+// 2: before an inner step of a gradient sweep.
+__global__ void k_pt_record(PTDev pd, int mode, int step) {
+    const long long s = *pd.sweep * pd.nsteps + step - 1;
+    if (mode == 1 && !*pd.rec_owed) return;
+    if (mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) *pd.rec_owed = 0;
+    if (s < 0 || s >= pd.max_rows) return;
     const size_t nw = (size_t)pd.T * pd.W, nx = nw * pd.D;
     double* cx = pd.h_chain + (size_t)s * nx;
     double* cl = pd.h_lnl + (size_t)s * nw;
@@ -785,14 +791,15 @@ static int launch_tail(EmpPT* p) {
     return EMP_OK;
 }
 
-// The history record of the previous sweep: only if still owed (the start
-// of a gradient sweep), or unconditionally, clearing the debt (the end of a
+// A history record (k_pt_record's modes): the previous sweep's state only if
+// still owed (1, the start of a gradient sweep), before an inner step of a
+// gradient sweep (2), or unconditionally, clearing the debt (0, the end of a
 // run).
-static int launch_record(EmpPT* p, int only_owed) {
+static int launch_record(EmpPT* p, int mode, int step = 0) {
     if (!p->dev.record) return EMP_OK;
     const size_t nx = (size_t)p->dev.T * p->dev.W * p->dev.D;
     const unsigned grid = (unsigned)std::min<size_t>((nx + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_pt_record, dim3(grid), dim3(256), 0, p->stream, p->dev, only_owed);
+    hipLaunchKernelGGL(k_pt_record, dim3(grid), dim3(256), 0, p->stream, p->dev, mode, step);
     EMP_HIP(hipGetLastError());
     return EMP_OK;
 }
@@ -800,7 +807,7 @@ static int launch_record(EmpPT* p, int only_owed) {
 static int launch_sweep(EmpPT* p) {
     for (int st = 0; st < p->desc.nsteps; ++st)
         for (int h = 0; h < 2; ++h) {
-            int rc = launch_half(p, h, st, h == 1 && st == p->desc.nsteps - 1, h == 0 && st == 0);
+            int rc = launch_half(p, h, st, h == 1 && st == p->desc.nsteps - 1, h == 0);
             if (rc) return rc;
         }
     return launch_tail(p);
@@ -821,6 +828,7 @@ static int launch_grad_sweep(EmpPT* p) {
     if (rc) return rc;
     for (int st = 0; st < p->desc.nsteps; ++st)
         for (int h = 0; h < 2; ++h) {
+            if (h == 0 && st > 0 && (rc = launch_record(p, 2, st))) return rc;
             hipLaunchKernelGGL(k_lg_precond, dim3((unsigned)d.T), dim3(LG_BLOCK), 0, p->stream, d, h, st);
             EMP_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_lg_propose, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, md, d, h, st);
@@ -938,8 +946,10 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     v.key0 = (uint32_t)d.seed; v.key1 = (uint32_t)(d.seed >> 32);
     v.adapt_mode = d.adapt_mode; v.adapt_tau = d.adapt_tau; v.adapt_nu = d.adapt_nu;
     v.max_sweeps = std::max<int64_t>(d.max_sweeps, 0);
+    v.nsteps = d.nsteps;
+    v.max_rows = v.max_sweeps * d.nsteps;
     v.record = d.record ? 1 : 0;
-    const size_t TW = (size_t)v.T * v.W, S = (size_t)v.max_sweeps;
+    const size_t TW = (size_t)v.T * v.W, S = (size_t)v.max_sweeps, R = (size_t)v.max_rows;
     int rc = 0;
     if ((rc = alloc(p, &v.X, TW * v.D)) || (rc = alloc(p, &v.lnl, TW)) || (rc = alloc(p, &v.lnp, TW)) ||
         (rc = alloc(p, &v.map, TW)) || (rc = alloc(p, &v.swap_l, TW)) ||
@@ -953,7 +963,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
         return rc;
     }
     if (v.record) {
-        if ((rc = alloc(p, &v.h_chain, S * TW * v.D)) || (rc = alloc(p, &v.h_lnl, S * TW)) || (rc = alloc(p, &v.h_lnp, S * TW))) {
+        if ((rc = alloc(p, &v.h_chain, R * TW * v.D)) || (rc = alloc(p, &v.h_lnl, R * TW)) || (rc = alloc(p, &v.h_lnp, R * TW))) {
             emp_pt_destroy(p);
             return rc;
         }
@@ -1047,7 +1057,7 @@ extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     for (int64_t sw = 0; sw < ns && !rc; ++sw) {
         for (int st = 0; st < p->desc.nsteps && !rc; ++st)
             for (int h = 0; h < 2 && !rc; ++h) {
-                rc = launch_prior(p, h, st, h == 1 && st == p->desc.nsteps - 1, h == 0 && st == 0);
+                rc = launch_prior(p, h, st, h == 1 && st == p->desc.nsteps - 1, h == 0);
                 if (rc) break;
                 rc = k < n ? launch_lik(p, h, st, ev[2 * k], ev[2 * k + 1]) : launch_lik(p, h, st);
                 ++k;
@@ -1169,6 +1179,7 @@ extern "C" int emp_pt_get(EmpPT* p, int32_t what, void* host_out, int64_t cap, i
     long long done = 0;
     EMP_HIP(hipMemcpy(&done, v.sweep, sizeof(done), hipMemcpyDeviceToHost));
     const long long S = std::min<long long>(done, v.max_sweeps);
+    const long long R = std::min<long long>(done * v.nsteps, v.max_rows);     // history rows (one per step)
     int64_t sh[4] = {1, 1, 1, 1};
     const void* src = nullptr;
     size_t elem = sizeof(double);
@@ -1179,9 +1190,9 @@ extern "C" int emp_pt_get(EmpPT* p, int32_t what, void* host_out, int64_t cap, i
     }
     bool host_src = false;
     switch (what) {
-        case EMP_PT_CHAIN: if (!v.record) goto norec; sh[0] = S; sh[1] = v.T; sh[2] = v.W; sh[3] = v.D; src = v.h_chain; break;
-        case EMP_PT_LNL: if (!v.record) goto norec; sh[0] = S; sh[1] = v.T; sh[2] = v.W; src = v.h_lnl; break;
-        case EMP_PT_LNP: if (!v.record) goto norec; sh[0] = S; sh[1] = v.T; sh[2] = v.W; src = v.h_lnp; break;
+        case EMP_PT_CHAIN: if (!v.record) goto norec; sh[0] = R; sh[1] = v.T; sh[2] = v.W; sh[3] = v.D; src = v.h_chain; break;
+        case EMP_PT_LNL: if (!v.record) goto norec; sh[0] = R; sh[1] = v.T; sh[2] = v.W; src = v.h_lnl; break;
+        case EMP_PT_LNP: if (!v.record) goto norec; sh[0] = R; sh[1] = v.T; sh[2] = v.W; src = v.h_lnp; break;
         case EMP_PT_BETAS: sh[0] = S; sh[1] = v.T; src = v.h_betas; break;
         case EMP_PT_TSW: sh[0] = S; sh[1] = std::max(v.T - 1, 0); src = v.h_tsw; break;
         case EMP_PT_SMD: sh[0] = S; sh[1] = std::max(v.T - 1, 0); src = v.h_smd; break;
@@ -1227,8 +1238,8 @@ extern "C" int emp_pt_lnl_moments(EmpPT* p, int64_t discard, double* mean, doubl
     EMP_HIP(hipStreamSynchronize(p->stream));
     long long done = 0;
     EMP_HIP(hipMemcpy(&done, v.sweep, sizeof(done), hipMemcpyDeviceToHost));
-    const long long S = std::min<long long>(done, v.max_sweeps);
-    if (discard < 0 || discard >= S) { emp_set_error("discard must lie in [0, %lld)", S); return EMP_EINVAL; }
+    const long long S = std::min<long long>(done * v.nsteps, v.max_rows);      // history rows
+    if (discard < 0 || discard >= S) { emp_set_error("discard must lie in [0, %lld) rows", S); return EMP_EINVAL; }
     double* d = nullptr;
     EMP_HIP(hipMalloc(&d, 3 * (size_t)v.T * sizeof(double)));
     hipLaunchKernelGGL(k_pt_lnl_moments, dim3((unsigned)v.T), dim3(MOM_BLOCK), 0, p->stream, v, (long long)discard, S,

@@ -147,7 +147,7 @@ class PTSampler:
                                    p0.ctypes.data, C.byref(h)))
         self._pt = h
         self._capacity = capacity
-        self._nsteps = nsteps
+        self._nsteps = int(nsteps)
         descs = move_descs(self.moves, self.ndim)
         _abi.check(L.emp_pt_set_moves(h, C.addressof(descs), len(descs)))
 
@@ -165,9 +165,8 @@ class PTSampler:
     def run_mcmc(self, initial_state, nsweeps, nsteps=1, progress=False, use_graph=True, capacity=None):
         """Run `nsweeps` sweeps from `initial_state` ([T, W, D] array or a
         PTState); continuing runs pass the previous state (or None)."""
-        if nsteps != 1:
-            raise NotImplementedError("nsteps > 1: the device records one sample per sweep; "
-                                      "run with nsteps=1")
+        if self._pt is not None and int(nsteps) != self._nsteps:
+            raise ValueError(f"nsteps is fixed at {self._nsteps} for this sampler")
         if initial_state is not None:
             p0 = initial_state.coords if isinstance(initial_state, PTState) else initial_state
             p0 = np.asarray(p0, dtype=np.float64)
@@ -205,7 +204,8 @@ class PTSampler:
         shape = (C.c_int64 * 4)()
         T, W, D = self.ntemps, self.nwalkers, self.ndim
         S = max(self._capacity, 1)
-        cap = {0: S * T * W * D, 1: S * T * W, 2: S * T * W, 3: S * T, 4: S * T, 5: S * T,
+        R = S * max(self._nsteps or 1, 1)          # history rows: one per step
+        cap = {0: R * T * W * D, 1: R * T * W, 2: R * T * W, 3: S * T, 4: S * T, 5: S * T,
                6: T * W, 7: T * W * D, 8: T * W, 9: T * W, 10: T, 11: 2}[what]
         buf = np.empty(cap, dtype=dtype)
         _abi.check(L.emp_pt_get(self._pt, what, buf.ctypes.data, buf.nbytes, shape))
@@ -214,7 +214,8 @@ class PTSampler:
 
     @property
     def iteration(self):
-        return int(self._get(11, np.int64).ravel()[0]) if self._pt else 0
+        """Stretch steps taken per walker: sweeps x nsteps (one history row each)."""
+        return int(self._get(11, np.int64).ravel()[0]) * self._nsteps if self._pt else 0
 
     @property
     def n_lnl_evaluated(self):
@@ -239,10 +240,11 @@ class PTSampler:
         return self._sel(self._get(2)[..., 0], discard, thin, flat)
 
     def get_log_prob(self, discard=0, thin=1, flat=False):
-        """Tempered log posterior beta_t lnL + ln prior with the ladder of each sweep."""
+        """Tempered log posterior beta_t lnL + ln prior, each row (step) with the
+        ladder of its sweep."""
         ll = self._get(1)[..., 0]
         lp = self._get(2)[..., 0]
-        b = self._get(3)[..., 0, 0][:, :, None]
+        b = np.repeat(self._get(3)[..., 0, 0], max(self._nsteps or 1, 1), axis=0)[:ll.shape[0], :, None]
         with np.errstate(invalid="ignore"):
             tempered = np.where(b == 0, 0.0, b * ll)
         return self._sel(tempered + lp, discard, thin, flat)

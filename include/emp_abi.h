@@ -165,13 +165,15 @@ typedef struct EmpPTDesc {
     int32_t record;             /* 1: keep chain/lnL/lnp history on device */
     double adapt_tau;           /* engine_config['adapt_tau'] */
     double adapt_nu;            /* engine_config['adapt_nu'] */
-    int64_t max_sweeps;         /* history capacity (sweeps) when record == 1 */
+    int64_t max_sweeps;         /* history capacity (sweeps; nsteps rows each) when record == 1 */
 } EmpPTDesc;
 
-/* What emp_pt_get returns. */
-#define EMP_PT_CHAIN 0      /* [S, T, W, D] positions after each sweep (history) */
-#define EMP_PT_LNL 1        /* [S, T, W] */
-#define EMP_PT_LNP 2        /* [S, T, W] log prior */
+/* What emp_pt_get returns.  The chain history has one row per stretch step
+ * (R = sweeps x nsteps rows, as reddemcee's backend counts iterations): the
+ * state after each step, a sweep's last row after its swap round. */
+#define EMP_PT_CHAIN 0      /* [R, T, W, D] positions (history) */
+#define EMP_PT_LNL 1        /* [R, T, W] */
+#define EMP_PT_LNP 2        /* [R, T, W] log prior */
 #define EMP_PT_BETAS 3      /* [S, T] ladder after each sweep */
 #define EMP_PT_TSW 4        /* [S, T-1] swap acceptance per adjacent pair per sweep */
 #define EMP_PT_SMD 5        /* [S, T-1] mean |log swap ratio| per pair per sweep */
@@ -237,7 +239,7 @@ int emp_pt_set_moves(EmpPT* pt, const EmpMoveDesc* moves, int32_t n);
 
 /* Evidence ingredients for reddemcee's get_evidence_ti / get_evidence_ss
  * (called at emp/emp.py:1434-1446; the estimators run on the host): from the
- * recorded lnL of sweeps [discard, sweeps done), per temperature t,
+ * recorded lnL of history rows [discard, rows recorded), per temperature t,
  * mean_out[t] = mean lnL, and lse_out[t] (t >= 1) = log mean exp((beta_{t-1} -
  * beta_t) lnL_t), the stepping stone to the next colder temperature under the
  * current ladder (lse_out[0] = 0), with var_out[t] its delta-method variance

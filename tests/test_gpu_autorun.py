@@ -79,3 +79,18 @@ def test_run_statistics_match_oracle():
     for j, p in enumerate(q for b in sim.blocks__ for q in b if q.fixed is None):
         assert p.value == sim.fit_max[j]
         assert p.limits[0] <= p.value_range[0] <= p.value_range[1] <= p.limits[1]
+
+
+def test_run_with_two_steps_per_sweep():
+    """The reference's default setup has nsteps = 2: the burn-in and the
+    chain are counted in steps (niter = nsteps x nsweeps, emp.py:2512-2527)."""
+    sim = _sim()
+    sim.engine_config["setup"] = np.array([3, 32, 40, 2])
+    sim._autorun_add_RV_ins()
+    sim.add_keplerian()
+    sim.run()
+    niter = 2 * 40
+    assert sim.reddemcee_discard == niter // 2
+    assert sim.chain.shape == (3, (niter - niter // 2) * 32, sim.model.ndim__)
+    assert sim.sampler.iteration == niter
+    assert np.isfinite(sim.like_max)

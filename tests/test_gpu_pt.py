@@ -278,3 +278,34 @@ def test_saved_run_round_trip(tmp_path):
     assert np.array_equal(r.get_tsw(), s.get_tsw()) and np.array_equal(r.get_smd(), s.get_smd())
     assert np.array_equal(r.acceptance_fraction, s.acceptance_fraction)
     assert np.array_equal(r.betas, s.betas)
+
+
+@pytest.mark.parametrize("gradient", [False, True])
+def test_nsteps_two_records_every_step(gradient):
+    """nsteps = 2 (the reference's default setup): one history row per step,
+    every row consistent with the oracle, the last row the final state, the
+    iteration count in steps."""
+    from astroemperor_amd import GradientMove, PTSampler, StretchMove
+    from astroemperor_amd.simulation import test_init
+    m = make_sim("51Peg", 1, parameterisation=1,
+                 conditions=[("Period 1", "limits", [4.0, 4.5]), ("Amplitude 1", "limits", [40, 70])]).model
+    T, W, ns = 3, 16, 3
+    moves = [(GradientMove(), 1.0)] if gradient else [(StretchMove(), 1.0)]
+    s = PTSampler(W, m.ndim__, m, ntemps=T, seed=21, moves=moves)
+    p0 = test_init(m, T, W, np.random.default_rng(2))
+    s.run_mcmc(p0, nsweeps=ns, nsteps=2)
+    chain, ll, lp = s.get_chain(), s.get_log_like(), s.get_log_prior()
+    assert chain.shape == (T, 2 * ns, W, m.ndim__)
+    assert s.iteration == 2 * ns
+    o = oracle_for(m)
+    for r in range(2 * ns):
+        for t in range(T):
+            for w in (0, W - 1):
+                assert abs(ll[t, r, w] - o.my_likelihood(chain[t, r, w])) <= 1e-10 * abs(ll[t, r, w]), (r, t, w)
+    st = s.get_last_sample()
+    assert np.array_equal(chain[:, -1], st.coords)
+    assert not np.array_equal(chain[:, 0], chain[:, 1])
+    assert np.all(s.acceptance_fraction <= 1.0)
+    # a different nsteps for the same sampler is refused
+    with pytest.raises(ValueError):
+        s.run_mcmc(None, nsweeps=1, nsteps=1)
