@@ -12,7 +12,9 @@ of writing and executing a script in a subprocess (emp/emp.py:137-175,
 2559-2582), the model is built in-process and sampled on the GPU.
 
 Extra ``engine_config`` keys: ``'backend'`` ('hip_fp64' or 'hip_mixed') and
-``'seed'``.  ``semantics`` selects 'reference' (default) or 'intended' MA.
+``'seed'``.  ``'moves'``, which the reference declares but never passes to
+reddemcee (emp/emp.py:2373, 576-593), reaches the sampler here (e.g.
+``[(StretchMove(), 0.8), (GradientMove(), 0.2)]``).  ``semantics`` selects 'reference' (default) or 'intended' MA.
 """
 from __future__ import annotations
 
@@ -221,7 +223,8 @@ class Simulation(StatsMixin):
                                  adapt_tau=self.engine_config["adapt_tau"],
                                  adapt_nu=self.engine_config["adapt_nu"],
                                  adapt_mode=self.engine_config["adapt_mode"], precision=prec,
-                                 seed=None if seed is None else int(rng.integers(0, 2 ** 63)))
+                                 seed=None if seed is None else int(rng.integers(0, 2 ** 63)),
+                                 moves=self.engine_config.get("moves"))
         self.sampler.D_ = np.diff(self.model.limits_free()).ravel()      # emp.py:595-602
         adapt_n = self.run_config.get("adaptation_nsweeps")
         if self.run_config.get("adaptation_batches") and adapt_n:       # endit_freeze1.scr:1-16

@@ -94,3 +94,17 @@ def test_run_with_two_steps_per_sweep():
     assert sim.chain.shape == (3, (niter - niter // 2) * 32, sim.model.ndim__)
     assert sim.sampler.iteration == niter
     assert np.isfinite(sim.like_max)
+
+
+def test_run_with_gradient_moves():
+    """engine_config['moves'] reaches the sampler: a run mixing gradient
+    (Langevin) sweeps 1:3 with stretch sweeps."""
+    from astroemperor_amd import GradientMove, StretchMove
+    sim = _sim()
+    sim.engine_config["setup"] = np.array([3, 32, 30, 1])
+    sim.engine_config["moves"] = [(StretchMove(), 0.75), (GradientMove(), 0.25)]
+    sim._autorun_add_RV_ins()
+    sim.add_keplerian()
+    sim.run()
+    assert [type(m).__name__ for m, _ in sim.sampler.moves] == ["StretchMove", "GradientMove"]
+    assert sim.chain.shape[1] == 15 * 32 and np.isfinite(sim.like_max)
