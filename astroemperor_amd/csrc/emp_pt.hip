@@ -98,7 +98,7 @@ static __host__ __device__ inline PTView view_of(const PTDev& pd) {
 // proposal outside the prior support) and, inside the support, the proposal
 // row.  The loads for the operands are issued with the rows they sit beside.
 // ---------------------------------------------------------------------------
-constexpr int PRIOR_BLOCK = 256;
+constexpr int PRIOR_BLOCK = 64;     // one wave per workgroup: the passes spread over more CUs
 constexpr int PRIOR_WAVES = PRIOR_BLOCK / 64;
 
 // Chain record of logical walker w (temperature-major) into history row s.
