@@ -44,7 +44,7 @@ PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
 # workload: rocprofv3 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE in KiB per
 # dispatch, the factors from the same counters on a 256 MiB copy (the summary
 # is profiles/k_pt_half_k_sys_sweep_hbm_traffic.md).
-TRAFFIC_KIB = {1: (505.3, 94.7), 2: (509.1, 94.7), 4: (525.4, 104.1), 8: (592.2, 99.8)}
+TRAFFIC_KIB = {1: (507.3, 94.7), 2: (511.6, 94.7), 4: (527.1, 95.6), 8: (591.3, 107.5)}
 
 
 def traffic_bytes(wpt):
