@@ -26,10 +26,13 @@ import numpy as np
 
 def _scale_activity(col, rv):
     """qol_utils.py:86-90: mean-subtract, then map [min, max] onto the file's
-    RV range."""
+    RV range.  A constant column gives 0/0; the loader's fillna(0)
+    (qol_utils.py:198) turns those NaNs into zeros."""
     col = col - col.mean()
     lo, hi = col.min(), col.max()
-    return (col - lo) / (hi - lo) * (rv.max() - rv.min()) + rv.min()
+    with np.errstate(invalid="ignore", divide="ignore"):
+        out = (col - lo) / (hi - lo) * (rv.max() - rv.min()) + rv.min()
+    return np.where(np.isnan(out), 0.0, out)
 
 
 class RVData:

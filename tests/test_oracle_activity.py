@@ -89,3 +89,14 @@ def test_oracle_term(moav):
     zeta = rng.uniform(-1, 1, 10)
     got = o.my_model(_with_zeta(m, th, zeta))[0] - base
     assert np.allclose(got, SA @ zeta, rtol=0, atol=1e-9 * np.abs(SA).max())
+
+
+def test_constant_activity_column_is_zero_filled():
+    """A constant column maps 0/0 to NaN, which the loader's fillna(0) zeroes."""
+    from astroemperor_amd.data import RVData
+    rng = np.random.default_rng(0)
+    t = np.sort(rng.uniform(0, 100, 20))
+    sa = np.stack([np.full(20, 3.0), rng.normal(size=20)], axis=1)
+    d = RVData.from_arrays([t], [rng.normal(size=20)], [np.ones(20)], sa_list=[sa])
+    assert np.all(d.SA[:, 0] == 0.0)
+    assert np.all(np.isfinite(d.SA))
