@@ -225,7 +225,7 @@ template <bool USE_LDS>
 __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     extern __shared__ double s_dyn[];
     __shared__ double s_red[64][2];
-    __shared__ double s_rate[64], s_bet[64], s_ld[64];
+    __shared__ double s_rate[64], s_smd[64], s_sig[64], s_bet[64], s_ld[64];
     const int tid = threadIdx.x;
     const long long sweep = *pd.sweep;
     const PTView pv = view_of(pd);
@@ -278,6 +278,7 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     if (tid < T - 1) {
         const double na = s_red[tid][0], di = s_red[tid][1];
         s_rate[tid] = na / W;
+        s_smd[tid] = di / W;
         if (sweep < pd.max_sweeps) {
             pd.h_tsw[(size_t)sweep * (T - 1) + tid] = na / W;
             pd.h_smd[(size_t)sweep * (T - 1) + tid] = di / W;
@@ -285,8 +286,9 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
     }
     if (USE_LDS)
         for (int e = tid; e < n; e += SWAP_BLOCK) pd.map[e] = Mp[e];
+    if (pd.adapt_mode == 2) pt_lnl_spread(pv, L, s_sig, SWAP_BLOCK);
     __syncthreads();
-    pt_ladder_update(pv, sweep, s_rate, s_bet, s_ld, SWAP_BLOCK);
+    pt_ladder_update(pv, sweep, s_rate, s_smd, s_sig, s_bet, s_ld, SWAP_BLOCK);
     __syncthreads();
     if (tid < T && sweep < pd.max_sweeps) pd.h_betas[(size_t)sweep * T + tid] = s_bet[tid];
     if (tid == 0) {
@@ -932,8 +934,8 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     if (d.nsteps < 1 || d.nsteps > (1 << 20)) { emp_set_error("nsteps must be in [1, 2^20]"); return EMP_EINVAL; }
     if (d.precision != EMP_FP64 && d.precision != EMP_MIXED) { emp_set_error("bad precision"); return EMP_EINVAL; }
     if (!(d.a > 1.0)) { emp_set_error("stretch scale a must be > 1"); return EMP_EINVAL; }
-    if (d.adapt_mode != -1 && d.adapt_mode != 0) { emp_set_error("adapt_mode must be -1 (frozen) or 0"); return EMP_EUNSUP; }
-    if (d.adapt_mode == 0 && !(d.adapt_tau > 0 && d.adapt_nu > 0)) { emp_set_error("adapt_tau and adapt_nu must be > 0"); return EMP_EINVAL; }
+    if (d.adapt_mode < -1 || d.adapt_mode > 2) { emp_set_error("adapt_mode must be -1 (frozen), 0, 1 or 2"); return EMP_EUNSUP; }
+    if (d.adapt_mode >= 0 && !(d.adapt_tau > 0 && d.adapt_nu > 0)) { emp_set_error("adapt_tau and adapt_nu must be > 0"); return EMP_EINVAL; }
     if (m->desc.ndim < 1) { emp_set_error("model has no free parameter"); return EMP_EINVAL; }
     for (int i = 0; i < d.ntemps; ++i)
         if (!(betas[i] >= 0.0 && betas[i] <= 1.0)) { emp_set_error("betas must lie in [0, 1]"); return EMP_EINVAL; }
@@ -1221,8 +1223,8 @@ norec:
 
 extern "C" int emp_pt_set_adaptation(EmpPT* p, int32_t mode, double tau, double nu) {
     if (!p) { emp_set_error("null engine"); return EMP_EINVAL; }
-    if (mode != -1 && mode != 0) { emp_set_error("adapt_mode must be -1 (frozen) or 0"); return EMP_EUNSUP; }
-    if (mode == 0 && !(tau > 0 && nu > 0)) { emp_set_error("adapt_tau and adapt_nu must be > 0"); return EMP_EINVAL; }
+    if (mode < -1 || mode > 2) { emp_set_error("adapt_mode must be -1 (frozen), 0, 1 or 2"); return EMP_EUNSUP; }
+    if (mode >= 0 && !(tau > 0 && nu > 0)) { emp_set_error("adapt_tau and adapt_nu must be > 0"); return EMP_EINVAL; }
     EMP_HIP(hipStreamSynchronize(p->stream));
     p->dev.adapt_mode = mode;
     p->dev.adapt_tau = tau;

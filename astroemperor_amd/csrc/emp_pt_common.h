@@ -328,20 +328,77 @@ __device__ inline void pt_swap_round(const PTView& pv, long long sweep, double* 
     __syncthreads();
 }
 
-// Vousden, Farr & Mandel (2016) ladder dynamics from this sweep's swap
-// acceptance `rate`: S_i = ln(T_i - T_{i-1}) moves by kappa (A_{i-1} - A_i),
-// kappa = tau / (n + tau) / nu; the coldest and hottest temperatures stay
-// fixed.  The new spacings (T_i - T_{i-1}) exp(dS_i) are formed in parallel
-// from the ladder `bet` (LDS copy of the current betas), then one thread
-// adds them up in order and writes pv.betas and bet.  All nthr threads call
-// this (it holds a barrier); s_d is LDS scratch of 64 doubles.
-__device__ inline void pt_ladder_update(const PTView& pv, long long sweep, const double* rate, double* bet,
-                                        double* s_d, int nthr) {
+// Spread of the log-likelihood at each temperature after the swap round, for
+// the specific-heat ladder: sig[t] = sample standard deviation (n - 1) of the
+// finite lnL of temperature t's W slots, 0 with fewer than two.  One wave per
+// temperature, two passes (mean, then squared deviations).  The caller
+// publishes sig with a barrier.
+__device__ inline void pt_lnl_spread(const PTView& pv, const double* L, double* sig, int nthr) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = nthr >> 6;
+    const int W = pv.W;
+    for (int t = wave; t < pv.T; t += nw) {
+        const double* l = L + (size_t)t * W;
+        double s = 0.0, c = 0.0;
+        for (int i = lane; i < W; i += 64)
+            if (l[i] > -__builtin_inf() && l[i] < __builtin_inf()) {
+                s += l[i];
+                c += 1.0;
+            }
+        s = wave_sum(s);
+        c = wave_sum(c);
+        const double mean = c > 0.0 ? s / c : 0.0;
+        double q = 0.0;
+        for (int i = lane; i < W; i += 64)
+            if (l[i] > -__builtin_inf() && l[i] < __builtin_inf()) {
+                const double d = l[i] - mean;
+                q += d * d;
+            }
+        q = wave_sum(q);
+        if (lane == 0) sig[t] = c > 1.0 ? sqrt(q / (c - 1.0)) : 0.0;
+    }
+}
+
+// Ladder dynamics after a sweep.  S_i = ln(T_i - T_{i-1}) (i = 1..T-2; the
+// coldest and hottest temperatures stay fixed) moves by kappa times a
+// balance term, kappa = tau / (n + tau) / nu:
+//   mode 0, equal swap acceptance (Vousden, Farr & Mandel 2016):
+//     A_{i-1} - A_i, A_k the acceptance of pair (k, k+1) this sweep;
+//   mode 1, equal swap distance: (d_i - d_{i-1}) / (d_i + d_{i-1}), d_k the
+//     mean |log swap ratio| of pair (k, k+1) this sweep;
+//   mode 2, specific heat (equal thermodynamic length, the metric
+//     sqrt(Var_beta lnL) dbeta = sqrt(C(beta)) / beta dbeta with the heat
+//     capacity C = beta^2 Var_beta lnL; Kofke 2002, Crooks 2007):
+//     (l_i - l_{i-1}) / (l_i + l_{i-1}), l_k = (beta_k - beta_{k+1})
+//     (sig_k + sig_{k+1}) / 2 with sig from pt_lnl_spread;
+// a pair that is closer than its neighbour (higher acceptance, smaller
+// distance or length) widens.  The normalised terms lie in [-1, 1] like the
+// acceptance difference, and are 0 when both distances are 0 or not finite.
+// The new spacings (T_i - T_{i-1}) exp(dS_i) are formed in parallel from the
+// ladder `bet` (LDS copy of the current betas), then one thread adds them up
+// in order and writes pv.betas and bet.  rate and smd hold T - 1 pair
+// values, sig T values (read in mode 2 only).  All nthr threads call this
+// (it holds a barrier); s_d is LDS scratch of 64 doubles.
+__device__ inline void pt_ladder_update(const PTView& pv, long long sweep, const double* rate, const double* smd,
+                                        const double* sig, double* bet, double* s_d, int nthr) {
     const int T = pv.T;
-    if (pv.adapt_mode != 0 || T < 3) return;
+    if (pv.adapt_mode < 0 || T < 3) return;
     const double kappa = pv.adapt_tau / ((double)sweep + pv.adapt_tau) / pv.adapt_nu;
     for (int i = (int)threadIdx.x + 1; i < T - 1; i += nthr) {
-        const double dS = kappa * (rate[i - 1] - rate[i]);
+        double dS;
+        if (pv.adapt_mode == 0) {
+            dS = kappa * (rate[i - 1] - rate[i]);
+        } else {
+            double lo, hi;                  // pair (i-1, i), pair (i, i+1)
+            if (pv.adapt_mode == 1) {
+                lo = smd[i - 1];
+                hi = smd[i];
+            } else {
+                lo = (bet[i - 1] - bet[i]) * 0.5 * (sig[i - 1] + sig[i]);
+                hi = (bet[i] - bet[i + 1]) * 0.5 * (sig[i] + sig[i + 1]);
+            }
+            const double s = lo + hi;
+            dS = (s > 0.0 && s < __builtin_inf()) ? kappa * (hi - lo) / s : 0.0;
+        }
         const double spacing = 1.0 / bet[i] - 1.0 / bet[i - 1];
         s_d[i] = spacing * exp(dS);
     }

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
     __shared__ TaskOps s_ops[SLOTS];
     __shared__ double s_bet[64], s_ld[64];
     __shared__ double s_red[64][2];
-    __shared__ double s_rate[64];
+    __shared__ double s_rate[64], s_smd[64], s_sig[64];
     __shared__ double s_tlp[SYS_MAX_TASKS];    // prior of each task's proposal
     __shared__ int s_list[SYS_MAX_TASKS];      // tasks with a finite prior, in order
     __shared__ int s_wcnt[MAX_WAVES];
@@ -200,6 +200,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
         if (tid < T - 1) {
             const double r = s_red[tid][0] / W;
             s_rate[tid] = r;
+            s_smd[tid] = s_red[tid][1] / W;
             sd.tsw_sum[(size_t)s * (T - 1) + tid] += r;
             sd.smd_sum[(size_t)s * (T - 1) + tid] += s_red[tid][1] / W;
         }
@@ -219,8 +220,9 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
                 }
             }
         }
+        if (pv.adapt_mode == 2) pt_lnl_spread(pv, L, s_sig, NT);
         __syncthreads();
-        pt_ladder_update(pv, sweep, s_rate, s_bet, s_ld, NT);
+        pt_ladder_update(pv, sweep, s_rate, s_smd, s_sig, s_bet, s_ld, NT);
         if (tid == 0) atomicAdd(&sd.ctl[3], s_cnt);
     }
     if (tid == 0) {
@@ -392,8 +394,8 @@ extern "C" int emp_systems_create(const EmpModelDesc* models, const EmpSystemsDe
     if (d.nsteps < 1) { emp_set_error("nsteps must be >= 1"); return EMP_EINVAL; }
     if (d.precision != EMP_FP64 && d.precision != EMP_MIXED) { emp_set_error("bad precision"); return EMP_EINVAL; }
     if (!(d.a > 1.0)) { emp_set_error("stretch scale a must be > 1"); return EMP_EINVAL; }
-    if (d.adapt_mode != -1 && d.adapt_mode != 0) { emp_set_error("adapt_mode must be -1 (frozen) or 0"); return EMP_EUNSUP; }
-    if (d.adapt_mode == 0 && !(d.adapt_tau > 0 && d.adapt_nu > 0)) { emp_set_error("adapt_tau and adapt_nu must be > 0"); return EMP_EINVAL; }
+    if (d.adapt_mode < -1 || d.adapt_mode > 2) { emp_set_error("adapt_mode must be -1 (frozen), 0, 1 or 2"); return EMP_EUNSUP; }
+    if (d.adapt_mode >= 0 && !(d.adapt_tau > 0 && d.adapt_nu > 0)) { emp_set_error("adapt_tau and adapt_nu must be > 0"); return EMP_EINVAL; }
     if (d.record_every < 0 || d.max_records < 0) { emp_set_error("record_every and max_records must be >= 0"); return EMP_EINVAL; }
     if (d.record_every > 0 && (d.record_temps < 1 || d.record_temps > T)) { emp_set_error("record_temps must be in [1, ntemps]"); return EMP_EINVAL; }
     for (int i = 0; i < S * T; ++i)

@@ -16,7 +16,11 @@ Moves (published methods; reddemcee/emcee are not vendored by the reference):
   * adjacent-temperature swaps, hottest pair first, walker i of the hotter
     chain paired with a keyed pseudo-random permutation of the colder chain,
     accepted with min(1, exp((beta_c - beta_h)(lnL_h - lnL_c)));
-  * ladder adaptation adapt_mode 0 (Vousden, Farr & Mandel 2016) or frozen.
+  * ladder adaptation: adapt_mode 0, equal swap acceptance (Vousden, Farr &
+    Mandel 2016); 1, equal mean |log swap ratio|; 2, specific heat (equal
+    thermodynamic length, the mode tests/02_51peg_advanced.ipynb labels
+    "Specific Heat"); -1, frozen.  reddemcee's own modes 1 and 2 are not in
+    the reference, so these two are this package's definitions.
 """
 from __future__ import annotations
 
@@ -27,6 +31,15 @@ import numpy as np
 
 from . import _abi
 from .moves import move_descs, normalise_moves
+
+
+def ladder_mode(adapt_mode):
+    """adapt_mode as the engine takes it: None means 0; -1 (frozen), 0, 1, 2."""
+    m = 0 if adapt_mode is None else int(adapt_mode)
+    if m not in (-1, 0, 1, 2):
+        raise NotImplementedError(f"adapt_mode {adapt_mode!r}: -1 (frozen), 0 (equal swap acceptance), "
+                                  "1 (equal swap distance) or 2 (specific heat)")
+    return m
 
 
 def default_ladder(ntemps, ndim):
@@ -99,8 +112,7 @@ class PTSampler:
             raise TypeError("log_prior_fn must belong to the same ReddModel")
         if ndim != model.ndim__:
             raise ValueError(f"ndim={ndim} but the model has {model.ndim__} free parameters")
-        if adapt_mode not in (0, None, -1):
-            raise NotImplementedError("adapt_mode 0 (Vousden et al. 2016) or a frozen ladder (-1) only")
+        adapt_mode = ladder_mode(adapt_mode)
         self.model = model
         self.nwalkers = int(nwalkers)
         self.ndim = int(ndim)
@@ -114,7 +126,7 @@ class PTSampler:
         self.tsw_history_bool = bool(tsw_history)
         self.smd_history_bool = bool(smd_history)
         self.adapt_tau, self.adapt_nu = float(adapt_tau), float(adapt_nu)
-        self.adapt_mode = 0 if adapt_mode in (0, None) else -1
+        self.adapt_mode = adapt_mode
         self.precision = precision
         self.a = float(a)
         self.moves = normalise_moves(moves, self.a)

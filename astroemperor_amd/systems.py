@@ -8,7 +8,7 @@ runs.  :class:`SystemsPT` runs all of them at once on the device: S models
 T x W ensemble shape, one kernel launch per sweep for the whole batch.
 
 Per system the moves are those of :class:`~astroemperor_amd.sampler.PTSampler`
-(stretch halves, hottest-first adjacent swaps, Vousden et al. 2016 ladder),
+(stretch halves, hottest-first adjacent swaps, ladder adaptation in every mode),
 and system 0 of a batch follows exactly the trajectory the single-system
 sampler would with the same seed, waves per walker and initial state.
 """
@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 from . import _abi
-from .sampler import default_ladder
+from .sampler import default_ladder, ladder_mode
 
 
 class SystemsPT:
@@ -68,7 +68,7 @@ class SystemsPT:
         d = _abi.EmpSystemsDesc()
         d.n_systems, d.ntemps, d.nwalkers, d.nsteps = S, T, W, int(nsteps)
         d.precision = prec
-        d.adapt_mode = 0 if adapt_mode in (0, None) else -1
+        d.adapt_mode = ladder_mode(adapt_mode)
         d.a = float(a)
         self.seed = int(seed) if seed is not None else int.from_bytes(os.urandom(8), "little")
         d.seed = self.seed & 0xFFFFFFFFFFFFFFFF

@@ -161,7 +161,12 @@ typedef struct EmpPTDesc {
     int32_t precision;          /* EMP_FP64 / EMP_MIXED */
     double a;                   /* stretch scale (emcee StretchMove default 2.0) */
     uint64_t seed;              /* Philox key */
-    int32_t adapt_mode;         /* -1: frozen ladder; 0: Vousden et al. 2016 dynamics */
+    int32_t adapt_mode;         /* engine_config['adapt_mode'] (emp/emp.py:2378, 806): -1 frozen
+                                 * ladder; 0 equal swap acceptance (Vousden et al. 2016); 1 equal
+                                 * mean |log swap ratio|; 2 specific heat (equal thermodynamic
+                                 * length, tests/02_51peg_advanced.ipynb "adapt_mode = 2  #
+                                 * Specific Heat").  Modes 1 and 2 are this library's own
+                                 * definitions: reddemcee's are not in the reference. */
     int32_t record;             /* 1: keep chain/lnL/lnp history on device */
     double adapt_tau;           /* engine_config['adapt_tau'] */
     double adapt_nu;            /* engine_config['adapt_nu'] */
@@ -210,7 +215,7 @@ int emp_pt_get(EmpPT* pt, int32_t what, void* host_out, int64_t capacity_bytes, 
 int emp_pt_set_state(EmpPT* pt, const double* p, const double* betas);
 
 /* Replaces: sampler.select_adjustment('00') (support/endit_freeze1.scr:11),
- * which freezes the ladder; mode -1 = frozen, 0 = Vousden et al. 2016. */
+ * which freezes the ladder; mode -1 = frozen, 0/1/2 as EmpPTDesc.adapt_mode. */
 int emp_pt_set_adaptation(EmpPT* pt, int32_t adapt_mode, double adapt_tau, double adapt_nu);
 
 /* Moves a sweep draws from (emcee's `moves=[(move, weight), ...]`, which
@@ -285,7 +290,7 @@ typedef struct EmpSystemsDesc {
     int32_t nwalkers;           /* W per system (even) */
     int32_t nsteps;             /* stretch steps per sweep */
     int32_t precision;          /* EMP_FP64 / EMP_MIXED */
-    int32_t adapt_mode;         /* -1 frozen, 0 Vousden et al. 2016 */
+    int32_t adapt_mode;         /* -1 frozen, 0/1/2 as EmpPTDesc.adapt_mode */
     double a;                   /* stretch scale */
     uint64_t seed;              /* Philox key; system s draws with s mixed into the counters */
     double adapt_tau, adapt_nu;

@@ -31,6 +31,15 @@ Pinning (see the tests that exercise each function):
                                   (tests/test_oracle_cosine_terms.py).
 * ``stretch_step`` / ``swap_step`` -- injected-random unit tests and the
                                   tempered-Gaussian moment tests.
+* ``ladder_update`` / ``lnl_spread`` -- fixed points: equal acceptances or
+                                  distances leave the ladder alone; under
+                                  sig(beta) = s / beta (a Gaussian likelihood)
+                                  a geometric ladder is the exact fixed point of
+                                  mode 2, and a perturbed one moves back toward
+                                  it; the spread against numpy's std(ddof=1)
+                                  (tests/test_oracle_golden.py).  Modes 1 and 2
+                                  follow this package's own definitions:
+                                  parity with reddemcee's is unpinned.
 * ``langevin_factor`` / ``langevin_logq`` / ``langevin_halfstep`` -- the
                                   proposal density against scipy's multivariate
                                   normal, detailed balance on random pairs, and
@@ -492,20 +501,55 @@ def swap_pair(Xh, lnlh, lnph, Xc, lnlc, lnpc, beta_h, beta_c, perm, u):
     return nacc, dist / W
 
 
-def ladder_update(betas, swap_rates, iteration, adapt_tau, adapt_nu):
-    """Vousden, Farr & Mandel (2016) ladder dynamics (adapt_mode 0).
+def lnl_spread(lnl_by_temp):
+    """Per-temperature sample standard deviation (n - 1) of the finite lnL of
+    each row of lnl_by_temp[T, W]; 0 with fewer than two finite values."""
+    out = np.zeros(len(lnl_by_temp))
+    for t, row in enumerate(np.asarray(lnl_by_temp, dtype=np.float64)):
+        f = row[np.isfinite(row)]
+        if len(f) > 1:
+            out[t] = np.sqrt(np.sum((f - f.mean()) ** 2) / (len(f) - 1))
+    return out
 
-    kappa = tau / (t + tau) / nu; S_i = ln(T_i - T_{i-1}) moves by
-    kappa * (A_{i-1} - A_i); the coldest (beta=1) and hottest temperatures
-    stay fixed.
+
+def ladder_update(betas, swap_rates, iteration, adapt_tau, adapt_nu, mode=0, smd=None, spread=None):
+    """Ladder dynamics after one sweep; the coldest (beta = 1) and hottest
+    temperatures stay fixed.
+
+    EMPEROR passes engine_config['adapt_mode'] to reddemcee
+    (emp/emp.py:806, 2378) and labels mode 2 "Specific Heat"
+    (tests/02_51peg_advanced.ipynb); reddemcee itself is not in the
+    reference.  S_i = ln(T_i - T_{i-1}), i = 1..T-2, moves by kappa times a
+    balance term, kappa = tau / (t + tau) / nu:
+      mode 0 (Vousden, Farr & Mandel 2016): A_{i-1} - A_i with the pair
+        acceptances `swap_rates`;
+      mode 1: (d_i - d_{i-1}) / (d_i + d_{i-1}) with the pair mean |log swap
+        ratio| `smd`;
+      mode 2: the same with the thermodynamic length of each pair,
+        l_k = (beta_k - beta_{k+1}) (sig_k + sig_{k+1}) / 2, sig the
+        per-temperature lnL spread `spread` (lnl_spread of the post-swap
+        state);
+    the normalised terms are 0 when the two distances sum to 0 or overflow.
     """
     betas = np.array(betas, dtype=np.float64)
     T = len(betas)
-    if T < 3:
+    if T < 3 or mode < 0:
         return betas
     kappa = adapt_tau / (iteration + adapt_tau) / adapt_nu
-    A = np.asarray(swap_rates, dtype=np.float64)
-    dS = kappa * (A[:-1] - A[1:])
+    if mode == 0:
+        A = np.asarray(swap_rates, dtype=np.float64)
+        dS = kappa * (A[:-1] - A[1:])
+    else:
+        if mode == 1:
+            d = np.asarray(smd, dtype=np.float64)
+        else:
+            sig = np.asarray(spread, dtype=np.float64)
+            d = (betas[:-1] - betas[1:]) * 0.5 * (sig[:-1] + sig[1:])
+        lo, hi = d[:-1], d[1:]
+        tot = lo + hi
+        ok = (tot > 0) & np.isfinite(tot)
+        dS = np.zeros(T - 2)
+        dS[ok] = kappa * (hi[ok] - lo[ok]) / tot[ok]
     with np.errstate(divide="ignore"):
         temps = 1.0 / betas
     dT = np.diff(temps[:-1]) * np.exp(dS)

@@ -55,3 +55,11 @@ def test_ma_order_above_four_refused():
     desc, keep = m.descriptor()
     assert _create(desc) == -4
     assert b"orders 1 to 4" in _abi.lib().emp_last_error()
+
+
+def test_ladder_mode_values():
+    from astroemperor_amd.sampler import ladder_mode
+    assert [ladder_mode(m) for m in (None, -1, 0, 1, 2)] == [0, -1, 0, 1, 2]
+    for bad in (3, -2):
+        with pytest.raises(NotImplementedError):
+            ladder_mode(bad)

@@ -210,7 +210,11 @@ def test_graph_equals_direct():
         assert np.array_equal(x, y)
 
 
-def test_ladder_adaptation_matches_oracle():
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_ladder_adaptation_matches_oracle(mode):
+    """Each sweep's ladder update from that sweep's statistics: the swap
+    acceptances (mode 0), the mean |log swap ratio| (mode 1) or the lnL spread
+    of the post-swap state, which is the sweep's history row (mode 2)."""
     from astroemperor_amd import PTSampler
     from astroemperor_amd.simulation import test_init
     from oracle import emp_oracle as orc
@@ -219,16 +223,41 @@ def test_ladder_adaptation_matches_oracle():
     T, W = 6, 32
     betas = np.array([1.0, 0.6, 0.3, 0.1, 0.02, 0.0])
     p0 = test_init(m, T, W, np.random.default_rng(5))
-    s = PTSampler(W, m.ndim__, m, ntemps=T, betas=betas, seed=3, adapt_tau=50, adapt_nu=2)
+    s = PTSampler(W, m.ndim__, m, ntemps=T, betas=betas, seed=3, adapt_tau=50, adapt_nu=2, adapt_mode=mode)
     s.run_mcmc(p0, nsweeps=30)
     bh = s.get_betas()                  # ladder after each sweep
-    tsw = s.get_tsw()
+    tsw, smd, ll = s.get_tsw(), s.get_smd(), s.get_log_like()
     prev = betas
+    moved = 0.0
     for n in range(30):
-        want = orc.ladder_update(prev, tsw[n], n, 50.0, 2.0)
+        want = orc.ladder_update(prev, tsw[n], n, 50.0, 2.0, mode=mode, smd=smd[n],
+                                 spread=orc.lnl_spread(ll[:, n, :]))
         assert np.allclose(bh[n], want, rtol=1e-12, atol=0), n
+        moved = max(moved, np.max(np.abs(bh[n] - prev)))
         prev = bh[n]
+    assert moved > 1e-3
     assert bh[-1][0] == 1.0 and bh[-1][-1] == 0.0 and np.all(np.diff(bh[-1]) < 0)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_ladder_modes_reach_geometric_on_gaussian(mode):
+    """On a Gaussian likelihood the swap acceptance, the swap distance and the
+    thermodynamic length of a pair all depend on beta_hot / beta_cold only,
+    so every mode's fixed point is the geometric ladder.  A badly spaced
+    start must end within 10% of it in each log spacing."""
+    from astroemperor_amd import PTSampler
+    rng = np.random.default_rng(17)
+    m, mean, var = _gaussian_model(rng)
+    betas = np.array([1.0, 0.9, 0.5, 0.05, 0.002, 1e-4])
+    T, W, S = len(betas), 128, 3000
+    p0 = np.stack([mean + rng.normal(size=(W, 3)) * np.sqrt(var / b) for b in betas])
+    s = PTSampler(W, 3, m, ntemps=T, betas=betas, seed=21, adapt_tau=100, adapt_nu=1, adapt_mode=mode)
+    s.run_mcmc(np.clip(p0, -199, 199), nsweeps=S)
+    bh = s.get_betas()
+    assert bh[-1][0] == 1.0 and bh[-1][-1] == 1e-4
+    steps = -np.diff(np.log(bh[-500:]).mean(0))
+    print(f"mode {mode}: log spacings {np.round(steps, 3)} (geometric {np.log(1e4) / 5:.3f})", flush=True)
+    assert np.all(np.abs(steps / (np.log(1e4) / 5) - 1) < 0.10)
 
 
 def test_history_survives_set_state_between_runs():

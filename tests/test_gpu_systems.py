@@ -66,6 +66,36 @@ def test_system0_matches_single_engine(precision, case):
     assert sysb.nsweeps == n
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_system0_ladder_modes_match_single_engine(mode):
+    """The swap-distance and specific-heat ladders in the batched engine:
+    system 0's ladder and state equal the single engine's bit for bit (the
+    per-temperature spread is one wave per temperature in both)."""
+    from astroemperor_amd import PTSampler, SystemsPT
+    m0 = make_sim("51Peg", 1, parameterisation=1).model
+    others = _models_small(2)
+    T, W, seed, n = 5, 32, 0x5eed_0002, 8
+    p0 = _p0([m0] + others, T, W, 13)
+    betas = np.array([1.0, 0.6, 0.3, 0.1, 0.0])
+    m0.set_waves_per_theta(1)
+    try:
+        ref = PTSampler(W, m0.ndim__, m0, ntemps=T, betas=betas, seed=seed, adapt_tau=20, adapt_nu=2,
+                        adapt_mode=mode, precision="fp64")
+        ref.run_mcmc(p0[0], nsweeps=n)
+        st = ref.get_last_sample()
+    finally:
+        m0.set_waves_per_theta(0)
+    sysb = SystemsPT([m0] + others, T, W, p0, betas=betas, seed=seed, adapt_tau=20, adapt_nu=2,
+                     adapt_mode=mode, precision="fp64")
+    sysb.set_tuning(waves_per_wg=16, waves_per_theta=1)
+    sysb.run(n)
+    x, ll, _ = sysb.state(0)
+    assert np.array_equal(x, st.coords)
+    assert np.array_equal(ll, st.log_like)
+    assert np.array_equal(sysb.betas[0], st.betas)
+    assert not np.array_equal(st.betas, betas)
+
+
 def _replay_sweep_sys(o, X, lnl, lnp, betas, sweep, seed, sysmix, a=2.0):
     from oracle import emp_oracle as orc
     T, W, D = X.shape

@@ -141,3 +141,49 @@ def test_ladder_update_fixed_points():
     # a colder pair accepting more than the next pushes T_1 up (beta_1 down)
     nb = orc.ladder_update(betas, [0.6, 0.2, 0.2], 0, 1000, 1)
     assert nb[1] < 0.5 and nb[0] == 1.0 and nb[-1] == 0.0
+
+
+def test_ladder_modes_swap_distance():
+    betas = np.array([1.0, 0.5, 0.25, 0.0])
+    # equal mean |log swap ratio|: no motion; zero distances: no motion, no NaN
+    for d in ([0.4, 0.4, 0.4], [0.0, 0.0, 0.0]):
+        assert np.array_equal(orc.ladder_update(betas, None, 3, 1000, 1, mode=1, smd=d), betas)
+    # pair (0, 1) closer than pair (1, 2): its spacing widens, beta_1 falls
+    nb = orc.ladder_update(betas, None, 0, 1000, 1, mode=1, smd=[0.1, 0.3, 0.3])
+    assert nb[1] < 0.5 and nb[0] == 1.0 and nb[-1] == 0.0
+    # the balance term is (d_1 - d_0) / (d_1 + d_0) = 0.5 at kappa = 1
+    assert np.isclose(1.0 / nb[1], 1.0 + np.exp(0.5), rtol=1e-15)
+    assert np.allclose(orc.ladder_update(betas, None, 0, 1000, 1, mode=-1, smd=[0.1, 0.3, 0.3]), betas)
+
+
+def test_ladder_mode_specific_heat_fixed_point():
+    # A Gaussian likelihood in d dimensions has sd(lnL) = sqrt(d / 2) / beta at
+    # every temperature; then every pair of a geometric ladder has the same
+    # thermodynamic length and the ladder stays.
+    geo = np.geomspace(1.0, 1e-4, 6)
+    sig = np.sqrt(1.5) / geo
+    nb = orc.ladder_update(geo, None, 0, 1000, 1, mode=2, spread=sig)
+    assert np.allclose(nb, geo, rtol=1e-13, atol=0)
+    # A perturbed ladder moves back to the geometric one under the same law
+    # (a flipped balance sign would run away from it).
+    b = np.array([1.0, 0.9, 0.5, 0.05, 0.002, 1e-4])
+    for n in range(400):
+        b = orc.ladder_update(b, None, n, 50, 1, mode=2, spread=np.sqrt(1.5) / b)
+    assert np.allclose(np.log(b), np.log(geo), rtol=0, atol=1e-3)
+    # known answer at T = 3: lengths 0.5 * (1 + 2) / 2 = 0.75 and
+    # 0.4 * (2 + 10) / 2 = 2.4, balance (2.4 - 0.75) / 3.15, kappa = 1
+    nb = orc.ladder_update([1.0, 0.5, 0.1], None, 0, 1000, 1, mode=2, spread=[1.0, 2.0, 10.0])
+    assert np.isclose(1.0 / nb[1], 1.0 + np.exp(1.65 / 3.15), rtol=1e-15)
+
+
+def test_lnl_spread():
+    rng = np.random.default_rng(0)
+    L = rng.normal(-100, 7, size=(4, 33))
+    L[1, 5] = -np.inf
+    L[2, :] = -np.inf
+    L[2, 3] = -5.0
+    got = orc.lnl_spread(L)
+    assert np.isclose(got[0], np.std(L[0], ddof=1), rtol=1e-14)
+    assert np.isclose(got[1], np.std(np.delete(L[1], 5), ddof=1), rtol=1e-14)
+    assert got[2] == 0.0
+    assert np.isclose(got[3], np.std(L[3], ddof=1), rtol=1e-14)
