@@ -108,3 +108,27 @@ def test_run_with_gradient_moves():
     sim.run()
     assert [type(m).__name__ for m, _ in sim.sampler.moves] == ["StretchMove", "GradientMove"]
     assert sim.chain.shape[1] == 15 * 32 and np.isfinite(sim.like_max)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_adaptation_then_frozen_ladder(mode):
+    """run_config['adaptation_batches'/'adaptation_nsweeps'] selects the
+    freeze script (emp.py:690-704, endit_freeze1.scr:1-16): the ladder adapts
+    during the burn-in sweeps, select_adjustment('00') freezes it, and the
+    remaining sweeps keep it."""
+    sim = _sim()
+    sim.engine_config["setup"] = np.array([4, 32, 60, 1])
+    sim.engine_config["adapt_mode"] = mode
+    sim.engine_config["adapt_tau"] = 20
+    sim.run_config["adaptation_batches"] = 2
+    sim.run_config["adaptation_nsweeps"] = 100
+    sim._autorun_add_RV_ins()
+    sim.add_keplerian()
+    sim.run()
+    d = sim.reddemcee_discard
+    bh = sim.sampler.get_betas()                        # ladder after each sweep
+    assert d == 30 and bh.shape == (60, 4)
+    assert np.max(np.abs(np.diff(bh[:d], axis=0))) > 0  # adapting
+    assert np.all(bh[d:] == bh[d - 1])                  # frozen
+    assert sim.sampler.adapt_mode == -1
+    assert sim.chain.shape == (4, 30 * 32, sim.model.ndim__) and np.isfinite(sim.like_max)
