@@ -68,3 +68,83 @@ class SavedRun:
     @property
     def acceptance_fraction(self):
         return self._a["accepted"] / max(self.iteration, 1)
+
+    @property
+    def backend(self):
+        """Per-temperature views as reddemcee's backend is read (:class:`RunView`)."""
+        return RunView(self)
+
+
+class TemperatureView:
+    """One temperature of a run's history, read as reddemcee's per-temperature
+    backend ``sampler.backend[t]`` is when EMPEROR copies it to disk
+    (emp/emp.py:751-761): ``iteration``, ``get_chain()`` [rows, W, D],
+    ``get_log_like()`` / ``get_log_prob()`` [rows, W], ``get_betas()`` [rows]
+    (the ladder of each row's sweep) and ``accepted`` [W].  Rows are stretch
+    steps (sweeps x nsteps)."""
+
+    def __init__(self, run, t):
+        self._run, self._t = run, int(t)
+
+    @property
+    def iteration(self):
+        return self._run.iteration
+
+    def get_chain(self, discard=0, thin=1, flat=False):
+        out = self._run.get_chain(discard, thin)[self._t]
+        return out.reshape(-1, out.shape[-1]) if flat else out
+
+    def get_log_like(self, discard=0, thin=1, flat=False):
+        out = self._run.get_log_like(discard, thin)[self._t]
+        return out.ravel() if flat else out
+
+    def get_log_prob(self, discard=0, thin=1, flat=False):
+        out = self._run.get_log_prob(discard, thin)[self._t]
+        return out.ravel() if flat else out
+
+    def get_betas(self, discard=0, thin=1):
+        per_sweep = self._run.get_betas()[:, self._t]
+        rows = np.repeat(per_sweep, max(self._run.iteration // max(len(per_sweep), 1), 1))
+        return rows[int(discard)::int(thin)]
+
+    @property
+    def accepted(self):
+        return np.asarray(self._run.accepted)[self._t]
+
+
+class RunView:
+    """The ensemble-level backend ``sampler.backend`` as EMPEROR reads it
+    (emp/emp.py:734-747): ``iteration``, ``tsw_history_bool`` /
+    ``smd_history_bool`` and ``tsw_history`` / ``smd_history`` [sweeps, T-1];
+    ``backend[t]`` is a :class:`TemperatureView`."""
+
+    def __init__(self, run):
+        self._run = run
+
+    @property
+    def iteration(self):
+        return self._run.iteration
+
+    @property
+    def tsw_history_bool(self):
+        return bool(getattr(self._run, "tsw_history_bool", True))
+
+    @property
+    def smd_history_bool(self):
+        return bool(getattr(self._run, "smd_history_bool", True))
+
+    @property
+    def tsw_history(self):
+        return self._run.get_tsw()
+
+    @property
+    def smd_history(self):
+        return self._run.get_smd()
+
+    def __len__(self):
+        return self._run.ntemps
+
+    def __getitem__(self, t):
+        if not -self._run.ntemps <= int(t) < self._run.ntemps:
+            raise IndexError(f"temperature {t} of {self._run.ntemps}")
+        return TemperatureView(self._run, int(t) % self._run.ntemps)

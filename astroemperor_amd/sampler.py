@@ -96,7 +96,11 @@ class PTSampler:
     itself may be passed as ``log_likelihood_fn``.  ``pool`` is accepted and
     ignored (the device replaces the process pool, support/pools/01.pool).
     ``moves`` takes emcee's forms (see :mod:`astroemperor_amd.moves`); the
-    default is the stretch move with scale ``a``.
+    default is the stretch move with scale ``a``.  ``backend`` must be None
+    (EMPEROR's default, emp/emp.py:570-573): the history stays on the device,
+    is read back through ``sampler.backend`` as reddemcee's is, and
+    ``save()`` writes it to disk; an HDF5 backend object raises
+    NotImplementedError.
     """
 
     def __init__(self, nwalkers, ndim, log_likelihood_fn, log_prior_fn=None, ntemps=None, pool=None,
@@ -113,6 +117,9 @@ class PTSampler:
         if ndim != model.ndim__:
             raise ValueError(f"ndim={ndim} but the model has {model.ndim__} free parameters")
         adapt_mode = ladder_mode(adapt_mode)
+        if backend is not None:
+            raise NotImplementedError("backend must be None: the history is kept on the device "
+                                      "(read it through sampler.backend, write it with save())")
         self.model = model
         self.nwalkers = int(nwalkers)
         self.ndim = int(ndim)
@@ -132,7 +139,6 @@ class PTSampler:
         self.moves = normalise_moves(moves, self.a)
         self.seed = int(seed) if seed is not None else int.from_bytes(os.urandom(8), "little")
         self.record = bool(record)
-        self.backend = backend
         self.D_ = None
         self._pt = None
         self._capacity = 0
@@ -275,9 +281,21 @@ class PTSampler:
         return self._get(5)[..., 0, 0][int(discard):]
 
     @property
+    def accepted(self):
+        """Accepted stretch proposals per temperature and walker [T, W]."""
+        return self._get(6, np.int64)[..., 0, 0]
+
+    @property
     def acceptance_fraction(self):
-        acc = self._get(6, np.int64)[..., 0, 0].astype(np.float64)
-        return acc / max(self.iteration, 1)
+        return self.accepted.astype(np.float64) / max(self.iteration, 1)
+
+    @property
+    def backend(self):
+        """The history as reddemcee's backend is read (emp/emp.py:734-761):
+        ``backend.iteration``, ``backend.tsw_history`` / ``smd_history`` and
+        per temperature ``backend[t].get_chain()`` and the other getters."""
+        from .backend import RunView
+        return RunView(self)
 
     def get_last_sample(self):
         return PTState(self._get(7)[..., 0], self._get(8)[..., 0, 0], self._get(9)[..., 0, 0], self.betas)

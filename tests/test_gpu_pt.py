@@ -307,6 +307,17 @@ def test_saved_run_round_trip(tmp_path):
     assert np.array_equal(r.get_tsw(), s.get_tsw()) and np.array_equal(r.get_smd(), s.get_smd())
     assert np.array_equal(r.acceptance_fraction, s.acceptance_fraction)
     assert np.array_equal(r.betas, s.betas)
+    # reddemcee's backend reads (emp/emp.py:734-761), live and saved
+    for src in (s, r):
+        bk = src.backend
+        assert bk.iteration == 30 and len(bk) == T and bk.tsw_history_bool and bk.smd_history_bool
+        assert np.array_equal(bk.tsw_history, s.get_tsw()) and np.array_equal(bk.smd_history, s.get_smd())
+        for t in range(T):
+            assert np.array_equal(bk[t].get_chain(), s.get_chain()[t])
+            assert np.array_equal(bk[t].get_log_like(), s.get_log_like()[t])
+            assert np.array_equal(bk[t].get_log_prob(), s.get_log_prob()[t])
+            assert np.array_equal(bk[t].get_betas(), s.get_betas()[:, t])
+            assert np.array_equal(bk[t].accepted, s.accepted[t])
 
 
 @pytest.mark.parametrize("gradient", [False, True])
@@ -335,6 +346,10 @@ def test_nsteps_two_records_every_step(gradient):
     assert np.array_equal(chain[:, -1], st.coords)
     assert not np.array_equal(chain[:, 0], chain[:, 1])
     assert np.all(s.acceptance_fraction <= 1.0)
+    # the per-temperature backend counts rows in steps; each row carries its sweep's ladder
+    bk = s.backend[1]
+    assert bk.iteration == 2 * ns and bk.get_chain().shape == (2 * ns, W, m.ndim__)
+    assert np.array_equal(bk.get_betas(), np.repeat(s.get_betas()[:, 1], 2))
     # a different nsteps for the same sampler is refused
     with pytest.raises(ValueError):
         s.run_mcmc(None, nsweeps=1, nsteps=1)

@@ -135,3 +135,40 @@ def test_descriptor_activity_columns():
     m0 = make_sim("GJ876", 1).model
     desc, _ = m0.descriptor()
     assert desc.off_sai == -1 and desc.n_sai == 0 and not desc.sai
+
+
+def test_backend_views_of_a_saved_run(tmp_path):
+    """sampler.backend / backend[t] as EMPEROR reads reddemcee's backend
+    (emp/emp.py:734-761), over a saved run: rows in steps (2 per sweep here),
+    each row with its sweep's ladder."""
+    from astroemperor_amd.backend import FORMAT, SavedRun
+    rng = np.random.default_rng(0)
+    T, S, R, W, D = 3, 4, 8, 6, 2
+    arrays = dict(chain=rng.normal(size=(T, R, W, D)), log_like=rng.normal(size=(T, R, W)),
+                  log_prob=rng.normal(size=(T, R, W)), beta_history=rng.uniform(size=(T, S)),
+                  accepted=rng.integers(0, 8, size=(T, W)), tsw_history=rng.uniform(size=(S, T - 1)),
+                  smd_history=rng.uniform(size=(S, T - 1)), betas=np.array([1.0, 0.5, 0.0]))
+    np.savez(tmp_path / "r.npz", format=np.array(FORMAT), iteration=np.array(R), **arrays)
+    bk = SavedRun(tmp_path / "r.npz").backend
+    assert bk.iteration == R and len(bk) == T
+    assert np.array_equal(bk.tsw_history, arrays["tsw_history"])
+    assert np.array_equal(bk.smd_history, arrays["smd_history"])
+    for t in (0, 2, -1):
+        tt = t % T
+        v = bk[t]
+        assert np.array_equal(v.get_chain(), arrays["chain"][tt])
+        assert np.array_equal(v.get_chain(discard=2, thin=3, flat=True), arrays["chain"][tt, 2::3].reshape(-1, D))
+        assert np.array_equal(v.get_log_like(flat=True), arrays["log_like"][tt].ravel())
+        assert np.array_equal(v.get_log_prob(discard=1), arrays["log_prob"][tt, 1:])
+        assert np.array_equal(v.get_betas(), np.repeat(arrays["beta_history"][tt], 2))
+        assert np.array_equal(v.accepted, arrays["accepted"][tt])
+    with pytest.raises(IndexError):
+        bk[T]
+
+
+def test_sampler_refuses_a_backend_object():
+    from astroemperor_amd import PTSampler
+    from tests.conftest import make_sim
+    m = make_sim("51Peg", 1, parameterisation=1).model
+    with pytest.raises(NotImplementedError):
+        PTSampler(16, m.ndim__, m, ntemps=2, backend=object())
