@@ -1,12 +1,13 @@
 """Chain storage: the run's history in one ``.npz`` file.
 
 The reference saves reddemcee's PTHDFBackend (emp/emp.py:722-762): per
-temperature ``chain`` [S, W, D], ``log_like`` and ``log_prob`` [S, W],
-``beta_history`` [S] and ``accepted`` [W], and ``tsw_history`` /
-``smd_history`` [S, T-1] with the ``iteration`` count.  h5py is not part of
-this stack, so the same arrays (temperature-major) go into a NumPy archive;
-:class:`SavedRun` reads it back with the getters' names and their
-``discard`` / ``thin`` / ``flat`` arguments (emp/emp.py:1401-1409).
+temperature ``chain`` [R, W, D], ``log_like`` and ``log_prob`` [R, W],
+``beta_history`` [R] and ``accepted`` [W] (R rows = sweeps x nsteps), and
+``tsw_history`` / ``smd_history`` [sweeps, T-1] with the ``iteration``
+count.  h5py is not part of this stack, so the same arrays
+(temperature-major) go into a NumPy archive; :class:`SavedRun` reads it back
+with the getters' names and their ``discard`` / ``thin`` / ``flat`` arguments
+(emp/emp.py:1401-1409).
 """
 from __future__ import annotations
 
@@ -34,7 +35,8 @@ class SavedRun:
             self._a = {k: f[k] for k in f.files}
         self.iteration = int(self._a["iteration"])
         self.betas = self._a["betas"]
-        self.ntemps, self.nsweeps, self.nwalkers, self.ndim = self._a["chain"].shape
+        self.ntemps, self.nrows, self.nwalkers, self.ndim = self._a["chain"].shape
+        self.nsweeps = len(self._a["tsw_history"])
 
     def _sel(self, arr, discard, thin, flat):
         arr = arr[:, int(discard)::int(thin)]
@@ -103,9 +105,7 @@ class TemperatureView:
         return out.ravel() if flat else out
 
     def get_betas(self, discard=0, thin=1):
-        per_sweep = self._run.get_betas()[:, self._t]
-        rows = np.repeat(per_sweep, max(self._run.iteration // max(len(per_sweep), 1), 1))
-        return rows[int(discard)::int(thin)]
+        return self._run.get_betas(discard, thin)[:, self._t]
 
     @property
     def accepted(self):

@@ -262,13 +262,18 @@ class PTSampler:
         ladder of its sweep."""
         ll = self._get(1)[..., 0]
         lp = self._get(2)[..., 0]
-        b = np.repeat(self._get(3)[..., 0, 0], max(self._nsteps or 1, 1), axis=0)[:ll.shape[0], :, None]
+        b = self.get_betas()[:ll.shape[0], :, None]
         with np.errstate(invalid="ignore"):
             tempered = np.where(b == 0, 0.0, b * ll)
         return self._sel(tempered + lp, discard, thin, flat)
 
     def get_betas(self, discard=0, thin=1):
-        return self._get(3)[..., 0, 0][int(discard)::int(thin)]
+        """The ladder of every history row [rows, T]: each sweep's ladder
+        (after its update) repeated for its nsteps rows, so that ``discard``
+        counts steps as EMPEROR passes it (emp/emp.py:962, while get_tsw and
+        get_smd take discard // nsteps, 1986-1989)."""
+        per_sweep = self._get(3)[..., 0, 0]
+        return np.repeat(per_sweep, max(self._nsteps or 1, 1), axis=0)[int(discard)::int(thin)]
 
     @property
     def betas(self):

@@ -346,10 +346,15 @@ def test_nsteps_two_records_every_step(gradient):
     assert np.array_equal(chain[:, -1], st.coords)
     assert not np.array_equal(chain[:, 0], chain[:, 1])
     assert np.all(s.acceptance_fraction <= 1.0)
-    # the per-temperature backend counts rows in steps; each row carries its sweep's ladder
+    # rows count steps, in the ladder history and the per-temperature backend
+    # too (EMPEROR discards ladder rows in steps, emp.py:962); the two rows of
+    # a sweep carry its ladder
+    bh = s.get_betas()
+    assert bh.shape == (2 * ns, T) and np.array_equal(bh[0::2], bh[1::2])
+    assert s.get_tsw().shape == (ns, T - 1)
     bk = s.backend[1]
     assert bk.iteration == 2 * ns and bk.get_chain().shape == (2 * ns, W, m.ndim__)
-    assert np.array_equal(bk.get_betas(), np.repeat(s.get_betas()[:, 1], 2))
+    assert np.array_equal(bk.get_betas(), bh[:, 1])
     # a different nsteps for the same sampler is refused
     with pytest.raises(ValueError):
         s.run_mcmc(None, nsweeps=1, nsteps=1)

@@ -140,16 +140,18 @@ def test_descriptor_activity_columns():
 def test_backend_views_of_a_saved_run(tmp_path):
     """sampler.backend / backend[t] as EMPEROR reads reddemcee's backend
     (emp/emp.py:734-761), over a saved run: rows in steps (2 per sweep here),
-    each row with its sweep's ladder."""
+    the ladder history one row per step."""
     from astroemperor_amd.backend import FORMAT, SavedRun
     rng = np.random.default_rng(0)
     T, S, R, W, D = 3, 4, 8, 6, 2
     arrays = dict(chain=rng.normal(size=(T, R, W, D)), log_like=rng.normal(size=(T, R, W)),
-                  log_prob=rng.normal(size=(T, R, W)), beta_history=rng.uniform(size=(T, S)),
+                  log_prob=rng.normal(size=(T, R, W)), beta_history=np.repeat(rng.uniform(size=(T, S)), 2, axis=1),
                   accepted=rng.integers(0, 8, size=(T, W)), tsw_history=rng.uniform(size=(S, T - 1)),
                   smd_history=rng.uniform(size=(S, T - 1)), betas=np.array([1.0, 0.5, 0.0]))
     np.savez(tmp_path / "r.npz", format=np.array(FORMAT), iteration=np.array(R), **arrays)
-    bk = SavedRun(tmp_path / "r.npz").backend
+    run = SavedRun(tmp_path / "r.npz")
+    assert (run.nrows, run.nsweeps) == (R, S)
+    bk = run.backend
     assert bk.iteration == R and len(bk) == T
     assert np.array_equal(bk.tsw_history, arrays["tsw_history"])
     assert np.array_equal(bk.smd_history, arrays["smd_history"])
@@ -160,7 +162,8 @@ def test_backend_views_of_a_saved_run(tmp_path):
         assert np.array_equal(v.get_chain(discard=2, thin=3, flat=True), arrays["chain"][tt, 2::3].reshape(-1, D))
         assert np.array_equal(v.get_log_like(flat=True), arrays["log_like"][tt].ravel())
         assert np.array_equal(v.get_log_prob(discard=1), arrays["log_prob"][tt, 1:])
-        assert np.array_equal(v.get_betas(), np.repeat(arrays["beta_history"][tt], 2))
+        assert np.array_equal(v.get_betas(), arrays["beta_history"][tt])
+        assert np.array_equal(v.get_betas(discard=4), arrays["beta_history"][tt, 4:])
         assert np.array_equal(v.accepted, arrays["accepted"][tt])
     with pytest.raises(IndexError):
         bk[T]
