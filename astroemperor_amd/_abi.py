@@ -18,7 +18,8 @@ PRIOR_KIND = {"Uniform": 0, "Normal": 1, "Jeffreys": 2, "Beta": 3, "Fixed": 4}
 EMP_PRIOR_NONE = -1
 SEM = {"reference": 0, "intended": 1}
 PT_WHAT = {"chain": 0, "lnl": 1, "lnp": 2, "betas": 3, "tsw": 4, "smd": 5, "accepted": 6,
-           "x": 7, "state_lnl": 8, "state_lnp": 9, "state_betas": 10, "nsweeps": 11}
+           "x": 7, "state_lnl": 8, "state_lnp": 9, "state_betas": 10, "nsweeps": 11,
+           "grad_steps": 12}
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "EUNSUP", -5: "EIO", -6: "EFAULT_INJECTED"}
 
 
@@ -96,11 +97,13 @@ _SIGS = {
     "emp_pt_set_state": (C.c_int, [_P, _P, _P]),
     "emp_pt_set_adaptation": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
     "emp_pt_set_moves": (C.c_int, [_P, _P, C.c_int32]),
+    "emp_pt_set_gradient_steps": (C.c_int, [_P, _P]),
     "emp_pt_lnl_moments": (C.c_int, [_P, C.c_int64, _P, _P, _P]),
     "emp_pt_checkpoint": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_restore": (C.c_int, [_P, C.c_char_p]),
     "emp_pt_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "emp_pt_time_halfstep": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "emp_pt_time_gradstep": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     "emp_pt_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "emp_systems_create": (C.c_int, [_P, C.POINTER(EmpSystemsDesc), _P, _P, C.POINTER(_P)]),
     "emp_systems_destroy": (None, [_P]),

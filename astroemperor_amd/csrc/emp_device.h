@@ -112,6 +112,7 @@ struct PlanetC {
     double mk_d0;    // 3 (1 - e)
     double K;        // amplitude (for gradients)
     double cw, sw;   // cos w, sin w (for gradients)
+    double bq, ibq;  // sqrt(1 - e^2) and its reciprocal (for gradients)
     int tp_mode;
     int bad;         // NaN-producing parameter set (reference would give NaN)
     int ktab;        // mixed path: this planet's per-vector Kepler table is built and used
@@ -169,6 +170,8 @@ __device__ inline PlanetC planet_setup(const double* th, int param, int use_c) {
     pc.K = A;
     pc.cw = cw;
     pc.sw = sw;
+    pc.bq = q;
+    pc.ibq = 1.0 / q;
     if (pc.bad) {   // the reference evaluates to NaN here; make that explicit
         pc.c1 = __builtin_nan("");
         pc.c2 = __builtin_nan("");

@@ -39,97 +39,94 @@ using namespace emp;
 
 struct GradLDS {
     double g[MAXD];            // gradient w.r.t. the full vector
-    double nat[KMAX][5];       // per planet: (M*tau, M, e, w, A) accumulators
-    double cosg[CMAX][3];      // per cosine term: (A, phase, frequency) accumulators
+    double nat[KMAX][5];       // per planet: (M*tau, M, e, w, A) sums
+    double cosg[CMAX][3];      // per cosine term: (A, phase, frequency) sums
 };
 
-// Per-wave partial sums, added over a slot's waves in wave order so that the
-// gradient does not depend on how the waves were scheduled.
-// MA coefficients and timescales: one entry per (set, order), order-minor.
-constexpr int QM = 4;
-constexpr int WA_OFF = 0, WA_JIT = IMAX, WA_MAC = 2 * IMAX, WA_MAT = WA_MAC + IMAX * QM, WA_COS = WA_MAT + IMAX * QM;
-constexpr int WA_SAI = WA_COS + 3 * CMAX, WA_N = WA_SAI + SAIMAX;
+// Per-wave partial sums, added over a slot's waves in wave order at the end,
+// so that the gradient does not depend on how the waves were scheduled.
+// acc: offsets, jitters, MA coefficients and timescales (one entry per
+// (set, order), order-minor; orders above 1 only in the EXT = 2 instances),
+// then, with the terms added after the MA, the cosine terms and the activity
+// coefficients.  nat: per planet the five natural-parameter sums, then the
+// acceleration coefficients.
+template <int EXT>
+struct GradWave {
+    static constexpr int QM = EXT == 2 ? 4 : 1;
+    static constexpr int OFF = 0, JIT = IMAX, MAC = 2 * IMAX, MAT = MAC + IMAX * QM, COS = MAT + IMAX * QM;
+    static constexpr int SAI = COS + (EXT >= 1 ? 3 * CMAX : 0);
+    static constexpr int N = SAI + (EXT >= 1 ? SAIMAX : 0);
+};
 constexpr int WN_ACC = KMAX * 5, WN_N = KMAX * 5 + AMAX;
+template <int EXT>
 struct GradWaveLDS {
-    double acc[NWAVES][WA_N];   // offsets, jitters, MA coefficients, MA timescales (per instrument / set),
-                                // cosine terms, activity coefficients
-    double nat[NWAVES][WN_N];   // natural-parameter sums per planet, then the acceleration terms
+    double acc[NWAVES][GradWave<EXT>::N];
+    double nat[NWAVES][WN_N];
 };
 
-// Model value of one epoch and its planet derivatives (natural parameters).
-struct EpochDeriv {
-    double mu;
-    double dM[KMAX], de[KMAX], dw[KMAX], dA[KMAX], tau[KMAX];
-};
-
-// PREC 0: FP64 Kepler solve; PREC 1: the mixed path's (FP32 start, one FP64
-// step; the per-vector Kepler tables must be in sl.ktab).
+// w times the derivatives of one planet's term at time t with respect to its
+// natural parameters, added to a = (dmu/dM tau, dmu/dM, dmu/de, dmu/dw,
+// dmu/dA).  With D = 1 - e cos E, b = sqrt(1 - e^2) (per planet, pc.bq):
+// D cos f = cos E - e, D sin f = b sin E.
 template <int PREC>
-__device__ __forceinline__ void epoch_deriv(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
-                                            int k, EpochDeriv& d) {
-    d.mu = sl.gam[k];
-    const double x = t - md.t0;
-    if (md.acc) {
-        double v = 0.0;
-        for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
-        d.mu += v * x;
-    }
-#pragma unroll
-    for (int p = 0; p < KMAX; ++p) {
-        d.dM[p] = d.de[p] = d.dw[p] = d.dA[p] = d.tau[p] = 0.0;
-        if (p < md.K) {
-            const PlanetC& pc = sl.planet[p];
-            double sE, cE;
-            planet_sc<PREC>(pc, tab, sl.ktab[p], t, sE, cE);
-            const double e = pc.e;
-            const double rD = rcp_path<PREC>(__builtin_fma(-e, cE, 1.0));
-            const double bq = sqrt((1.0 - e) * (1.0 + e));
-            const double cosf = (cE - e) * rD, sinf = bq * sE * rD;
-            const double cfw = cosf * pc.cw - sinf * pc.sw;
-            const double sfw = sinf * pc.cw + cosf * pc.sw;
-            const double A = pc.K;
-            d.mu += A * (cfw + e * pc.cw);
-            d.dM[p] = -A * sfw * bq * rD * rD;
-            d.de[p] = -A * sfw * (bq * sE * rD * rD + sE * rD / bq) + A * pc.cw;
-            d.dw[p] = -A * (sfw + e * pc.sw);
-            d.dA[p] = cfw + e * pc.cw;
-            d.tau[p] = pc.tp_mode ? (t - pc.tp) : t;
-        }
-    }
+__device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab& tab, const float2* kt, double t,
+                                             double w, double (&a)[8]) {
+    double sE, cE;
+    planet_sc<PREC>(pc, tab, kt, t, sE, cE);
+    const double e = pc.e;
+    const double rD = rcp_path<PREC>(__builtin_fma(-e, cE, 1.0));
+    const double u = cE - e, v = pc.bq * sE;
+    const double sfw = __builtin_fma(v, pc.cw, u * pc.sw) * rD;     // sin(f + w)
+    const double cfw = __builtin_fma(u, pc.cw, -(v * pc.sw)) * rD;  // cos(f + w)
+    const double wA = w * pc.K;
+    const double ws = wA * sfw;
+    const double dM = -(ws * pc.bq) * (rD * rD);
+    const double tau = pc.tp_mode ? t - pc.tp : t;
+    a[0] = __builtin_fma(dM, tau, a[0]);
+    a[1] += dM;
+    a[2] = __builtin_fma(-ws, sE * rD * __builtin_fma(pc.bq, rD, pc.ibq), __builtin_fma(wA, pc.cw, a[2]));
+    a[3] = __builtin_fma(-wA, __builtin_fma(e, pc.sw, sfw), a[3]);
+    a[4] = __builtin_fma(w, __builtin_fma(e, pc.cw, cfw), a[4]);
 }
 
-// Accumulates w = dlnL/dmu_i times the epoch's derivatives.
-__device__ __forceinline__ void accumulate(const ModelDev& md, const EpochDeriv& d, double w, double x,
-                                           double (&acc_nat)[KMAX][5], double (&acc_acc)[AMAX]) {
+// Sum of eight per-lane values over the wave, scattered: halves of the
+// values are exchanged across lanes 32 apart, then 16, then 8 (each lane
+// keeps the half its lane bit selects and adds its partner's), and the
+// remaining value is summed over each aligned group of eight lanes.  Lane
+// 8 i then holds the total of value i; the first n are added to dst[i].
+// 10 exchanges instead of 48 for eight separate wave sums.  Whole wave active.
+__device__ __forceinline__ void wave_reduce_add8(double (&v)[8], int n, double* dst, int lane) {
+    double a[4], b[2];
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
 #pragma unroll
-    for (int p = 0; p < KMAX; ++p) {
-        acc_nat[p][0] = __builtin_fma(w * d.dM[p], d.tau[p], acc_nat[p][0]);
-        acc_nat[p][1] = __builtin_fma(w, d.dM[p], acc_nat[p][1]);
-        acc_nat[p][2] = __builtin_fma(w, d.de[p], acc_nat[p][2]);
-        acc_nat[p][3] = __builtin_fma(w, d.dw[p], acc_nat[p][3]);
-        acc_nat[p][4] = __builtin_fma(w, d.dA[p], acc_nat[p][4]);
+    for (int i = 0; i < 4; ++i) {
+        const double keep = h5 ? v[i + 4] : v[i], send = h5 ? v[i] : v[i + 4];
+        a[i] = keep + __shfl_xor(send, 32, 64);
     }
-    if (md.acc) {
-        double xp = x;
 #pragma unroll
-        for (int a = AMAX - 1; a >= 0; --a) {
-            if (a < md.acc) {
-                acc_acc[a] = __builtin_fma(w, xp, acc_acc[a]);
-                xp *= x;
-            }
-        }
+    for (int i = 0; i < 2; ++i) {
+        const double keep = h4 ? a[i + 2] : a[i], send = h4 ? a[i] : a[i + 2];
+        b[i] = keep + __shfl_xor(send, 16, 64);
     }
+    double c = (h3 ? b[1] : b[0]) + dpp_f64<0x128, 0xf>(0.0, h3 ? b[0] : b[1]);   // row_ror 8: lane ^ 8
+    c += dpp_f64<0x141, 0xf>(0.0, c);   // row_half_mirror: lane i with 7 - i of its group of 8
+    c += dpp_f64<0xB1, 0xf>(0.0, c);    // quad_perm [1, 0, 3, 2]
+    c += dpp_f64<0x4E, 0xf>(0.0, c);    // quad_perm [2, 3, 0, 1]
+    const int i = lane >> 3;
+    if ((lane & 7) == 0 && i < n) dst[i] += c;
 }
 
 // Adds w = dlnL/dS_n times the derivatives of the terms added after the MA
 // at epoch n (time t, instrument k).
+template <int EXT>
 __device__ __forceinline__ void post_accumulate(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
                                                 int n, int k, double w, double* wacc) {
+    using GW = GradWave<EXT>;
     if (md.sai_w) {
         const int b0 = md.sai_base[k], nc = md.sai_base[k + 1] - b0;
-        for (int c = 0; c < nc; ++c) atomicAdd(&wacc[WA_SAI + b0 + c], w * md.sai[(size_t)c * md.N + n]);
+        for (int c = 0; c < nc; ++c) atomicAdd(&wacc[GW::SAI + b0 + c], w * md.sai[(size_t)c * md.N + n]);
     }
-    double* acc = wacc + WA_COS;
+    double* acc = wacc + GW::COS;
     for (int c = 0; c < md.n_cos; ++c) {
         const double r = reduce_2pi(cos_phase(sl, c, t));
         double sn, cs;
@@ -276,77 +273,284 @@ __device__ __forceinline__ double lam_chunk_q(const ModelDev& md, const SlotLDS&
     return st[0];
 }
 
+// Per-lane sums of the per-instrument gradient terms (offset, jitter, MA(1)
+// coefficient and timescale) of the lane's epochs, kept in registers while
+// the lane's instrument stays the same and added to the wave's LDS sums when
+// it changes (epochs grouped by instrument, as the per-instrument MA orders
+// them, change instrument a few times per lane in a whole pass), instead of
+// four same-address LDS atomics per epoch.
+struct InsAcc {
+    int k = -1;
+    double off = 0.0, jit = 0.0, mac = 0.0, mat = 0.0;
+    template <int EXT>
+    __device__ __forceinline__ void flush(double* wacc) {
+        using GW = GradWave<EXT>;
+        if (k < 0) return;
+        atomicAdd(&wacc[GW::OFF + k], off);
+        atomicAdd(&wacc[GW::JIT + k], jit);
+        atomicAdd(&wacc[GW::MAC + k * GW::QM], mac);
+        atomicAdd(&wacc[GW::MAT + k * GW::QM], mat);
+        off = jit = mac = mat = 0.0;
+    }
+    template <int EXT>
+    __device__ __forceinline__ void to(int kk, double* wacc) {
+        if (kk != k) {
+            flush<EXT>(wacc);
+            k = kk;
+        }
+    }
+};
+
+// The MA adjoint at one chunk of a tile taken in reverse: lambda of the
+// lane's epoch from eps (the forward pass's scratch row), with the carry from
+// the later epochs; returns w = dlnL/dmu = -lambda (0 for an invalid lane)
+// after adding the epoch's offset, jitter, post-MA and MA-parameter terms.
+template <int PREC, int WPT, int EXT>
+__device__ __forceinline__ double ma_chunk_w(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, ScanLDS& sc,
+                                             const double* eps, double* wacc, InsAcc& ia, bool valid, int j, int tb,
+                                             int tn, int lane, int wave, int wsub, int& parity, double& lam_carry,
+                                             double (&lq_carry)[4]) {
+    using GW = GradWave<EXT>;
+    constexpr bool POST = EXT >= 1;
+    const bool maq = EXT == 2 && md.ma_q > 1;
+    double u = 0.0, amul = 1.0, ie = 0.0, t = 0.0, ef = 0.0;
+    int k = 0;
+    if (valid) {
+        t = tl.t[j];
+        k = tl.ins[j];
+        const double err2 = tl.s2[j] + sl.jit2[k];
+        ie = rcp_path<PREC>(err2);
+        const double e_i = eps[tb + j];
+        ef = (POST && has_post(md)) ? e_i - epoch_post(md, sl, tl.tab, t, tb + j, k) : e_i;
+        u = -ef * ie;
+        const int n1 = tb + j + 1;     // the next epoch's multiplier
+        amul = 0.0;
+        if (!maq && n1 < md.N) {
+            const bool in_tile = j + 1 < tn;
+            const int k1 = in_tile ? tl.ins[j + 1] : md.ins[n1];
+            const double c1 = in_tile ? tl.dt[j + 1] : md.dt[n1];
+            amul = ma_mult(md, sl, k1, c1);
+        }
+    }
+    double sa = amul, sb = u;
+    double lam = 0.0;
+    if (maq) {
+        lam = md.ma_q == 2
+                  ? lam_chunk_q<2, WPT>(md, sl, tl, valid, j, tb, tn, t, k, u, lane, wave, wsub, parity, lq_carry)
+                  : lam_chunk_q<4, WPT>(md, sl, tl, valid, j, tb, tn, t, k, u, lane, wave, wsub, parity, lq_carry);
+    } else if (WPT == 1) {
+        wave_affine_suffix_scan(sa, sb, lane);
+        lam = __builtin_fma(sa, lam_carry, sb);
+        lam_carry = __shfl(lam, 0, 64);
+    } else {
+        wave_affine_suffix_scan(sa, sb, lane);
+        if (lane == 0) { sc.tot[parity][wave][0] = sa; sc.tot[parity][wave][1] = sb; }
+        __syncthreads();
+        const int w0 = wave - wsub;
+        double pa = 1.0, pb = 0.0;
+        for (int w = WPT - 1; w >= 0; --w) {
+            if (w == wsub) lam = __builtin_fma(sa, __builtin_fma(pa, lam_carry, pb), sb);
+            const double ta = sc.tot[parity][w0 + w][0], tbv = sc.tot[parity][w0 + w][1];
+            pb = __builtin_fma(ta, pb, tbv);
+            pa = ta * pa;
+        }
+        lam_carry = __builtin_fma(pa, lam_carry, pb);
+        parity ^= 1;
+    }
+    if (!valid) return 0.0;
+    const double w = -lam;
+    ia.to<EXT>(k, wacc);
+    ia.off += w;
+    if (md.has_jit) ia.jit = __builtin_fma(sl.jit[k], ef * ef * ie * ie - ie, ia.jit);
+    if (POST && has_post(md)) post_accumulate<EXT>(md, sl, tl.tab, t, tb + j, k, -u, wacc);
+    if (maq) {
+        // d a_{n,c} / d phi_c = -exp(-dt / tau_c), d a_{n,c} / d tau_c = a_{n,c} dt / tau_c^2
+        const int n = tb + j, set = md.ma_global ? 0 : k;
+        const double* ph = sl.theta + md.off_ma + 2 * md.ma_q * set;
+        for (int c = 0; c < md.ma_q; ++c) {
+            const int n1 = n - 1 - c, j1 = j - 1 - c;
+            if (n1 < 0) break;
+            const int k1 = j1 >= 0 ? tl.ins[j1] : md.ins[n1];
+            if (!(md.ma_global || k1 == k)) break;
+            const double dt = __builtin_fabs(t - (j1 >= 0 ? tl.t[j1] : md.t[n1]));
+            const double tau = ph[2 * c + 1];
+            const double wgt = exp(-dt / tau);
+            const double le = lam * eps[n1];
+            atomicAdd(&wacc[GW::MAC + set * GW::QM + c], le * -wgt);
+            atomicAdd(&wacc[GW::MAT + set * GW::QM + c], le * -(ph[2 * c] * wgt * dt / (tau * tau)));
+        }
+    } else {
+        // da_i/dc = a_i / c, da_i/dtau = a_i dt / tau^2 (a_i = 0 at a segment start)
+        const double col = tl.dt[j];
+        const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
+        if (!start) {
+            const int n = tb + j;
+            const double le = lam * eps[n - 1];
+            const double wgt = md.ma_wfixed ? col : exp(-col / sl.ma_tau[k]);
+            if (md.ma_global) {
+                atomicAdd(&wacc[GW::MAC], le * -wgt);
+                if (!md.ma_wfixed) {
+                    const double tau = sl.ma_tau[k];
+                    atomicAdd(&wacc[GW::MAT], le * -(sl.ma_c[k] * wgt * col / (tau * tau)));
+                }
+            } else {
+                ia.mac = __builtin_fma(le, -wgt, ia.mac);
+                if (!md.ma_wfixed) {
+                    const double tau = sl.ma_tau[k];
+                    ia.mat = __builtin_fma(le, -(sl.ma_c[k] * wgt * col / (tau * tau)), ia.mat);
+                }
+            }
+        }
+    }
+    return w;
+}
+
+// Without the MA: the lane's epoch's residual, its share of chi^2 and of the
+// log-determinant, w = dlnL/dmu = r / err2 and the epoch's offset, jitter and
+// post-MA terms.
+template <int PREC, int EXT>
+__device__ __forceinline__ double plain_w(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, double* wacc,
+                                          InsAcc& ia, int j, int tb, double& chi, double& prod, int& pexp) {
+    using GW = GradWave<EXT>;
+    constexpr bool POST = EXT >= 1;
+    const double t = tl.t[j];
+    const int k = tl.ins[j];
+    const double mu = epoch_mu<PREC>(md, sl, tl.tab, t, k);
+    const double err2 = tl.s2[j] + sl.jit2[k];
+    const double cs = (POST && has_post(md)) ? epoch_post(md, sl, tl.tab, t, tb + j, k) : 0.0;
+    const double r = tl.y[j] - mu - cs;
+    const double ie = rcp_path<PREC>(err2);
+    const double w = r * ie;
+    chi = __builtin_fma(r * r, ie, chi);
+    prod *= err2;
+    pexp += __builtin_amdgcn_frexp_exp(prod);
+    prod = __builtin_amdgcn_frexp_mant(prod);
+    ia.to<EXT>(k, wacc);
+    ia.off += w;
+    if (md.has_jit) ia.jit = __builtin_fma(sl.jit[k], w * w - ie, ia.jit);
+    if (POST && has_post(md)) post_accumulate<EXT>(md, sl, tl.tab, t, tb + j, k, w, wacc);
+    return w;
+}
+
+// Adjoint kernel.  Per tile (forward without the MA, in reverse with it):
+// stage 1 finds w = dlnL/dmu of each lane's epochs (CPT chunks of 64 WPT
+// epochs, held in registers), stage 2 then takes the planets one at a time,
+// adding w times each planet's derivatives over the lane's epochs and
+// reducing the five sums over the wave once per planet per tile, and the
+// acceleration terms likewise.  No derivative is held across planets, which
+// keeps the register count near the likelihood kernel's.  With the MA a
+// forward pass first computes lnL and stores eps (one scratch row per
+// vector).  `stale` (optional, per vector): vectors whose flag is 0 are
+// skipped; a computed vector's flag is cleared.
 // EXT: 0 plain models, 1 with terms added after the MA, 2 also MA orders
-// above 1 (its order-q scans hold the most registers).
+// above 1.
 template <int PREC, int WPT, int EXT>
 __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                     int64_t ld, double* __restrict__ out_lnl,
                                                     double* __restrict__ out_grad,
-                                                    unsigned long long* nonfinite, double* eps_scratch) {
+                                                    unsigned long long* nonfinite, double* eps_scratch, int* stale) {
     constexpr int SLOTS = NWAVES / WPT;
+    constexpr int G = 64 * WPT;
+    constexpr int CPT = TILE / G;             // chunks of a tile per wave
     constexpr bool POST = EXT >= 1;
+    using GW = GradWave<EXT>;
     __shared__ SlotLDS s_slot[SLOTS];
     __shared__ GradLDS s_grad[SLOTS];
-    __shared__ GradWaveLDS s_gw;
+    __shared__ GradWaveLDS<EXT> s_gw;
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int slot = wave / WPT, wsub = wave % WPT;
-    const int g = wsub * 64 + lane, G = 64 * WPT;
+    const int g = wsub * 64 + lane;
     const int64_t b = (int64_t)blockIdx.x * SLOTS + slot;
-    const bool live = b < B;
+    const bool live = b < B && (!stale || stale[b] != 0);
+    if (!__syncthreads_or(live)) return;       // no vector of this workgroup to compute
     SlotLDS& sl = s_slot[slot];
     GradLDS& gl = s_grad[slot];
     if (live) expand_theta(md, sl, theta + b * ld, g, G);
     for (int j = g; j < MAXD; j += G) gl.g[j] = 0.0;
     for (int j = g; j < KMAX * 5; j += G) (&gl.nat[0][0])[j] = 0.0;
     double* wacc = s_gw.acc[wave];
-    for (int j = lane; j < WA_N; j += 64) wacc[j] = 0.0;
-    for (int j = lane; j < WN_N; j += 64) s_gw.nat[wave][j] = 0.0;
+    double* wnat = s_gw.nat[wave];
+    for (int j = lane; j < GW::N; j += 64) wacc[j] = 0.0;
+    for (int j = lane; j < WN_N; j += 64) wnat[j] = 0.0;
     tab_load(md, s_tile);
     __syncthreads();
     slot_prepare<PREC, WPT>(md, sl, s_tile, live, wsub, lane);
 
-    double acc_nat[KMAX][5];
-#pragma unroll
-    for (int p = 0; p < KMAX; ++p)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) acc_nat[p][q] = 0.0;
-    double acc_acc[AMAX];
-#pragma unroll
-    for (int a = 0; a < AMAX; ++a) acc_acc[a] = 0.0;
+    const bool ma = md.ma_active;
+    double* eps = ma ? eps_scratch + (live ? b : 0) * (int64_t)md.N : nullptr;
     double lnl_v = 0.0;
-
-    if (!md.ma_active) {
-        double chi = 0.0, prod = 1.0;
-        int pexp = 0;
-        for (int tb = 0; tb < md.N; tb += TILE) {
-            const int tn = min(TILE, md.N - tb);
-            __syncthreads();
-            tile_load(md, s_tile, tb, tn);
-            __syncthreads();
-            for (int cb = 0; cb < tn; cb += G) {
-                const int j = cb + g;
-                if (!(live && j < tn)) continue;
-                const double t = s_tile.t[j];
-                const int k = s_tile.ins[j];
-                EpochDeriv d;
-                epoch_deriv<PREC>(md, sl, s_tile.tab, t, k, d);
-                const double err2 = s_tile.s2[j] + sl.jit2[k];
-                const double cs = (POST && has_post(md)) ? epoch_post(md, sl, s_tile.tab, t, tb + j, k) : 0.0;
-                const double r = s_tile.y[j] - d.mu - cs;
-                const double ie = rcp_path<PREC>(err2);
-                const double w = r * ie;
-                chi = __builtin_fma(r * r, ie, chi);
-                prod *= err2;
-                pexp += __builtin_amdgcn_frexp_exp(prod);
-                prod = __builtin_amdgcn_frexp_mant(prod);
-                accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
-                atomicAdd(&wacc[WA_OFF + k], w);
-                if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (w * w - ie));
-                if (POST && has_post(md)) post_accumulate(md, sl, s_tile.tab, t, tb + j, k, w, wacc);
+    if (ma) {
+        // Pass 1: the likelihood, storing eps.
+        lnl_v = block_lnl<PREC, WPT, false, true, true, POST>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr,
+                                                             false, eps);
+        __syncthreads();   // eps rows written by every wave of the slot
+    }
+    double chi = 0.0, prod = 1.0;
+    int pexp = 0;
+    double lam_carry = 0.0;
+    double lq_carry[4] = {0.0, 0.0, 0.0, 0.0};     // MA(q >= 2) adjoint state
+    int parity = 0;
+    InsAcc ia;
+    const int ntile = (md.N + TILE - 1) / TILE;
+    for (int i = 0; i < ntile; ++i) {
+        const int it = ma ? ntile - 1 - i : i;
+        const int tb = it * TILE, tn = min(TILE, md.N - tb);
+        __syncthreads();
+        tile_load(md, s_tile, tb, tn);
+        __syncthreads();
+        double wv[CPT];
+        if (!ma) {
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+                const int j = c * G + g;
+                wv[c] = (live && j < tn) ? plain_w<PREC, EXT>(md, sl, s_tile, wacc, ia, j, tb, chi, prod, pexp) : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int c = CPT - 1; c >= 0; --c) {
+                wv[c] = 0.0;
+                if (c * G >= tn) continue;            // uniform over the workgroup
+                const int j = c * G + g;
+                wv[c] = ma_chunk_w<PREC, WPT, EXT>(md, sl, s_tile, s_scan, eps, wacc, ia, live && j < tn, j, tb, tn,
+                                                   lane, wave, wsub, parity, lam_carry, lq_carry);
             }
         }
+        if (live) {
+            for (int p = 0; p < md.K; ++p) {
+                double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                const PlanetC& pc = sl.planet[p];
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const int j = c * G + g;
+                    if (j < tn) planet_wgrad<PREC>(pc, s_tile.tab, sl.ktab[p], s_tile.t[j], wv[c], a);
+                }
+                wave_reduce_add8(a, 5, wnat + 5 * p, lane);
+            }
+            if (md.acc) {
+                double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int c = 0; c < CPT; ++c) {
+                    const int j = c * G + g;
+                    if (j < tn) {
+                        const double x = s_tile.t[j] - md.t0;
+                        double xp = wv[c] * x;
+#pragma unroll
+                        for (int q = AMAX - 1; q >= 0; --q) {
+                            if (q < md.acc) {
+                                a[q] += xp;
+                                xp *= x;
+                            }
+                        }
+                    }
+                }
+                wave_reduce_add8(a, md.acc, wnat + WN_ACC, lane);
+            }
+        }
+    }
+    ia.flush<EXT>(wacc);
+    if (!ma) {
         double ldet = live ? log(prod) + (double)pexp * LN2 : 0.0;
         chi = wave_sum(chi);
         ldet = wave_sum(ldet);
@@ -356,150 +560,24 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
         const int w0 = wave - wsub;
         for (int w = 0; w < WPT; ++w) { c2 += s_scan.red[w0 + w][0]; l2 += s_scan.red[w0 + w][1]; }
         lnl_v = -0.5 * (c2 + l2) + md.lnl_const;
-    } else {
-        // Pass 1: the likelihood, storing eps.
-        double* eps = eps_scratch + (live ? b : 0) * (int64_t)md.N;
-        lnl_v = block_lnl<PREC, WPT, false, true, true, POST>(md, sl, s_tile, s_scan, live, wsub, nullptr, nullptr,
-                                                             false, eps);
-        __syncthreads();   // eps rows written by every wave of the slot
-        // Pass 2: epochs in reverse, lambda by suffix scans.
-        double lam_carry = 0.0;
-        double lq_carry[4] = {0.0, 0.0, 0.0, 0.0};     // MA(q >= 2) adjoint state
-        const bool maq = EXT == 2 && md.ma_q > 1;
-        int parity = 0;
-        const int ntile = (md.N + TILE - 1) / TILE;
-        for (int it = ntile - 1; it >= 0; --it) {
-            const int tb = it * TILE, tn = min(TILE, md.N - tb);
-            __syncthreads();
-            tile_load(md, s_tile, tb, tn);
-            __syncthreads();
-            for (int cb = ((tn - 1) / G) * G; cb >= 0; cb -= G) {
-                const int j = cb + g;
-                const bool valid = live && j < tn;
-                EpochDeriv d;
-                double u = 0.0, amul = 1.0, e_i = 0.0, ie = 0.0, t = 0.0, ef = 0.0;
-                int k = 0;
-                if (valid) {
-                    t = s_tile.t[j];
-                    k = s_tile.ins[j];
-                    epoch_deriv<PREC>(md, sl, s_tile.tab, t, k, d);
-                    const double err2 = s_tile.s2[j] + sl.jit2[k];
-                    ie = rcp_path<PREC>(err2);
-                    e_i = eps[tb + j];
-                    ef = (POST && has_post(md)) ? e_i - epoch_post(md, sl, s_tile.tab, t, tb + j, k) : e_i;
-                    u = -ef * ie;
-                    const int n1 = tb + j + 1;     // the next epoch's multiplier
-                    amul = 0.0;
-                    if (!maq && n1 < md.N) {
-                        const bool in_tile = j + 1 < tn;
-                        const int k1 = in_tile ? s_tile.ins[j + 1] : md.ins[n1];
-                        const double c1 = in_tile ? s_tile.dt[j + 1] : md.dt[n1];
-                        amul = ma_mult(md, sl, k1, c1);
-                    }
-                }
-                double sa = amul, sb = u;
-                double lam;
-                if (maq) {
-                    lam = md.ma_q == 2
-                              ? lam_chunk_q<2, WPT>(md, sl, s_tile, valid, j, tb, tn, t, k, u, lane, wave, wsub,
-                                                    parity, lq_carry)
-                              : lam_chunk_q<4, WPT>(md, sl, s_tile, valid, j, tb, tn, t, k, u, lane, wave, wsub,
-                                                    parity, lq_carry);
-                } else if (WPT == 1) {
-                    wave_affine_suffix_scan(sa, sb, lane);
-                    lam = __builtin_fma(sa, lam_carry, sb);
-                    lam_carry = __shfl(lam, 0, 64);
-                } else {
-                    wave_affine_suffix_scan(sa, sb, lane);
-                    if (lane == 0) { s_scan.tot[parity][wave][0] = sa; s_scan.tot[parity][wave][1] = sb; }
-                    __syncthreads();
-                    const int w0 = wave - wsub;
-                    double pa = 1.0, pb = 0.0;
-                    for (int w = WPT - 1; w >= 0; --w) {
-                        if (w == wsub) lam = __builtin_fma(sa, __builtin_fma(pa, lam_carry, pb), sb);
-                        const double ta = s_scan.tot[parity][w0 + w][0], tbv = s_scan.tot[parity][w0 + w][1];
-                        pb = __builtin_fma(ta, pb, tbv);
-                        pa = ta * pa;
-                    }
-                    lam_carry = __builtin_fma(pa, lam_carry, pb);
-                    parity ^= 1;
-                }
-                if (valid) {
-                    const double w = -lam;
-                    accumulate(md, d, w, t - md.t0, acc_nat, acc_acc);
-                    atomicAdd(&wacc[WA_OFF + k], w);
-                    if (md.has_jit) atomicAdd(&wacc[WA_JIT + k], sl.jit[k] * (ef * ef * ie * ie - ie));
-                    if (POST && has_post(md)) post_accumulate(md, sl, s_tile.tab, t, tb + j, k, -u, wacc);
-                    if (maq) {
-                        // d a_{n,c} / d phi_c = -exp(-dt / tau_c), d a_{n,c} / d tau_c = a_{n,c} dt / tau_c^2
-                        const int n = tb + j, set = md.ma_global ? 0 : k;
-                        const double* ph = sl.theta + md.off_ma + 2 * md.ma_q * set;
-                        for (int c = 0; c < md.ma_q; ++c) {
-                            const int n1 = n - 1 - c, j1 = j - 1 - c;
-                            if (n1 < 0) break;
-                            const int k1 = j1 >= 0 ? s_tile.ins[j1] : md.ins[n1];
-                            if (!(md.ma_global || k1 == k)) break;
-                            const double dt = __builtin_fabs(t - (j1 >= 0 ? s_tile.t[j1] : md.t[n1]));
-                            const double tau = ph[2 * c + 1];
-                            const double wgt = exp(-dt / tau);
-                            const double le = lam * eps[n1];
-                            atomicAdd(&wacc[WA_MAC + set * QM + c], le * -wgt);
-                            atomicAdd(&wacc[WA_MAT + set * QM + c], le * -(ph[2 * c] * wgt * dt / (tau * tau)));
-                        }
-                    }
-                    // da_i/dc = a_i / c, da_i/dtau = a_i dt / tau^2 (a_i = 0 at a segment start)
-                    const double col = s_tile.dt[j];
-                    const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
-                    if (!maq && !start) {
-                        const int n = tb + j;
-                        const double e_prev = eps[n - 1];
-                        const int set = md.ma_global ? 0 : k;
-                        const double wgt = md.ma_wfixed ? col : exp(-col / sl.ma_tau[k]);
-                        atomicAdd(&wacc[WA_MAC + set * QM], lam * e_prev * -wgt);
-                        if (!md.ma_wfixed) {
-                            const double tau = sl.ma_tau[k];
-                            atomicAdd(&wacc[WA_MAT + set * QM],
-                                      lam * e_prev * -(sl.ma_c[k] * wgt * col / (tau * tau)));
-                        }
-                    }
-                }
-            }
-        }
-    }
-    // Reduce the natural-parameter accumulators over the slot's waves.
-#pragma unroll
-    for (int p = 0; p < KMAX; ++p)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            if (p < md.K) {
-                const double v = wave_sum(acc_nat[p][q]);
-                if (lane == 0) s_gw.nat[wave][p * 5 + q] = v;
-            }
-        }
-#pragma unroll
-    for (int a = 0; a < AMAX; ++a) {
-        if (a < md.acc) {
-            const double v = wave_sum(acc_acc[a]);
-            if (lane == 0) s_gw.nat[wave][WN_ACC + a] = v;
-        }
     }
     __syncthreads();
     if (live && wsub == 0) {
         const int w0 = wave;
-        for (int e = lane; e < (POST ? WA_N : WA_COS); e += 64) {
+        for (int e = lane; e < GW::N; e += 64) {
             double v = 0.0;
             for (int w = 0; w < WPT; ++w) v += s_gw.acc[w0 + w][e];
-            if (e >= WA_SAI) {
-                if (md.sai_w && e - WA_SAI < md.sai_base[md.I]) gl.g[md.off_sai + (e - WA_SAI)] = v;
-            } else if (e >= WA_COS) {
-                (&gl.cosg[0][0])[e - WA_COS] = v;
-            } else if (e >= WA_MAC) {
-                const bool tau = e >= WA_MAT;
-                const int r = e - (tau ? WA_MAT : WA_MAC), set = r / QM, c = r - set * QM;
+            if (POST && e >= GW::SAI) {
+                if (md.sai_w && e - GW::SAI < md.sai_base[md.I]) gl.g[md.off_sai + (e - GW::SAI)] = v;
+            } else if (POST && e >= GW::COS) {
+                (&gl.cosg[0][0])[e - GW::COS] = v;
+            } else if (e >= GW::MAC) {
+                const bool tau = e >= GW::MAT;
+                const int r = e - (tau ? GW::MAT : GW::MAC), set = r / GW::QM, c = r - set * GW::QM;
                 if (md.ma_active && set < (md.ma_global ? 1 : md.I) && c < md.ma_q)
                     gl.g[md.off_ma + 2 * md.ma_q * set + 2 * c + (tau ? 1 : 0)] = v;
-            } else if (e >= WA_JIT) {
-                if (md.has_jit && e - WA_JIT < md.I) gl.g[md.off_jit + (e - WA_JIT)] = v;
+            } else if (e >= GW::JIT) {
+                if (md.has_jit && e - GW::JIT < md.I) gl.g[md.off_jit + (e - GW::JIT)] = v;
             } else if (e < md.I) {
                 gl.g[md.off_off + e] = v;
             }
@@ -578,34 +656,40 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
         if (lane == 0) {
             if (bad) { v = -__builtin_inf(); atomicAdd(nonfinite, 1ull); }
             out_lnl[b] = v;
+            if (stale) stale[b] = 0;
         }
     }
 }
 
 // Each model runs the instance with only the features it uses: the terms
 // added after the MA cost scalar registers, MA orders above 1 vector
-// registers (the order-4 scans spill).
+// registers.
 template <int PREC, int WPT>
 static void launch_grad_t(const ModelDev& md, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
-                          unsigned long long* nf, double* scratch, hipStream_t s) {
+                          unsigned long long* nf, double* scratch, int* stale, hipStream_t s) {
     constexpr int SLOTS = NWAVES / WPT;
     const int64_t grid = (B + SLOTS - 1) / SLOTS;
     if (md.ma_active && md.ma_q > 1)
         hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, 2>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
-                           lnl, grad, nf, scratch);
+                           lnl, grad, nf, scratch, stale);
     else if (has_post(md))
         hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, 1>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
-                           lnl, grad, nf, scratch);
+                           lnl, grad, nf, scratch, stale);
     else
         hipLaunchKernelGGL((k_lnl_grad<PREC, WPT, 0>), dim3((unsigned)grid), dim3(BLOCK), 0, s, md, th, B, ld,
-                           lnl, grad, nf, scratch);
+                           lnl, grad, nf, scratch, stale);
 }
 
-// The MA adjoint keeps one row of eps per vector: a model-owned scratch of
-// B x N doubles, grown (never shrunk) on demand.
+// Doubles of scratch one launch over B vectors needs: with the moving
+// average, one row of eps per vector (B x N); none without it.
+size_t emp_grad_scratch_doubles(const EmpModel* m, int64_t B) {
+    return m->dev.ma_active ? (size_t)B * (size_t)m->dev.N : 0;
+}
+
+// The MA adjoint's scratch, model-owned, grown (never shrunk) on demand.
 static int ensure_grad_scratch(EmpModel* m, int64_t B) {
-    if (!m->dev.ma_active) return EMP_OK;
-    const size_t need = (size_t)B * (size_t)m->dev.N;
+    const size_t need = emp_grad_scratch_doubles(m, B);
+    if (!need) return EMP_OK;
     if (m->grad_scratch_cap >= need) return EMP_OK;
     if (m->d_grad_scratch) {
         EMP_HIP(hipDeviceSynchronize());
@@ -618,13 +702,29 @@ static int ensure_grad_scratch(EmpModel* m, int64_t B) {
     return EMP_OK;
 }
 
+// Waves per vector of the adjoint kernel for a batch of B.  Four by default:
+// with one or two waves per vector a workgroup holds four or eight slots of
+// LDS and the kernel's registers allow three or four waves per SIMD, with
+// four both allow four to six (measured on 8192 vectors of 7 planets x 10^4
+// epochs: mixed 5.1 ms at 4, 6.8 ms at 2, 6.1 ms at 8; FP64 within 4%).
+// Fewer when a vector has fewer than two epochs per lane, more while the
+// batch leaves SIMDs short of waves.
+int emp_pick_grad_wpt(const EmpModel* m, int64_t B) {
+    if (m->wpt_override) return m->wpt_override;
+    const int64_t target_waves = (int64_t)m->n_cu * 4 * 6;
+    int wpt = 4;
+    while (wpt > 1 && 64 * wpt > m->dev.N / 2) wpt /= 2;
+    while (wpt < 8 && B * wpt < target_waves && 64 * (wpt * 2) <= m->dev.N / 2) wpt *= 2;
+    return wpt;
+}
+
 int emp_launch_grad_ws(const EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
-                       double* eps_scratch, hipStream_t s, int precision) {
+                       double* eps_scratch, hipStream_t s, int precision, int* stale, int wpt) {
     if (B <= 0) return EMP_OK;
     if (m->dev.ma_active && !eps_scratch) { emp_set_error("moving-average gradient needs a scratch buffer"); return EMP_EINVAL; }
-    const int wpt = emp_pick_wpt(m, B, precision);
+    if (wpt <= 0) wpt = emp_pick_grad_wpt(m, B);
     unsigned long long* nf = m->d_nonfinite;
-#define EMP_CASE(P, W) if (precision == P && wpt == W) launch_grad_t<P, W>(m->dev, th, B, ld, lnl, grad, nf, eps_scratch, s); else
+#define EMP_CASE(P, W) if (precision == P && wpt == W) launch_grad_t<P, W>(m->dev, th, B, ld, lnl, grad, nf, eps_scratch, stale, s); else
     EMP_CASE(0, 1) EMP_CASE(0, 2) EMP_CASE(0, 4) EMP_CASE(0, 8)
     EMP_CASE(1, 1) EMP_CASE(1, 2) EMP_CASE(1, 4) EMP_CASE(1, 8) { emp_set_error("bad precision/wpt"); return EMP_EINVAL; }
 #undef EMP_CASE

@@ -141,5 +141,11 @@ int emp_launch_grad(EmpModel* m, const double* d_theta, int64_t B, int64_t ld, d
                     double* d_grad, hipStream_t s, int precision = EMP_FP64);
 // The same with a caller-owned scratch of B x N doubles (used only when the
 // moving average is active); allocates nothing, so it may be captured.
+// stale (optional, per vector): only vectors flagged 1 are computed, and their
+// flags are cleared.  wpt: waves per vector (0: emp_pick_grad_wpt).
 int emp_launch_grad_ws(const EmpModel* m, const double* d_theta, int64_t B, int64_t ld, double* d_lnl,
-                       double* d_grad, double* eps_scratch, hipStream_t s, int precision = EMP_FP64);
+                       double* d_grad, double* eps_scratch, hipStream_t s, int precision = EMP_FP64,
+                       int* stale = nullptr, int wpt = 0);
+int emp_pick_grad_wpt(const EmpModel* m, int64_t B);
+// Doubles of scratch one adjoint launch over B vectors needs.
+size_t emp_grad_scratch_doubles(const EmpModel* m, int64_t B);

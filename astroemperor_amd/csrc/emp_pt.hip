@@ -57,7 +57,7 @@ struct PTDev {
     double* h_tsw;      // [S][T-1]
     double* h_smd;      // [S][T-1]
     // Gradient (Langevin) move, allocated when the move is enabled.
-    double lg_eps, lg_e2h;  // step eps and eps^2 / 2
+    double* lg_eps;     // [T] step eps of each temperature
     double* lg_g;       // [slots][D] grad lnL at each slot's position
     double* lg_l;       // [slots] lnL from the adjoint kernel at each slot's position
     double* lg_L1;      // [T][D][D] Cholesky factor L of the complementary half's covariance, L1[k D + i] = L_ik
@@ -68,6 +68,8 @@ struct PTDev {
     double* lg_lp;      // [T W/2] lnL at the proposals
     double* lg_lq;      // [T W/2] log density of the forward proposal (up to the shared constant)
     double* lg_scr;     // [T W][N] moving-average adjoint scratch (null without the moving average)
+    int* lg_stale;      // [slots] 1: lg_g / lg_l are not those of the slot's position (a stretch move
+                        // moved it); cleared by the adjoint launch that recomputes them
 };
 
 struct EmpPT {
@@ -81,8 +83,12 @@ struct EmpPT {
     hipGraphExec_t graph_run[5] = {};         // 2, 4, 8, 16, 32 stretch sweeps in one graph
     double w_stretch = 1.0, w_grad = 0.0;     // move weights (one move per sweep)
     int wpt = 1;
+    int grad_wpt = 2;            // waves per vector of the adjoint launches (one shape for both, so a
+                                 // slot's gradient does not depend on which launch computed it)
     size_t swap_lds = 0;         // dynamic LDS of the swap kernel, 0 = global path
     long long host_sweeps = 0;   // mirror of the device counter
+    long long hist_base = 0;     // first sweep whose history the engine holds (a restore from a
+                                 // checkpoint without chain rows starts the history there)
     double last_ms_per_sweep = -1.0, last_ms_half = -1.0;
 };
 
@@ -209,6 +215,8 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
     }
     pt_eval_accept<PREC, WPT, true, POST>(md, pv, s_ops, live, sl, s_tile, s_scan, s_acc, slot, wsub, false,
                               (unsigned long long*)(pd.sweep + 1));
+    // an accepted stretch move leaves the slot's gradient out of date
+    if (pd.lg_stale && live && wsub == 0 && lane == 0 && s_acc[slot]) pd.lg_stale[s_ops[slot].sk] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -338,6 +346,11 @@ __global__ void k_mask_lnl(double* lnl, const double* lnp, int64_t n) {
 __global__ void k_identity(int* map, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) map[i] = (int)i;
+}
+
+__global__ void k_fill_i32(int* a, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -597,6 +610,7 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_propose(ModelDev md, PTDev p
     const int t = b / pd.W2, k = half * pd.W2 + b % pd.W2;
     const int sk = pd.map[t * pd.W + k];
     const double beta = pd.betas[t];
+    const double eps = pd.lg_eps[t], e2h = 0.5 * eps * eps;
     const double* x = pd.X + (size_t)sk * D;
     const double* g = pd.lg_g + (size_t)sk * D;
     const double* L1 = pd.lg_L1 + (size_t)t * D * D;
@@ -618,7 +632,7 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_propose(ModelDev md, PTDev p
         const int j = lane + 64 * h;
         if (j < D && act[j]) {
             const double xi = lg_normal(pd, sweep, half, step, t, k, j);
-            u[h] = __builtin_fma(pd.lg_e2h * beta, v[h], pd.lg_eps * xi);
+            u[h] = __builtin_fma(e2h * beta, v[h], eps * xi);
             lq -= 0.5 * xi * xi;
         }
     }
@@ -693,11 +707,12 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_lg_accept(PTDev pd, int half, i
         }
     }
     // reverse proposal: xi' = r / eps - (eps / 2) beta L^T g(y)
+    const double eps = pd.lg_eps[t];
     double lq = 0.0;
     for (int h = 0; h < 2; ++h) {
         const int j = lane + 64 * h;
         if (j < D && act[j]) {
-            const double rho = dv[h] / pd.lg_eps - 0.5 * pd.lg_eps * beta * v[h];
+            const double rho = dv[h] / eps - 0.5 * eps * beta * v[h];
             lq -= 0.5 * rho * rho;
         }
     }
@@ -815,18 +830,21 @@ static int launch_sweep(EmpPT* p) {
     return launch_tail(p);
 }
 
-// A sweep of gradient moves: gradients at every current position, then per
-// half-step the preconditioner, the proposals, their gradients and the
-// acceptance.  Allocates nothing (the scratch is the engine's own), so it
-// can be captured.
-static int launch_grad_sweep(EmpPT* p) {
+// A sweep of gradient moves: gradients at the current positions that a
+// stretch move has moved since their last gradient (the other slots keep
+// theirs: swaps exchange map entries, not states), then per half-step the
+// preconditioner, the proposals, their gradients and the acceptance.
+// Allocates nothing (the scratch is the engine's own), so it can be captured.
+// ev: optional [2 nsteps][2] events recorded around each proposal adjoint.
+static int launch_grad_sweep(EmpPT* p, hipEvent_t* ev = nullptr) {
     const PTDev& d = p->dev;
     const ModelDev& md = p->model->dev;
     const int64_t TW = (int64_t)d.T * d.W, tasks = (int64_t)d.T * d.W2;
     const unsigned tgrid = (unsigned)((tasks + PRIOR_WAVES - 1) / PRIOR_WAVES);
     int rc = launch_record(p, 1);
     if (rc) return rc;
-    rc = emp_launch_grad_ws(p->model, d.X, TW, d.D, d.lg_l, d.lg_g, d.lg_scr, p->stream, p->desc.precision);
+    rc = emp_launch_grad_ws(p->model, d.X, TW, d.D, d.lg_l, d.lg_g, d.lg_scr, p->stream, p->desc.precision,
+                            d.lg_stale, p->grad_wpt);
     if (rc) return rc;
     for (int st = 0; st < p->desc.nsteps; ++st)
         for (int h = 0; h < 2; ++h) {
@@ -835,9 +853,12 @@ static int launch_grad_sweep(EmpPT* p) {
             EMP_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_lg_propose, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, md, d, h, st);
             EMP_HIP(hipGetLastError());
+            hipEvent_t* e = ev ? ev + 2 * (2 * st + h) : nullptr;
+            if (e) EMP_HIP(hipEventRecord(e[0], p->stream));
             rc = emp_launch_grad_ws(p->model, d.lg_Xp, tasks, d.D, d.lg_lp, d.lg_gp, d.lg_scr, p->stream,
-                                    p->desc.precision);
+                                    p->desc.precision, nullptr, p->grad_wpt);
             if (rc) return rc;
+            if (e) EMP_HIP(hipEventRecord(e[1], p->stream));
             hipLaunchKernelGGL(k_lg_accept, dim3(tgrid), dim3(PRIOR_BLOCK), 0, p->stream, d, h, st);
             EMP_HIP(hipGetLastError());
         }
@@ -895,6 +916,17 @@ static int alloc(EmpPT* p, T** out, size_t n) {
     return EMP_OK;
 }
 
+// Every slot's gradient is recomputed by the next gradient sweep (after a
+// state was loaded or restored; no-op while the gradient move was never on).
+static int mark_all_stale(EmpPT* p) {
+    PTDev& d = p->dev;
+    if (!d.lg_stale) return EMP_OK;
+    const int64_t n = (int64_t)d.T * d.W;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, d.lg_stale, n, 1);
+    EMP_HIP(hipGetLastError());
+    return EMP_OK;
+}
+
 // Evaluates lnp and lnl of every slot (slots hold the positions in logical
 // order when the map is the identity, which is how states are loaded).
 static int eval_state(EmpPT* p) {
@@ -902,6 +934,7 @@ static int eval_state(EmpPT* p) {
     const int64_t n = (int64_t)d.T * d.W;
     hipLaunchKernelGGL(k_identity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, d.map, n);
     EMP_HIP(hipGetLastError());
+    if (int rc = mark_all_stale(p)) return rc;
     int rc = emp_launch_prior(p->model, d.X, n, d.D, d.lnp, p->stream);
     if (rc) return rc;
     rc = emp_launch_lnl(p->model, d.X, n, d.D, d.lnl, p->desc.precision, 0, p->stream);
@@ -1082,6 +1115,41 @@ extern "C" int emp_pt_time_halfstep(EmpPT* p, int32_t n, double* ms_avg) {
     return EMP_OK;
 }
 
+// Runs ceil(n / (2 nsteps)) gradient sweeps launch by launch (no graph) and
+// times the first n adjoint launches over the proposals (T W / 2 vectors
+// each) with events recorded on the engine stream around each launch.  The
+// sweeps are ordinary ones: they advance the chain and use history capacity.
+extern "C" int emp_pt_time_gradstep(EmpPT* p, int32_t n, double* ms_avg) {
+    if (!p || n < 1 || !ms_avg) { emp_set_error("bad argument"); return EMP_EINVAL; }
+    if (!(p->w_grad > 0.0) || !p->dev.lg_g) { emp_set_error("the gradient move is not enabled"); return EMP_EINVAL; }
+    const int per = 2 * p->desc.nsteps;
+    const int64_t ns = ((int64_t)n + per - 1) / per;
+    if (p->host_sweeps + ns > p->dev.max_sweeps) {
+        emp_set_error("history capacity %lld sweeps exceeded (%lld done, %lld needed to time %d launches)",
+                      (long long)p->dev.max_sweeps, p->host_sweeps, (long long)ns, n);
+        return EMP_EINVAL;
+    }
+    std::vector<hipEvent_t> ev(2 * (size_t)per * ns, nullptr);
+    int rc = EMP_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { rc = emp_hip_fail(hipGetLastError(), "event", __FILE__, __LINE__); break; }
+    for (int64_t sw = 0; sw < ns && !rc; ++sw) rc = launch_grad_sweep(p, ev.data() + 2 * (size_t)per * sw);
+    if (!rc) rc = launch_record(p, 0);
+    if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "sync", __FILE__, __LINE__);
+    double tot = 0.0;
+    for (int i = 0; i < n && !rc; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) rc = emp_hip_fail(hipGetLastError(), "timing", __FILE__, __LINE__);
+        tot += ms;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    p->host_sweeps += ns;
+    *ms_avg = tot / n;
+    return EMP_OK;
+}
+
 // Times graph-replayed stretch sweeps at 1, 2, 4 and 8 waves per walker on a
 // copy of the state (positions, likelihoods, map, ladder, counters),
 // restores the state after each shape and keeps the fastest.  Each shape: an
@@ -1182,6 +1250,9 @@ extern "C" int emp_pt_get(EmpPT* p, int32_t what, void* host_out, int64_t cap, i
     EMP_HIP(hipMemcpy(&done, v.sweep, sizeof(done), hipMemcpyDeviceToHost));
     const long long S = std::min<long long>(done, v.max_sweeps);
     const long long R = std::min<long long>(done * v.nsteps, v.max_rows);     // history rows (one per step)
+    // history held from sweep S0 (row R0) on
+    const long long S0 = std::min<long long>(p->hist_base, S), R0 = S0 * v.nsteps;
+    const size_t TW = (size_t)v.T * v.W, T1 = (size_t)std::max(v.T - 1, 1);
     int64_t sh[4] = {1, 1, 1, 1};
     const void* src = nullptr;
     size_t elem = sizeof(double);
@@ -1192,18 +1263,25 @@ extern "C" int emp_pt_get(EmpPT* p, int32_t what, void* host_out, int64_t cap, i
     }
     bool host_src = false;
     switch (what) {
-        case EMP_PT_CHAIN: if (!v.record) goto norec; sh[0] = R; sh[1] = v.T; sh[2] = v.W; sh[3] = v.D; src = v.h_chain; break;
-        case EMP_PT_LNL: if (!v.record) goto norec; sh[0] = R; sh[1] = v.T; sh[2] = v.W; src = v.h_lnl; break;
-        case EMP_PT_LNP: if (!v.record) goto norec; sh[0] = R; sh[1] = v.T; sh[2] = v.W; src = v.h_lnp; break;
-        case EMP_PT_BETAS: sh[0] = S; sh[1] = v.T; src = v.h_betas; break;
-        case EMP_PT_TSW: sh[0] = S; sh[1] = std::max(v.T - 1, 0); src = v.h_tsw; break;
-        case EMP_PT_SMD: sh[0] = S; sh[1] = std::max(v.T - 1, 0); src = v.h_smd; break;
+        case EMP_PT_CHAIN:
+            if (!v.record) goto norec;
+            sh[0] = R - R0; sh[1] = v.T; sh[2] = v.W; sh[3] = v.D; src = v.h_chain + (size_t)R0 * TW * v.D;
+            break;
+        case EMP_PT_LNL: if (!v.record) goto norec; sh[0] = R - R0; sh[1] = v.T; sh[2] = v.W; src = v.h_lnl + (size_t)R0 * TW; break;
+        case EMP_PT_LNP: if (!v.record) goto norec; sh[0] = R - R0; sh[1] = v.T; sh[2] = v.W; src = v.h_lnp + (size_t)R0 * TW; break;
+        case EMP_PT_BETAS: sh[0] = S - S0; sh[1] = v.T; src = v.h_betas + (size_t)S0 * v.T; break;
+        case EMP_PT_TSW: sh[0] = S - S0; sh[1] = std::max(v.T - 1, 0); src = v.h_tsw + (size_t)S0 * T1; break;
+        case EMP_PT_SMD: sh[0] = S - S0; sh[1] = std::max(v.T - 1, 0); src = v.h_smd + (size_t)S0 * T1; break;
         case EMP_PT_ACCEPTED: sh[0] = v.T; sh[1] = v.W; src = v.accepted; elem = sizeof(long long); break;
         case EMP_PT_STATE_X: sh[0] = v.T; sh[1] = v.W; sh[2] = v.D; src = Xg.data(); host_src = true; break;
         case EMP_PT_STATE_LNL: sh[0] = v.T; sh[1] = v.W; src = lg.data(); host_src = true; break;
         case EMP_PT_STATE_LNP: sh[0] = v.T; sh[1] = v.W; src = pg.data(); host_src = true; break;
         case EMP_PT_STATE_BETAS: sh[0] = v.T; src = v.betas; break;
         case EMP_PT_NSWEEPS: sh[0] = 2; src = v.sweep; elem = sizeof(long long); break;
+        case EMP_PT_GRAD_STEPS:
+            if (!v.lg_eps) { emp_set_error("the gradient move was never enabled"); return EMP_EINVAL; }
+            sh[0] = v.T; src = v.lg_eps;
+            break;
         default: emp_set_error("unknown quantity %d", what); return EMP_EINVAL;
     }
     {
@@ -1241,7 +1319,10 @@ extern "C" int emp_pt_lnl_moments(EmpPT* p, int64_t discard, double* mean, doubl
     long long done = 0;
     EMP_HIP(hipMemcpy(&done, v.sweep, sizeof(done), hipMemcpyDeviceToHost));
     const long long S = std::min<long long>(done * v.nsteps, v.max_rows);      // history rows
-    if (discard < 0 || discard >= S) { emp_set_error("discard must lie in [0, %lld) rows", S); return EMP_EINVAL; }
+    // rows held start at the history base; discard counts from there
+    const long long R0 = std::min<long long>(p->hist_base * v.nsteps, S);
+    if (discard < 0 || R0 + discard >= S) { emp_set_error("discard must lie in [0, %lld) rows", S - R0); return EMP_EINVAL; }
+    discard += R0;
     double* d = nullptr;
     EMP_HIP(hipMalloc(&d, 3 * (size_t)v.T * sizeof(double)));
     hipLaunchKernelGGL(k_pt_lnl_moments, dim3((unsigned)v.T), dim3(MOM_BLOCK), 0, p->stream, v, (long long)discard, S,
@@ -1288,17 +1369,35 @@ extern "C" int emp_pt_set_moves(EmpPT* p, const EmpMoveDesc* moves, int32_t n) {
             (rc = alloc(p, &v.lg_L1, (size_t)v.T * D * D)) || (rc = alloc(p, &v.lg_L2, (size_t)v.T * D * D)) ||
             (rc = alloc(p, &v.lg_act, (size_t)v.T * D)) || (rc = alloc(p, &v.lg_Xp, T2 * D)) ||
             (rc = alloc(p, &v.lg_gp, T2 * D)) || (rc = alloc(p, &v.lg_lp, T2)) || (rc = alloc(p, &v.lg_lq, T2)) ||
+            (rc = alloc(p, &v.lg_eps, (size_t)v.T)) || (rc = alloc(p, &v.lg_stale, TW)) ||
             (m->dev.ma_active && (rc = alloc(p, &v.lg_scr, TW * (size_t)m->dev.N)))) {
             v.lg_g = nullptr;     // the allocations stay with the engine; a later call retries
+            v.lg_stale = nullptr;
             return rc;
         }
+        p->grad_wpt = emp_pick_grad_wpt(m, (int64_t)T2);
+        if ((rc = mark_all_stale(p))) return rc;
+        EMP_HIP(hipStreamSynchronize(p->stream));
     }
     v.a = a;
-    v.lg_eps = eps;
-    v.lg_e2h = 0.5 * eps * eps;
+    if (wg > 0.0) {
+        const std::vector<double> steps((size_t)v.T, eps);
+        EMP_HIP(hipMemcpy(v.lg_eps, steps.data(), steps.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     p->w_stretch = ws;
     p->w_grad = wg;
     drop_graphs(p);
+    return EMP_OK;
+}
+
+extern "C" int emp_pt_set_gradient_steps(EmpPT* p, const double* eps) {
+    if (!p || !eps) { emp_set_error("null argument"); return EMP_EINVAL; }
+    PTDev& v = p->dev;
+    if (!(p->w_grad > 0.0) || !v.lg_eps) { emp_set_error("the gradient move is not enabled (emp_pt_set_moves)"); return EMP_EINVAL; }
+    for (int t = 0; t < v.T; ++t)
+        if (!(eps[t] > 0.0 && eps[t] < __builtin_inf())) { emp_set_error("gradient step %d must be finite and > 0", t); return EMP_EINVAL; }
+    EMP_HIP(hipStreamSynchronize(p->stream));
+    EMP_HIP(hipMemcpy(v.lg_eps, eps, (size_t)v.T * sizeof(double), hipMemcpyHostToDevice));
     return EMP_OK;
 }
 
@@ -1318,30 +1417,64 @@ extern "C" int emp_pt_set_state(EmpPT* p, const double* x, const double* betas) 
     return EMP_OK;
 }
 
-// Checkpoint layout: magic, version, T, W, D, sweeps, then X, lnl, lnp (in
-// logical walker order), betas, accepted (little-endian native binary).
+// Checkpoint layout (version 2, little-endian native binary): magic; int32
+// {version, T, W, D, nsteps, record}; int64 {sweeps, likelihoods evaluated,
+// first sweep of the history held}; X, lnl, lnp in logical walker order;
+// betas; accepted; then the history from that first sweep on: betas, tsw and
+// smd per sweep and, when record = 1, the chain, lnl and lnp rows.
 static const char CK_MAGIC[8] = {'E', 'M', 'P', 'P', 'T', 'C', 'K', '1'};
+constexpr int32_t CK_VERSION = 2;
+constexpr size_t CK_CHUNK = (size_t)1 << 23;     // doubles per staged block of history (64 MiB)
+
+// Device <-> file copies of n doubles through a bounded host buffer.
+static bool ck_put(FILE* f, const double* dev, size_t n, std::vector<double>& buf) {
+    for (size_t o = 0; o < n; o += CK_CHUNK) {
+        const size_t c = std::min(CK_CHUNK, n - o);
+        buf.resize(c);
+        if (hipMemcpy(buf.data(), dev + o, c * 8, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        if (fwrite(buf.data(), 8, c, f) != c) return false;
+    }
+    return true;
+}
+
+static bool ck_get(FILE* f, double* dev, size_t n, std::vector<double>& buf) {
+    for (size_t o = 0; o < n; o += CK_CHUNK) {
+        const size_t c = std::min(CK_CHUNK, n - o);
+        buf.resize(c);
+        if (fread(buf.data(), 8, c, f) != c) return false;
+        if (dev && hipMemcpy(dev + o, buf.data(), c * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
+    }
+    return true;
+}
 
 extern "C" int emp_pt_checkpoint(EmpPT* p, const char* path) {
     if (!p || !path) { emp_set_error("null argument"); return EMP_EINVAL; }
     const PTDev& v = p->dev;
     EMP_HIP(hipStreamSynchronize(p->stream));
-    const size_t TW = (size_t)v.T * v.W;
-    std::vector<double> X, l, pr, b(v.T);
+    const size_t TW = (size_t)v.T * v.W, T1 = (size_t)std::max(v.T - 1, 1);
+    std::vector<double> X, l, pr, b(v.T), buf;
     std::vector<long long> acc(TW);
     int rc = gather_state(p, X, l, pr);
     if (rc) return rc;
-    long long sw = 0;
+    long long cnt[2] = {0, 0};
     EMP_HIP(hipMemcpy(b.data(), v.betas, b.size() * 8, hipMemcpyDeviceToHost));
     EMP_HIP(hipMemcpy(acc.data(), v.accepted, acc.size() * 8, hipMemcpyDeviceToHost));
-    EMP_HIP(hipMemcpy(&sw, v.sweep, 8, hipMemcpyDeviceToHost));
+    EMP_HIP(hipMemcpy(cnt, v.sweep, 16, hipMemcpyDeviceToHost));
+    const long long sw = cnt[0], base = std::min<long long>(p->hist_base, sw);
+    const size_t ns = (size_t)(sw - base), nr = ns * (size_t)v.nsteps, r0 = (size_t)base * v.nsteps;
     FILE* f = fopen(path, "wb");
     if (!f) { emp_set_error("cannot open %s for writing", path); return EMP_EIO; }
-    int32_t hdr[4] = {1, v.T, v.W, v.D};
-    bool ok = fwrite(CK_MAGIC, 1, 8, f) == 8 && fwrite(hdr, 4, 4, f) == 4 && fwrite(&sw, 8, 1, f) == 1 &&
+    int32_t hdr[6] = {CK_VERSION, v.T, v.W, v.D, v.nsteps, v.record};
+    long long hdr64[3] = {sw, cnt[1], base};
+    bool ok = fwrite(CK_MAGIC, 1, 8, f) == 8 && fwrite(hdr, 4, 6, f) == 6 && fwrite(hdr64, 8, 3, f) == 3 &&
               fwrite(X.data(), 8, X.size(), f) == X.size() && fwrite(l.data(), 8, l.size(), f) == l.size() &&
               fwrite(pr.data(), 8, pr.size(), f) == pr.size() && fwrite(b.data(), 8, b.size(), f) == b.size() &&
               fwrite(acc.data(), 8, acc.size(), f) == acc.size();
+    ok = ok && ck_put(f, v.h_betas + (size_t)base * v.T, ns * v.T, buf) &&
+         ck_put(f, v.h_tsw + (size_t)base * T1, ns * T1, buf) && ck_put(f, v.h_smd + (size_t)base * T1, ns * T1, buf);
+    if (v.record)
+        ok = ok && ck_put(f, v.h_chain + r0 * TW * v.D, nr * TW * v.D, buf) &&
+             ck_put(f, v.h_lnl + r0 * TW, nr * TW, buf) && ck_put(f, v.h_lnp + r0 * TW, nr * TW, buf);
     ok = (fclose(f) == 0) && ok;
     if (!ok) { emp_set_error("write error on %s", path); return EMP_EIO; }
     return EMP_OK;
@@ -1353,31 +1486,60 @@ extern "C" int emp_pt_restore(EmpPT* p, const char* path) {
     FILE* f = fopen(path, "rb");
     if (!f) { emp_set_error("cannot open %s", path); return EMP_EIO; }
     char mg[8];
-    int32_t hdr[4];
-    long long sw = 0;
-    const size_t TW = (size_t)v.T * v.W;
-    std::vector<double> X(TW * v.D), l(TW), pr(TW), b(v.T);
+    int32_t hdr[6];
+    long long hdr64[3];
+    const size_t TW = (size_t)v.T * v.W, T1 = (size_t)std::max(v.T - 1, 1);
+    std::vector<double> X(TW * v.D), l(TW), pr(TW), b(v.T), buf;
     std::vector<long long> acc(TW);
-    bool ok = fread(mg, 1, 8, f) == 8 && memcmp(mg, CK_MAGIC, 8) == 0 && fread(hdr, 4, 4, f) == 4 &&
-              hdr[0] == 1 && hdr[1] == v.T && hdr[2] == v.W && hdr[3] == v.D && fread(&sw, 8, 1, f) == 1 &&
+    bool ok = fread(mg, 1, 8, f) == 8 && memcmp(mg, CK_MAGIC, 8) == 0 && fread(hdr, 4, 6, f) == 6 &&
+              hdr[0] == CK_VERSION && hdr[1] == v.T && hdr[2] == v.W && hdr[3] == v.D && hdr[4] == v.nsteps &&
+              (hdr[5] == 0 || hdr[5] == 1) && fread(hdr64, 8, 3, f) == 3 &&
               fread(X.data(), 8, X.size(), f) == X.size() && fread(l.data(), 8, l.size(), f) == l.size() &&
               fread(pr.data(), 8, pr.size(), f) == pr.size() && fread(b.data(), 8, b.size(), f) == b.size() &&
               fread(acc.data(), 8, acc.size(), f) == acc.size();
-    fclose(f);
-    if (!ok) { emp_set_error("%s is not a checkpoint of this engine shape", path); return EMP_EIO; }
-    if (sw > v.max_sweeps) { emp_set_error("checkpoint has %lld sweeps, capacity is %lld", sw, (long long)v.max_sweeps); return EMP_EINVAL; }
+    if (!ok) { fclose(f); emp_set_error("%s is not a checkpoint of this engine shape (T, W, D, nsteps)", path); return EMP_EIO; }
+    const long long sw = hdr64[0], nlnl = hdr64[1], base = hdr64[2];
+    if (sw < 0 || nlnl < 0 || base < 0 || base > sw) { fclose(f); emp_set_error("%s: bad sweep counters", path); return EMP_EIO; }
+    if (sw > v.max_sweeps) {
+        fclose(f);
+        emp_set_error("checkpoint has %lld sweeps, capacity is %lld", sw, (long long)v.max_sweeps);
+        return EMP_EINVAL;
+    }
+    for (int i = 0; i < v.T; ++i)
+        if (!(b[i] >= 0.0 && b[i] <= 1.0)) { fclose(f); emp_set_error("%s: betas outside [0, 1]", path); return EMP_EIO; }
     EMP_HIP(hipStreamSynchronize(p->stream));
+    // A recording engine restored from a file without chain rows holds the
+    // history from the restored sweep on (hist_base); otherwise it takes the
+    // file's history.
+    const bool file_rec = hdr[5] == 1;
+    const long long nbase = (v.record && !file_rec) ? sw : base;
+    const size_t ns = (size_t)(sw - base), nr = ns * (size_t)v.nsteps, r0 = (size_t)base * v.nsteps;
+    const bool take = nbase == base;
+    ok = ck_get(f, take ? v.h_betas + (size_t)base * v.T : nullptr, ns * v.T, buf) &&
+         ck_get(f, take ? v.h_tsw + (size_t)base * T1 : nullptr, ns * T1, buf) &&
+         ck_get(f, take ? v.h_smd + (size_t)base * T1 : nullptr, ns * T1, buf);
+    if (ok && file_rec) {
+        const bool into = v.record && take;
+        ok = ck_get(f, into ? v.h_chain + r0 * TW * v.D : nullptr, nr * TW * v.D, buf) &&
+             ck_get(f, into ? v.h_lnl + r0 * TW : nullptr, nr * TW, buf) &&
+             ck_get(f, into ? v.h_lnp + r0 * TW : nullptr, nr * TW, buf);
+    }
+    fclose(f);
+    if (!ok) { emp_set_error("%s: truncated history", path); return EMP_EIO; }
+    const long long cnt[2] = {sw, nlnl};
     EMP_HIP(hipMemcpy(v.X, X.data(), X.size() * 8, hipMemcpyHostToDevice));
     EMP_HIP(hipMemcpy(v.lnl, l.data(), l.size() * 8, hipMemcpyHostToDevice));
     EMP_HIP(hipMemcpy(v.lnp, pr.data(), pr.size() * 8, hipMemcpyHostToDevice));
     EMP_HIP(hipMemcpy(v.betas, b.data(), b.size() * 8, hipMemcpyHostToDevice));
     EMP_HIP(hipMemcpy(v.accepted, acc.data(), acc.size() * 8, hipMemcpyHostToDevice));
-    EMP_HIP(hipMemcpy(v.sweep, &sw, 8, hipMemcpyHostToDevice));
+    EMP_HIP(hipMemcpy(v.sweep, cnt, 16, hipMemcpyHostToDevice));
     EMP_HIP(hipMemset(v.rec_owed, 0, sizeof(int)));
     const int64_t n = (int64_t)TW;
     hipLaunchKernelGGL(k_identity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, v.map, n);
     EMP_HIP(hipGetLastError());
+    if (int rc = mark_all_stale(p)) return rc;
     EMP_HIP(hipStreamSynchronize(p->stream));
     p->host_sweeps = sw;
+    p->hist_base = nbase;
     return EMP_OK;
 }

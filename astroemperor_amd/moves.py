@@ -5,6 +5,8 @@ engine configuration leaves it at None (emp/emp.py:2373), i.e. the stretch
 move alone, which stays the default here.  Each sweep makes one kind of move at
 every temperature, chosen with probability weight / sum(weights).
 """
+import numpy as np
+
 from . import _abi
 
 
@@ -32,20 +34,31 @@ class GradientMove:
     where L L^T is the covariance of the other half's walkers at the same
     temperature (Leimkuhler, Matthews & Weare 2018), and accepts with the exact
     Metropolis-Hastings ratio of beta lnL + ln prior (Roberts & Tweedie 1996).
-    Gradients come from the FP64 adjoint kernel.  The default step,
-    1.65 ndim^(-1/6), is Roberts & Rosenthal's (1998) optimal MALA scale for a
-    target whose covariance the preconditioner matches.
+    Gradients come from the adjoint kernel.  The default step, 1.65
+    ndim^(-1/6), is Roberts & Rosenthal's (1998) optimal MALA scale for a
+    target whose covariance the preconditioner matches.  ``step`` may also
+    be one value per temperature (hottest last, as the ladder), e.g. the
+    steps :meth:`PTSampler.adapt_gradient_steps` found.
     """
 
     kind = _abi.EMP_MOVE_GRADIENT
 
     def __init__(self, step=None):
-        self.step = None if step is None else float(step)
+        if step is None or np.ndim(step) == 0:
+            self.step = None if step is None else float(step)
+            self.steps = None
+        else:
+            self.steps = np.ascontiguousarray(step, dtype=np.float64)
+            if self.steps.ndim != 1 or not np.all((self.steps > 0) & np.isfinite(self.steps)):
+                raise ValueError("per-temperature steps must be a 1-D array of finite values > 0")
+            self.step = float(self.steps[0])
 
     def scale(self, ndim):
         return self.step if self.step is not None else 1.65 * float(ndim) ** (-1.0 / 6.0)
 
     def __repr__(self):
+        if self.steps is not None:
+            return f"GradientMove(steps={np.array2string(self.steps, precision=3)})"
         return f"GradientMove(step={self.step})"
 
 

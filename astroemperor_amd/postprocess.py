@@ -49,9 +49,10 @@ class StatsMixin:
         self.constrain = {"method": "range", "sigma": 3, "types": ["Keplerian", "Jitter"],
                           "tol": 1e-4, "known_methods": ["sigma", "GM", "range", "None"]}
         self.switch_evidence = True
-        # reddemcee's default 'hybrid' estimator is not available; 'ti' is the
-        # estimator EMPEROR falls back to (emp.py:1432-1447)
-        self.evidence_method = "ti"
+        # EMPEROR's default (emp.py:129).  reddemcee's hybrid estimator is not
+        # available here: get_evidence_hybrid raises, and the flow falls back
+        # to classic thermodynamic integration as emp.py:1443-1446 does.
+        self.evidence_method = "hybrid"
         self._crit_tolerance = 5
         self._crit_current = "BIC"
         self._crit_msg = ""
@@ -127,6 +128,7 @@ class StatsMixin:
             if z[0] == -np.inf:
                 z = self.sampler.get_evidence_ti(discard=d, pchip=False)[:2]
         except Exception:
+            print("Thermodynamic Integration interpolation failed! Using classic method...")
             z = self.sampler.get_evidence_ti(discard=d, pchip=False)[:2]
         return float(z[0]), float(z[1])
 

@@ -166,8 +166,16 @@ class PTSampler:
         self._pt = h
         self._capacity = capacity
         self._nsteps = int(nsteps)
+        self._apply_moves()
+
+    def _apply_moves(self):
         descs = move_descs(self.moves, self.ndim)
-        _abi.check(L.emp_pt_set_moves(h, C.addressof(descs), len(descs)))
+        _abi.check(_abi.lib().emp_pt_set_moves(self._pt, C.addressof(descs), len(descs)))
+        for m, w in self.moves:
+            if getattr(m, "steps", None) is not None and w > 0:
+                if len(m.steps) != self.ntemps:
+                    raise ValueError(f"GradientMove has {len(m.steps)} steps for {self.ntemps} temperatures")
+                _abi.check(_abi.lib().emp_pt_set_gradient_steps(self._pt, m.steps.ctypes.data))
 
     def close(self):
         if self._pt:
@@ -204,8 +212,56 @@ class PTSampler:
         """Replace the move mix (emcee's forms); takes effect from the next sweep."""
         self.moves = normalise_moves(moves, self.a)
         if self._pt is not None:
-            descs = move_descs(self.moves, self.ndim)
-            _abi.check(_abi.lib().emp_pt_set_moves(self._pt, C.addressof(descs), len(descs)))
+            self._apply_moves()
+
+    @property
+    def gradient_steps(self):
+        """The gradient move's step at each temperature [T] (once the move
+        has been enabled)."""
+        return self._get(12).ravel()
+
+    def adapt_gradient_steps(self, nsweeps=16, target=0.5, batch=1, step=None):
+        """Burn-in for the gradient move: runs `nsweeps` gradient-only sweeps
+        in batches of `batch` and after each batch moves every temperature's
+        step towards the acceptance `target` (in log step: x10 or /10 until
+        the target is bracketed, then bisection), then restores the move mix
+        with the adapted steps held fixed.  The sweeps are ordinary sweeps of
+        the chain.  Returns the steps [T]."""
+        if self._pt is None:
+            raise ValueError("run the sampler first (it needs an initial state)")
+        if not (0.0 < target < 1.0) or batch < 1:
+            raise ValueError("target must lie in (0, 1) and batch be >= 1")
+        from .moves import GradientMove
+        mix = self.moves
+        grad = [m for m, w in mix if isinstance(m, GradientMove)]
+        g0 = grad[0] if grad else GradientMove(step)
+        if g0.steps is not None:
+            logs = np.log(g0.steps.copy())
+        else:
+            logs = np.full(self.ntemps, np.log(g0.scale(self.ndim) if step is None else float(step)))
+        lo = np.full(self.ntemps, -np.inf)
+        hi = np.full(self.ntemps, np.inf)
+        done = 0
+        while done < nsweeps:
+            n = min(batch, nsweeps - done)
+            self.set_moves([(GradientMove(np.exp(logs)), 1.0)])
+            a0 = self.accepted.sum(1)
+            self.run_mcmc(None, nsweeps=n, nsteps=self._nsteps)
+            acc = (self.accepted.sum(1) - a0) / float(n * self._nsteps * self.nwalkers)
+            for t in range(self.ntemps):
+                if acc[t] > target:
+                    lo[t] = logs[t]
+                else:
+                    hi[t] = logs[t]
+                if np.isfinite(lo[t]) and np.isfinite(hi[t]):
+                    logs[t] = 0.5 * (lo[t] + hi[t])
+                else:
+                    logs[t] += np.log(10.0) if acc[t] > target else -np.log(10.0)
+            done += n
+        steps = np.exp(logs)
+        new_mix = [(GradientMove(steps), w) if isinstance(m, GradientMove) else (m, w) for m, w in mix]
+        self.set_moves(new_mix)
+        return steps
 
     def select_adjustment(self, code):
         """reddemcee's select_adjustment('00') freezes the ladder
@@ -224,7 +280,7 @@ class PTSampler:
         S = max(self._capacity, 1)
         R = S * max(self._nsteps or 1, 1)          # history rows: one per step
         cap = {0: R * T * W * D, 1: R * T * W, 2: R * T * W, 3: S * T, 4: S * T, 5: S * T,
-               6: T * W, 7: T * W * D, 8: T * W, 9: T * W, 10: T, 11: 2}[what]
+               6: T * W, 7: T * W * D, 8: T * W, 9: T * W, 10: T, 11: 2, 12: T}[what]
         buf = np.empty(cap, dtype=dtype)
         _abi.check(L.emp_pt_get(self._pt, what, buf.ctypes.data, buf.nbytes, shape))
         n = int(np.prod(list(shape)))
@@ -368,4 +424,12 @@ class PTSampler:
     def time_halfstep(self, n=20):
         v = C.c_double()
         _abi.check(_abi.lib().emp_pt_time_halfstep(self._pt, int(n), C.byref(v)))
+        return v.value
+
+    def time_gradstep(self, n=4):
+        """Mean duration (ms) of n adjoint launches over the proposals of
+        gradient sweeps, each timed with events on the engine stream; runs
+        ceil(n / (2 nsteps)) ordinary gradient sweeps."""
+        v = C.c_double()
+        _abi.check(_abi.lib().emp_pt_time_gradstep(self._pt, int(n), C.byref(v)))
         return v.value

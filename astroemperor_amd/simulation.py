@@ -11,8 +11,10 @@ switches (``keplerian_parameterisation``, ``use_c``, ``moav``,
 of writing and executing a script in a subprocess (emp/emp.py:137-175,
 2559-2582), the model is built in-process and sampled on the GPU.
 
-Extra ``engine_config`` keys: ``'backend'`` ('hip_fp64' or 'hip_mixed') and
-``'seed'``.  ``'moves'``, which the reference declares but never passes to
+Extra ``engine_config`` keys: ``'backend'`` ('hip_fp64' or 'hip_mixed'),
+``'seed'`` and ``'init'``: 'prior' (default, set_init/test_init as the
+reference) or 'map' (walkers drawn around the posterior mode that a batched
+L-BFGS search finds, with the Laplace covariance; ``autograd.map_init``).  ``'moves'``, which the reference declares but never passes to
 reddemcee (emp/emp.py:2373, 576-593), reaches the sampler here (e.g.
 ``[(StretchMove(), 0.8), (GradientMove(), 0.2)]``).  ``semantics`` selects 'reference' (default) or 'intended' MA.
 """
@@ -129,7 +131,7 @@ class Simulation(StatsMixin):
         self.engine_config = {"setup": np.array([5, 100, 500, 2]), "ntemps": 5, "betas": None,
                               "moves": None, "tsw_history": True, "smd_history": True,
                               "adapt_tau": 1000, "adapt_nu": 1, "adapt_mode": 0, "progress": True,
-                              "backend": "hip_fp64", "seed": None}
+                              "backend": "hip_fp64", "seed": None, "init": "prior"}
         self.run_config = {"burnin": None, "adaptation_batches": None, "adaptation_nsweeps": None,
                            "thin": 1, "discard": 0.5, "logger_level": "CRITICAL"}
 
@@ -214,7 +216,12 @@ class Simulation(StatsMixin):
         self.reddemcee_thin = self.run_config["thin"]
         seed = self.engine_config.get("seed")
         rng = np.random.default_rng(seed)
-        p1 = test_init(self.model, ntemps, nwalkers, rng)
+        if self.engine_config.get("init", "prior") == "map":
+            from .autograd import map_init
+            p1, self.map_mode, self.map_lnpost = map_init(self.model, ntemps, nwalkers, rng,
+                                                          betas=self.engine_config["betas"])
+        else:
+            p1 = test_init(self.model, ntemps, nwalkers, rng)
         prec = "mixed" if self.engine_config.get("backend") == "hip_mixed" else "fp64"
         self.sampler = PTSampler(nwalkers, self.model.ndim__, self.model.my_likelihood, self.model.my_prior,
                                  ntemps=ntemps, betas=self.engine_config["betas"],

@@ -188,6 +188,7 @@ typedef struct EmpPTDesc {
 #define EMP_PT_STATE_LNP 9  /* [T, W] */
 #define EMP_PT_STATE_BETAS 10 /* [T] */
 #define EMP_PT_NSWEEPS 11   /* [2] sweeps done, likelihoods evaluated (int64) */
+#define EMP_PT_GRAD_STEPS 12 /* [T] gradient-move step of each temperature (needs the move enabled once) */
 
 /* Replaces: reddemcee.PTSampler(nwalkers, ndim, my_likelihood, my_prior,
  * ntemps=, betas=, adapt_tau=, adapt_nu=, adapt_mode=) (emp/emp.py:576-593)
@@ -239,8 +240,14 @@ typedef struct EmpMoveDesc {
 } EmpMoveDesc;
 
 /* One or two moves, each kind at most once (default: stretch alone, weight 1,
- * a from EmpPTDesc). */
+ * a from EmpPTDesc).  A gradient move sets the step of every temperature to
+ * its scale. */
 int emp_pt_set_moves(EmpPT* pt, const EmpMoveDesc* moves, int32_t n);
+
+/* No reference counterpart: the gradient move's step per temperature
+ * (eps: [T] host, each finite and > 0), e.g. adapted during burn-in towards a
+ * target acceptance and then held fixed.  Needs the gradient move enabled. */
+int emp_pt_set_gradient_steps(EmpPT* pt, const double* eps);
 
 /* Evidence ingredients for reddemcee's get_evidence_ti / get_evidence_ss
  * (called at emp/emp.py:1434-1446; the estimators run on the host): from the
@@ -252,13 +259,26 @@ int emp_pt_set_moves(EmpPT* pt, const EmpMoveDesc* moves, int32_t n);
 int emp_pt_lnl_moments(EmpPT* pt, int64_t discard, double* mean_out, double* lse_out, double* var_out);
 
 /* No reference counterpart (the reference cannot resume a run): binary
- * checkpoint of the full engine state, bitwise-resumable. */
+ * checkpoint of the full engine state, bitwise-resumable, with the recorded
+ * history (ladder, tsw, smd per sweep; chain, lnL, ln prior rows when the
+ * engine records).  Restore checks the engine shape (T, W, D, nsteps), the
+ * sweep count against the capacity and the ladder (betas in [0, 1]).  A
+ * recording engine restored from a checkpoint without chain rows holds the
+ * history from the restored sweep on: getters return rows from there and
+ * emp_pt_lnl_moments counts `discard` from there. */
 int emp_pt_checkpoint(EmpPT* pt, const char* path);
 int emp_pt_restore(EmpPT* pt, const char* path);
 
 /* Average device time of the last emp_pt_run's sweeps in milliseconds
  * (HIP events on the engine stream), and of its half-step kernels. */
 int emp_pt_last_timing(EmpPT* pt, double* ms_per_sweep, double* ms_halfstep_kernel);
+
+/* No reference counterpart (measurement): runs ceil(n / (2 nsteps)) gradient
+ * sweeps launch by launch (no graph) and returns the mean duration of the
+ * first n adjoint launches over the proposals (T W / 2 vectors each), timed
+ * with HIP events recorded around each launch on the engine stream.  The
+ * sweeps are ordinary ones.  Needs the gradient move enabled. */
+int emp_pt_time_gradstep(EmpPT* pt, int32_t n, double* ms_avg);
 
 /* No reference counterpart (launch shape): times graph-replayed stretch
  * sweeps (8 x trials, after one untimed replay) at 1, 2, 4 and 8 waves per

@@ -31,7 +31,7 @@ def peg51():
 
 
 def make_sim(folder, k, parameterisation=1, moav=None, jitter=True, acceleration=0,
-             conditions=(), semantics="reference", sinusoid=0, magnetic_cycle=0, switch_SA=False):
+             conditions=(), semantics="reference", sinusoid=0, magnetic_cycle=0, switch_SA=False, use_c=False):
     """A Simulation with the blocks autorun(k, k) would add, conditions applied."""
     from astroemperor_amd import Simulation
     sim = Simulation()
@@ -40,6 +40,7 @@ def make_sim(folder, k, parameterisation=1, moav=None, jitter=True, acceleration
     sim.load_data(folder)
     sim.set_engine("reddemcee")
     sim.keplerian_parameterisation = parameterisation
+    sim.use_c = use_c
     sim.acceleration = acceleration
     sim.switch_jitter = jitter
     sim.semantics = semantics

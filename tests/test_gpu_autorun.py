@@ -76,6 +76,11 @@ def test_run_statistics_match_oracle():
     assert np.isclose(sim.DIC, dic, rtol=1e-10)
     z, dz = sim.evidence
     assert np.isfinite(z) and z < sim.like_max
+    # EMPEROR's default 'hybrid' is not available: the flow falls back to
+    # classic TI (emp.py:1443-1446)
+    assert sim.evidence_method == "hybrid"
+    classic = sim.sampler.get_evidence_ti(discard=sim.reddemcee_discard, pchip=False)[:2]
+    assert (z, dz) == (float(classic[0]), float(classic[1]))
     for j, p in enumerate(q for b in sim.blocks__ for q in b if q.fixed is None):
         assert p.value == sim.fit_max[j]
         assert p.limits[0] <= p.value_range[0] <= p.value_range[1] <= p.limits[1]

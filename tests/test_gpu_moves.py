@@ -78,7 +78,7 @@ def _replay(o, post, X, lnl, lnp, betas, sweep, seed, move, eps, a=2.0):
                     r = _philox([t * W + k, sweep & M32, sweep >> 32, 0x20000000 | (half << 8) | 255], key)
                     ua.append(_u01(r[0], r[1]))
                 got = orc.langevin_halfstep(X[t], lnl[t], lnp[t], grad[t], betas[t], active, other, xi, ua,
-                                            post, eps)
+                                            post, eps[t] if np.ndim(eps) else eps)
             else:
                 partner, uz, ua = [], [], []
                 for k in active:
@@ -109,7 +109,9 @@ def test_gradient_move_replay_matches_oracle(mix):
     sim = make_sim("51Peg", 1, parameterisation=1,
                    conditions=[("Period 1", "limits", [4.0, 4.5]), ("Amplitude 1", "limits", [40, 70])])
     m = sim.model
-    T, W, seed, eps = 3, 32, 0x2468_ace0_1357 + (mix == "gradient"), 0.5
+    T, W, seed = 3, 32, 0x2468_ace0_1357 + (mix == "gradient")
+    # one step for every temperature, or one per temperature
+    eps = 0.5 if mix == "gradient" else np.array([0.45, 0.6, 0.8])
     betas = np.array([1.0, 0.3, 0.05])
     ws, wg = (0.0, 1.0) if mix == "gradient" else (4.0, 1.0)
     burn, S = 400, 6
@@ -139,6 +141,34 @@ def test_gradient_move_replay_matches_oracle(mix):
     assert grad_acc >= 1, kinds
     if mix == "gradient":
         assert acc.sum() > 0
+    else:
+        assert np.array_equal(s.gradient_steps, eps)
+
+
+def test_gradient_step_adaptation_reaches_target():
+    """adapt_gradient_steps on a tempered Gaussian: after the burn-in the
+    acceptance of gradient sweeps at every temperature sits near the target,
+    and the move mix comes back with those steps held fixed."""
+    from astroemperor_amd import GradientMove, PTSampler, StretchMove
+    rng = np.random.default_rng(3)
+    m, mean, var = _gaussian_model(rng)
+    betas = np.array([1.0, 0.4, 0.1, 0.02])
+    T, W = len(betas), 128
+    p0 = mean + rng.normal(size=(T, W, 3)) * np.sqrt(var / betas[:, None, None])
+    s = PTSampler(W, 3, m, ntemps=T, betas=betas, adapt_mode=-1, seed=21,
+                  moves=[(StretchMove(), 4.0), (GradientMove(8.0), 1.0)])
+    s.run_mcmc(p0, nsweeps=4, capacity=200)
+    steps = s.adapt_gradient_steps(nsweeps=40, target=0.5, batch=2)
+    assert steps.shape == (T,) and np.array_equal(s.gradient_steps, steps)
+    kinds = [type(mv).__name__ for mv, _ in s.moves]
+    assert kinds == ["StretchMove", "GradientMove"] and [w for _, w in s.moves] == [4.0, 1.0]
+    s.set_moves([(GradientMove(steps), 1.0)])
+    a0 = s.accepted.sum(1)
+    s.run_mcmc(None, nsweeps=20)
+    acc = (s.accepted.sum(1) - a0) / (20.0 * W)
+    print(f"steps {steps}, acceptance {acc}", flush=True)
+    assert np.all(np.abs(acc - 0.5) < 0.15), acc
+    s.close()
 
 
 def test_gradient_move_gaussian_moments():
@@ -185,12 +215,17 @@ def test_gradient_mix_graph_equals_direct_and_resumes(tmp_path):
         if use_graph:
             s.checkpoint(tmp_path / "ck.bin")
             s.run_mcmc(None, nsweeps=10)
-            ref = s.get_chain()[:, -10:].copy()
             s2 = PTSampler(W, m.ndim__, m, ntemps=T, seed=77, moves=moves)
             s2.run_mcmc(p0, nsweeps=0, capacity=34)
             s2.restore(tmp_path / "ck.bin")
             s2.run_mcmc(None, nsweeps=10)
-            assert np.array_equal(s2.get_chain()[:, -10:], ref)
+            # a fresh engine restored from the checkpoint holds the whole
+            # history, the rows before the checkpoint included
+            assert s2.iteration == s.iteration == 34
+            for get in ("get_chain", "get_log_like", "get_log_prior", "get_betas", "get_tsw", "get_smd"):
+                a, b = getattr(s, get)(), getattr(s2, get)()
+                assert np.isfinite(b).all() and np.array_equal(a, b), get
+            assert np.array_equal(s.lnl_moments(4)[0], s2.lnl_moments(4)[0])
             s2.close()
         s.close()
     assert np.array_equal(out[0][0], out[1][0])

@@ -195,6 +195,52 @@ def test_checkpoint_restore_bitwise(tmp_path):
     assert np.array_equal(a.betas, b.betas)
 
 
+def test_restore_without_history_and_bad_files(tmp_path):
+    """A checkpoint of an engine that keeps no chain rows, restored into a
+    recording engine: the history starts at the restored sweep, every row
+    returned was written after the restore, and the run continues bitwise.
+    Files with a negative sweep count or a ladder outside [0, 1] are refused."""
+    import struct
+    from astroemperor_amd import PTSampler
+    from astroemperor_amd.simulation import test_init
+    m = make_sim("51Peg", 1, parameterisation=1).model
+    T, W = 3, 16
+    p0 = test_init(m, T, W, np.random.default_rng(8))
+    a = PTSampler(W, m.ndim__, m, ntemps=T, seed=5, record=False)
+    a.run_mcmc(p0, nsweeps=6, capacity=20)
+    a.checkpoint(tmp_path / "nohist.bin")
+    a.run_mcmc(None, nsweeps=5)
+    want = a.get_last_sample()
+    b = PTSampler(W, m.ndim__, m, ntemps=T, seed=5)
+    b.run_mcmc(p0, nsweeps=0, capacity=20)
+    b.restore(tmp_path / "nohist.bin")
+    assert b.get_chain().shape[1] == 0 and b.get_tsw().shape[0] == 0
+    b.run_mcmc(None, nsweeps=5)
+    got = b.get_last_sample()
+    assert np.array_equal(got.coords, want.coords) and np.array_equal(got.log_like, want.log_like)
+    ch, ll = b.get_chain(), b.get_log_like()
+    assert ch.shape[1] == 5 and np.isfinite(ch).all() and np.isfinite(ll).all()
+    assert np.array_equal(ch[:, -1], got.coords) and np.array_equal(b.get_betas()[-1], got.betas)
+    assert b.get_tsw().shape[0] == 5
+    assert np.isfinite(b.lnl_moments(1)[0]).all()
+    # corrupt copies: sweep count < 0, then a beta of 2
+    raw = bytearray((tmp_path / "nohist.bin").read_bytes())
+    off = 8 + 6 * 4
+    bad = bytearray(raw)
+    bad[off:off + 8] = struct.pack("<q", -3)
+    (tmp_path / "neg.bin").write_bytes(bytes(bad))
+    with pytest.raises(RuntimeError, match="sweep counters"):
+        b.restore(tmp_path / "neg.bin")
+    bad = bytearray(raw)
+    boff = off + 3 * 8 + (T * W * m.ndim__ + 2 * T * W) * 8
+    bad[boff:boff + 8] = struct.pack("<d", 2.0)
+    (tmp_path / "beta.bin").write_bytes(bytes(bad))
+    with pytest.raises(RuntimeError, match="betas"):
+        b.restore(tmp_path / "beta.bin")
+    a.close()
+    b.close()
+
+
 def test_graph_equals_direct():
     from astroemperor_amd import PTSampler
     from astroemperor_amd.simulation import test_init

@@ -147,6 +147,58 @@ def test_other_system_replays_oracle():
     assert np.allclose(s.get_log_like(sys_i)[-1], lnl, rtol=1e-10, atol=0)
 
 
+def _lpt_order(models):
+    """The queue order the library builds: longest first by N (51 k + 46 (+23
+    with an active MA)), stable."""
+    cost = [m.data.ndat * (m.kplan__ * 51.0 + 46.0 +
+                           (23.0 if m.block("MOAV") is not None and
+                            (m.semantics == "intended" or m.block("MOAV").is_global) else 0.0))
+            for m in models]
+    return sorted(range(len(models)), key=lambda i: -cost[i])
+
+
+@pytest.mark.parametrize("waves_per_wg", [8, 16])
+def test_queue_pops_many_systems_per_workgroup(waves_per_wg):
+    """More systems than persistent workgroups (one per compute unit, so each
+    workgroup pops about three systems per sweep, reusing its LDS tile and
+    slot memory): systems that sit past the first pop in the longest-first
+    queue replay the oracle sweep by sweep, and every system's stored lnL
+    matches the oracle."""
+    from astroemperor_amd import SystemsPT, device_info
+    n_cu = device_info()["n_cu"]
+    S = 3 * n_cu + 17
+    models = _models_small(S, seed=61)
+    T, W, seed, n = 2, 16, 0x0dd_ba11_5eed, 3
+    betas = np.array([1.0, 0.25])
+    p0 = _p0(models, T, W, 14)
+    s = SystemsPT(models, T, W, p0, betas=betas, seed=seed, adapt_mode=-1, precision="fp64",
+                  record_every=1, record_temps=T, max_records=n)
+    s.set_tuning(waves_per_wg=waves_per_wg, waves_per_theta=1, wgs_per_cu=1, order="lpt")
+    s.run(n, use_graph=False)
+    assert s.nsweeps == n
+    order = _lpt_order(models)
+    grid = min(S, n_cu)
+    for pos in (grid, 2 * grid + 3, S - 1):          # second, third and last pops
+        sys_i = order[pos]
+        chain = s.get_chain(sys_i)
+        o = oracle_for(models[sys_i])
+        X = np.array(p0[sys_i])
+        lnl = np.array([[o.my_likelihood(x) for x in X[t]] for t in range(T)])
+        lnp = np.array([[o.my_prior(x) for x in X[t]] for t in range(T)])
+        for sweep in range(n):
+            _replay_sweep_sys(o, X, lnl, lnp, betas, sweep, seed, sys_i)
+            assert np.allclose(chain[sweep], X, rtol=1e-12, atol=1e-12), (pos, sys_i, sweep)
+        assert np.allclose(s.get_log_like(sys_i)[-1], lnl, rtol=1e-10, atol=0), (pos, sys_i)
+    rng = np.random.default_rng(3)
+    for sys_i in rng.choice(S, 24, replace=False):
+        x, ll, _ = s.state(int(sys_i))
+        o = oracle_for(models[sys_i])
+        t, w = int(rng.integers(0, T)), int(rng.integers(0, W))
+        want = o.my_likelihood(x[t, w])
+        assert abs(ll[t, w] - want) <= 1e-10 * abs(want), (sys_i, ll[t, w], want)
+    s.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_lnl_matches_oracle(precision):
     from astroemperor_amd import SystemsPT

@@ -187,3 +187,21 @@ def test_lnl_spread():
     assert np.isclose(got[1], np.std(np.delete(L[1], 5), ddof=1), rtol=1e-14)
     assert got[2] == 0.0
     assert np.isclose(got[3], np.std(L[3], ddof=1), rtol=1e-14)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_oracle_acceleration_first_coefficient_on_highest_power(order):
+    """acc.model:2, np.polyval(concat(theta, [0]), X_ - X_[0]): the oracle's
+    model is a1 x^q + ... + aq x + gamma, the sum written out term by term."""
+    from tests.conftest import make_sim, oracle_for
+    m = make_sim("51Peg", 0, acceleration=order).model
+    o = oracle_for(m)
+    x = m.data.X - m.data.X[0]
+    rng = np.random.default_rng(order)
+    a = rng.uniform(-10, 10, order) / np.array([x[-1] ** (order - i) for i in range(order)])
+    mu, _ = o.my_model(np.concatenate([a, [0.5, 2.0]]))
+    want = 0.5 + sum(a[i] * x ** (order - i) for i in range(order))
+    assert np.max(np.abs(mu - want)) <= 1e-13 * np.max(np.abs(want))
+    if order > 1:
+        rev = 0.5 + sum(a[i] * x ** (i + 1) for i in range(order))
+        assert np.max(np.abs(mu - rev)) > 1.0

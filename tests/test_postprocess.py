@@ -104,3 +104,39 @@ def test_constrain_range_and_sigma():
     sim.constrain["method"] = "GM"
     with pytest.raises(NotImplementedError):
         sim._constrain_next_run()
+
+
+class _FakeEvidenceSampler:
+    """Records which estimator the evidence flow asked for."""
+
+    def __init__(self, ti_value=-10.0, pchip_value=-11.0, hybrid=None):
+        self.calls = []
+        self.ti_value, self.pchip_value, self.hybrid = ti_value, pchip_value, hybrid
+
+    def get_evidence_hybrid(self, discard=0):
+        self.calls.append(("hybrid", discard))
+        if self.hybrid is None:
+            raise NotImplementedError
+        return self.hybrid
+
+    def get_evidence_ti(self, discard=0, pchip=True):
+        self.calls.append(("ti", discard, pchip))
+        return (self.pchip_value, 0.5) if pchip else (self.ti_value, 0.25)
+
+
+def test_evidence_flow_follows_emperor():
+    """emp.py:1432-1447: the configured estimator first; -inf falls back to TI
+    with PCHIP, then classic TI; an exception goes straight to classic TI."""
+    from astroemperor_amd import Simulation
+    sim = Simulation()
+    assert sim.evidence_method == "hybrid"
+    sim.reddemcee_discard = 7
+    sim.sampler = _FakeEvidenceSampler()
+    assert sim._get_evidence() == (-10.0, 0.25)
+    assert sim.sampler.calls == [("hybrid", 7), ("ti", 7, False)]
+    sim.sampler = _FakeEvidenceSampler(hybrid=(-np.inf, 0.0))
+    assert sim._get_evidence() == (-11.0, 0.5)
+    assert sim.sampler.calls == [("hybrid", 7), ("ti", 7, True)]
+    sim.sampler = _FakeEvidenceSampler(hybrid=(-np.inf, 0.0), pchip_value=-np.inf)
+    assert sim._get_evidence() == (-10.0, 0.25)
+    assert sim.sampler.calls == [("hybrid", 7), ("ti", 7, True), ("ti", 7, False)]
