@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
 """PT sweeps/s and lnL evals/s of the device engine, FP32-mixed vs FP64.
 
-Workload (default): 3 Keplerians (ln P, K, M0, e, w), 4 instruments with
-80/150/400/900 epochs over 6000 d (1530 epochs), per-instrument offset,
-jitter and MA(1) (fixed 5 d timescale), 10 temperatures x 256 walkers,
-synthetic data from astroemperor_amd.synth.c2.  One step = one PT sweep
-(one red/blue stretch step at every temperature + one swap round + ladder
-adaptation + chain record), inputs resident in device memory.
+Headline workload (the largest single-GPU configuration): 7 Keplerians
+(ln P, K, M0, e, w) over 10000 epochs of 5 instruments (1000-3000 each),
+per-instrument offset, jitter and MA(1) with a fixed 5 d timescale, 50 free
+parameters, 16 temperatures x 1024 walkers, gradient (Langevin) sweeps mixed
+1:4 with stretch sweeps; synthetic data from astroemperor_amd.synth.c5.  One
+step = one PT sweep (one red/blue step of the sweep's move at every
+temperature + one swap round + ladder adaptation), inputs resident in device
+memory.  Before the timed sweeps the gradient move's step is adapted per
+temperature towards 50% acceptance and then held fixed.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--only LEG]
 
-Prints one JSON line.  `value` is the FP32-mixed path's PT sweeps/s.
+Prints one JSON line.  `value` is the FP32-mixed path's PT sweeps/s on the
+headline workload; the other configurations are objects in `configs`.
 """
 from __future__ import annotations
 
@@ -39,73 +43,240 @@ METRIC = "PT sweeps/s + lnL evals/s, FP32-mixed vs FP64 on 1 GPU at |ΔlnL|<1e-6
 OPS_KEP = {"fp64": 24 + 3 + 20 * 5, "mixed": 24 * 0.5 + 3 + 4 * 4 + 20 * 1}
 OPS_EPOCH = 6 + 20 * 2
 OPS_MA = 3 + 20 * 1
+# The adjoint (lnL and its gradient) of one vector: the value above once,
+# plus the derivative block, FP64 on both paths: per planet-epoch 24 ops
+# (the four natural-parameter derivatives of the planet term and their five
+# products with dlnL/dmu), per epoch 4 (offset and jitter terms), per MA
+# epoch 4 (the reverse recurrence and the coefficient term).  The kernel's
+# second evaluation of each planet term is not counted.
+OPS_DKEP = 24
+OPS_DEPOCH = 4
+OPS_DMA = 4
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
-# HBM bytes per launch of k_pt_half<mixed, W waves per walker> on the default
-# workload: rocprofv3 2.00 x FETCH_SIZE + 1.00 x WRITE_SIZE in KiB per
-# dispatch, the factors from the same counters on a 256 MiB copy (the summary
-# is profiles/k_pt_half_k_sys_sweep_hbm_traffic.md).
-TRAFFIC_KIB = {1: (507.3, 94.7), 2: (511.6, 94.7), 4: (527.1, 95.6), 8: (591.3, 107.5)}
+# HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per
+# dispatch) on the headline workload, each scaled by what the same counter
+# reads for a plain 256 MiB device copy (FETCH 2.00, WRITE 1.00).  Keys:
+# (kernel, precision).  Filled from the committed counter summaries.
+TRAFFIC_KIB = {}
 
 
-def traffic_bytes(wpt):
-    f, w = TRAFFIC_KIB[wpt]
+def traffic_bytes(kernel, prec):
+    v = TRAFFIC_KIB.get((kernel, prec))
+    if v is None:
+        return None
+    f, w = v
     return int((2.0 * f + 1.0 * w) * 1024)
 
 
+def _ma_epochs(model):
+    ma = model.block("MOAV")
+    return model.data.ndat if (ma is not None and (model.semantics == "intended" or ma.is_global)) else 0
+
+
 def flops_per_theta(model, precision):
+    """FP64-equivalent flops of one lnL evaluation (2 per op)."""
     d = model.data
-    kep = model.kplan__ * d.ndat
-    ma = d.ndat if (model.block("MOAV") is not None and
-                    (model.semantics == "intended" or model.block("MOAV").is_global)) else 0
-    ops = kep * OPS_KEP[precision] + d.ndat * OPS_EPOCH + ma * OPS_MA
+    ops = model.kplan__ * d.ndat * OPS_KEP[precision] + d.ndat * OPS_EPOCH + _ma_epochs(model) * OPS_MA
     return 2.0 * ops
 
 
-def run_gpu(work, precision, steps, warmup, seed, half_launches):
-    half_launches += half_launches & 1          # whole sweeps: two likelihood launches each
-    from astroemperor_amd import PTSampler
+def flops_grad_per_theta(model, precision):
+    """FP64-equivalent flops of one lnL + gradient evaluation (2 per op)."""
+    d = model.data
+    ops = (model.kplan__ * d.ndat * (OPS_KEP[precision] + OPS_DKEP) + d.ndat * (OPS_EPOCH + OPS_DEPOCH)
+           + _ma_epochs(model) * (OPS_MA + OPS_DMA))
+    return 2.0 * ops
+
+
+def roofline(kernel, prec, flops, ms, extra=None):
+    ach = flops / (ms * 1e-3) / 1e12
+    out = {"bound": "valu", "kernel": f"{kernel}<{prec}>", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
+           "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": traffic_bytes(kernel, prec),
+           "flops_per_launch": flops, "launch_ms": round(ms, 5)}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def _oracle_dlnl(model, coords, lnl_stored, n, rng):
+    """|lnL - lnL_oracle| at n sampled positions: the engine's stored values
+    and fresh FP64 / mixed evaluations, against the FP64 oracle."""
+    from oracle.emp_oracle import OracleModel
+    d = model.data
+    o = OracleModel(model.spec(), d.X, d.Y, d.YERR, d.FLAG, SAI=d.SA)
+    th = coords.reshape(-1, coords.shape[-1])
+    st = lnl_stored.ravel()
+    pick = rng.choice(len(th), min(n, len(th)), replace=False)
+    ref = np.array([o.my_likelihood(th[i]) for i in pick])
+    f64 = model.lnl_batch(th[pick], "fp64")
+    mix = model.lnl_batch(th[pick], "mixed")
+    fin = np.isfinite(ref)
+    return {"n": int(fin.sum()),
+            "stored_max_abs": float(np.max(np.abs(st[pick][fin] - ref[fin]))),
+            "mixed_max_abs": float(np.max(np.abs(mix[fin] - ref[fin]))),
+            "fp64_max_rel": float(np.max(np.abs(f64[fin] - ref[fin]) / np.abs(ref[fin]))),
+            "bar": "mixed < 1e-6 absolute, FP64 < 1e-10 relative"}
+
+
+def _state_dlnl(model, coords):
+    """|lnL_mixed - lnL_fp64| over a set of positions [..., D]."""
+    th = coords.reshape(-1, coords.shape[-1])
+    a = model.lnl_batch(th, "fp64")
+    b = model.lnl_batch(th, "mixed")
+    fin = np.isfinite(a) & np.isfinite(b)
+    d = np.abs(a[fin] - b[fin])
+    return {"n": int(fin.sum()), "max": float(d.max()) if d.size else None,
+            "p99": float(np.percentile(d, 99)) if d.size else None,
+            "mismatched_finiteness": int((np.isfinite(a) != np.isfinite(b)).sum())}
+
+
+# ---------------------------------------------------------------------------
+# Headline: the large mixed workload.
+# ---------------------------------------------------------------------------
+def run_c5(seed, steps, warmup, adapt_sweeps=16, precisions=("mixed", "fp64"), oracle_n=16):
+    from astroemperor_amd import GradientMove, PTSampler, StretchMove, synth
     from astroemperor_amd.simulation import test_init
+    work = synth.c5()
+    m = work["model"]
+    T, W, _, _ = work["setup"]
+    p0 = test_init(m, T, W, np.random.default_rng(seed))
+    mix = [(StretchMove(), 4.0), (GradientMove(), 1.0)]
+    out = {"workload": f"7 Keplerians, 5 instruments, {m.data.ndat} epochs, per-instrument MA(1), {T} temps x "
+                       f"{W} walkers, gradient (Langevin) sweeps mixed 1:4 with stretch sweeps",
+           "ndim": int(m.ndim__), "ntemps": T, "nwalkers": W, "adapt_sweeps": adapt_sweeps,
+           "default_gradient_step": GradientMove().scale(m.ndim__)}
+    flops_l = {p: flops_per_theta(m, p) for p in ("fp64", "mixed")}
+    flops_g = {p: flops_grad_per_theta(m, p) for p in ("fp64", "mixed")}
+    states = {}
+    for prec in precisions:
+        extra = 2 * 6 + 8               # per-move timings and the timed kernel launches
+        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec, record=False, moves=mix,
+                      adapt_tau=1000, adapt_nu=1)
+        s.run_mcmc(p0, nsweeps=1, capacity=1 + adapt_sweeps + warmup + steps + extra)
+        if warmup:
+            s.run_mcmc(None, nsweeps=warmup)
+        # the step is adapted right before the timed sweeps: the ensemble
+        # (the move's preconditioner) contracts during burn-in
+        steps_t = s.adapt_gradient_steps(nsweeps=adapt_sweeps, target=0.5, batch=1)
+        wpt, _ = s.autotune(3)
+        n0 = s.n_lnl_evaluated
+        t0 = time.perf_counter()
+        s.run_mcmc(None, nsweeps=steps)
+        wall = time.perf_counter() - t0
+        n1 = s.n_lnl_evaluated
+        # kernel timings inside ordinary sweeps, events on the engine stream
+        nh = 6
+        n2 = s.n_lnl_evaluated
+        ms_half = s.time_halfstep(nh)
+        ev_half = (s.n_lnl_evaluated - n2) / nh
+        ms_grad = s.time_gradstep(4)
+        kinds = {}
+        for name, mv in (("stretch", [(StretchMove(), 1.0)]), ("gradient", [(GradientMove(steps_t), 1.0)])):
+            s.set_moves(mv)
+            s.run_mcmc(None, nsweeps=1)            # captures this kind's graph
+            a1 = s.accepted.sum()
+            s.run_mcmc(None, nsweeps=5)
+            kinds[name] = {"ms_per_sweep": round(s.last_timing()[0], 3),
+                           "acceptance": round(float(s.accepted.sum() - a1) / (5 * T * W), 4)}
+        st = s.get_last_sample()
+        states[prec] = st
+        props = T * W // 2
+        r_half = roofline("k_pt_half", prec, flops_l[prec] * ev_half, ms_half,
+                          {"lnl_evaluated_per_launch": round(ev_half, 1), "proposals_per_launch": props})
+        r_grad = roofline("k_lnl_grad", prec, flops_g[prec] * props, ms_grad, {"vectors_per_launch": props})
+        # share of the mix's time: 4 stretch sweeps (2 half-step launches each) per gradient sweep (2 proposal
+        # adjoints plus the one over the slots a stretch move moved)
+        t_mix = 4 * kinds["stretch"]["ms_per_sweep"] + kinds["gradient"]["ms_per_sweep"]
+        share_half = 4 * 2 * ms_half / t_mix
+        share_grad = 2 * ms_grad / t_mix
+        out[prec] = {"sweeps_per_s": round(steps / wall, 3), "ms_per_sweep": round(1e3 * wall / steps, 4),
+                     "lnl_evals_per_s": round(steps * T * W / wall, 1),
+                     "lnl_evaluated_per_s": round((n1 - n0) / wall, 1), "waves_per_walker": wpt,
+                     "gradient_steps": [float(v) for v in steps_t], "by_move": kinds,
+                     "kernel_time_share": {"k_pt_half": round(share_half, 3), "k_lnl_grad": round(share_grad, 3)},
+                     "roofline_k_pt_half": r_half, "roofline_k_lnl_grad": r_grad}
+        print(f"[bench] c5 {prec}: {steps / wall:.2f} sweeps/s, half-step {ms_half:.3f} ms "
+              f"(frac {r_half['frac']}), adjoint {ms_grad:.3f} ms (frac {r_grad['frac']}), {kinds}",
+              file=sys.stderr, flush=True)
+        s.close()
+    if "mixed" in out and "fp64" in out:
+        out["mixed_over_fp64"] = round(out["mixed"]["sweeps_per_s"] / out["fp64"]["sweeps_per_s"], 3)
+    st = states.get("mixed") or next(iter(states.values()))
+    out["dlnl_mixed_vs_fp64"] = _state_dlnl(m, st.coords)
+    out["dlnl_vs_oracle"] = _oracle_dlnl(m, st.coords, st.log_like, oracle_n, np.random.default_rng(seed))
+    return out, m
+
+
+def run_cpu_c5(model, seed, sweeps, cores, temps=2, walkers=128):
+    """The reconstructed reference CPU path (per-walker numpy likelihood,
+    the oracle's C Kepler solver, pool.map per half-step per temperature) on
+    a bounded sample of the headline workload: `sweeps` stretch sweeps of
+    `temps` temperatures x `walkers` walkers.  Its cost is linear in the
+    walkers moved (each map holds walkers / 2 likelihoods of ~60 ms), so the
+    headline's rate is the sample's times temps x walkers / (16 x 1024)."""
+    from astroemperor_amd.sampler import default_ladder
+    from astroemperor_amd.simulation import test_init
+    from oracle import cpu_reference
+    d = model.data
+    p0 = test_init(model, temps, walkers, np.random.default_rng(seed))
+    r = cpu_reference.run(model.spec(), d.X, d.Y, d.YERR, d.FLAG, p0, default_ladder(temps, model.ndim__), sweeps,
+                          ncores=cores, seed=seed)
+    r.pop("final")
+    scale = temps * walkers / (16.0 * 1024.0)
+    return {"value": round(r["sweeps_per_s"] * scale, 6), "unit": "PT sweeps/s", "cores": r["cores"],
+            "kind": "port",
+            "sample": f"{sweeps} stretch sweeps of {temps} temps x {walkers} walkers of the same model and data "
+                      f"({sweeps * temps * walkers} likelihoods); the headline's 16 x 1024 ensemble costs "
+                      f"{1 / scale:.0f}x a sample sweep",
+            "sample_sweeps_per_s": round(r["sweeps_per_s"], 4), "lnl_evals_per_s": round(r["lnl_evals_per_s"], 1),
+            "seconds": round(r["seconds"], 2), "os_cpu_count": r["os_cpu_count"], "cpu_model": r["cpu_model"],
+            "label": "reconstructed reference CPU path (numpy + oracle C Kepler + multiprocessing)"}
+
+
+# ---------------------------------------------------------------------------
+# Other configurations.
+# ---------------------------------------------------------------------------
+def run_c2(seed, steps, warmup, half_launches=40):
+    """3 Keplerians, 4 instruments (80/150/400/900 epochs), MA(1),
+    10 temps x 256 walkers, stretch sweeps."""
+    from astroemperor_amd import PTSampler, synth
+    from astroemperor_amd.simulation import test_init
+    work = synth.c2()
     m = work["model"]
     T, W, _, nsteps = work["setup"]
     p0 = test_init(m, T, W, np.random.default_rng(seed))
-    s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=precision, adapt_tau=1000, adapt_nu=1)
-    cap = warmup + steps + (half_launches + 1) // 2 + 4
-    s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=cap)
-    wpt, _ = s.autotune(10)
-    n0 = s.n_lnl_evaluated
-    t0 = time.perf_counter()
-    s.run_mcmc(None, nsweeps=steps)
-    t1 = time.perf_counter()
-    n1 = s.n_lnl_evaluated
-    ms_sweep_events, _ = s.last_timing()
-    wall = t1 - t0
-    ms_half = s.time_halfstep(half_launches)
-    n2 = s.n_lnl_evaluated
-    return dict(sampler=s, seconds=wall, sweeps_per_s=steps / wall, ms_per_sweep=1e3 * wall / steps,
-                ms_per_sweep_events=ms_sweep_events,
-                proposals_per_s=steps * T * W * nsteps / wall,
-                lnl_evaluated_per_s=(n1 - n0) / wall,
-                accept_in_support=(n1 - n0) / (steps * T * W),
-                ms_halfstep=ms_half, evals_per_halfstep=(n2 - n1) / half_launches,
-                T=T, W=W, waves_per_walker=wpt)
-
-
-def dlnl_report(work, samplers):
-    """|lnL_mixed - lnL_fp64| on positions the samplers visited."""
-    m = work["model"]
-    thetas = []
-    for s in samplers:
-        ch = s.get_chain()                              # [T, S, W, D]
-        S = ch.shape[1]
-        pick = np.linspace(0, S - 1, min(S, 16)).astype(int)
-        thetas.append(ch[:, pick].reshape(-1, ch.shape[-1]))
-    th = np.concatenate(thetas)
-    a = m.lnl_batch(th, "fp64")
-    b = m.lnl_batch(th, "mixed")
-    fin = np.isfinite(a) & np.isfinite(b)
-    d = np.abs(a[fin] - b[fin])
-    return dict(n=int(fin.sum()), max=float(d.max()), p99=float(np.percentile(d, 99)),
-                mismatched_finiteness=int((np.isfinite(a) != np.isfinite(b)).sum()))
+    out = {"workload": "3 Keplerians, 4 instruments (80/150/400/900 epochs), MA(1), "
+                       f"{T} temps x {W} walkers, 1 stretch step per sweep",
+           "epochs": int(m.data.ndat), "ndim": int(m.ndim__)}
+    last = None
+    for prec in ("mixed", "fp64"):
+        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec, adapt_tau=1000, adapt_nu=1)
+        s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=warmup + steps + half_launches // 2 + 4)
+        wpt, _ = s.autotune(10)
+        n0 = s.n_lnl_evaluated
+        t0 = time.perf_counter()
+        s.run_mcmc(None, nsweeps=steps)
+        wall = time.perf_counter() - t0
+        n1 = s.n_lnl_evaluated
+        ms_half = s.time_halfstep(half_launches)
+        ev = (s.n_lnl_evaluated - n1) / half_launches
+        out[prec] = {"sweeps_per_s": round(steps / wall, 2), "ms_per_sweep": round(1e3 * wall / steps, 5),
+                     "proposals_per_s": round(steps * T * W * nsteps / wall, 1),
+                     "lnl_evaluated_per_s": round((n1 - n0) / wall, 1),
+                     "in_support_fraction": round((n1 - n0) / (steps * T * W * nsteps), 4),
+                     "waves_per_walker": wpt,
+                     "roofline": roofline("k_pt_half", prec, flops_per_theta(m, prec) * ev, ms_half,
+                                          {"lnl_evaluated_per_launch": round(ev, 1),
+                                           "proposals_per_launch": T * W // 2})}
+        if prec == "mixed":
+            last = s.get_last_sample().coords
+        s.close()
+        print(f"[bench] c2 {prec}: {out[prec]['sweeps_per_s']} sweeps/s", file=sys.stderr, flush=True)
+    out["mixed_over_fp64"] = round(out["mixed"]["sweeps_per_s"] / out["fp64"]["sweeps_per_s"], 3)
+    out["dlnl_mixed_vs_fp64"] = _state_dlnl(m, last)
+    out["dlnl_vs_oracle"] = _oracle_dlnl(m, last, m.lnl_batch(last.reshape(-1, last.shape[-1]), "mixed"), 32,
+                                         np.random.default_rng(seed))
+    return out
 
 
 def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), trial_sweeps=2):
@@ -113,6 +284,7 @@ def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), tr
     ensemble advanced by one persistent kernel launch per sweep, replayed
     from a hipGraph, launch shape from the autotuner."""
     from astroemperor_amd import SystemsPT, synth
+    from oracle.emp_oracle import OracleModel
     t0 = time.perf_counter()
     work = synth.c3(n_systems)
     models = work["models"]
@@ -122,13 +294,13 @@ def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), tr
     build_s = time.perf_counter() - t0
     N = np.array([m.data.ndat for m in models])
     K = np.array([m.kplan__ for m in models])
-    flops_sys = np.array([flops_per_theta(m, "mixed") for m in models])
     out = {"workload": f"{len(models)} independent systems, epochs lognormal(median 200) clipped to "
                        "[50, 2000], 1-4 Keplerians, 1-3 instruments, each 8 temps x 64 walkers",
            "n_systems": len(models), "epochs_total": int(N.sum()), "epochs_max": int(N.max()),
            "planet_epochs_total": int((N * K).sum()), "ntemps": T, "nwalkers": W, "host_build_s": round(build_s, 1)}
     states = {}
     for prec in precisions:
+        flops_sys = np.array([flops_per_theta(m, prec) for m in models])
         t1 = time.perf_counter()
         s = SystemsPT(models, T, W, p0, seed=seed, precision=prec, adapt_tau=1000, adapt_nu=1)
         create_s = time.perf_counter() - t1
@@ -144,27 +316,25 @@ def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), tr
         ev_ms = s.last_ms_per_sweep()
         props = steps * T * W * nsteps * len(models)
         frac_eval = (n1 - n0) / props
-        # algorithmic FP64-equivalent flops of one sweep: every evaluated lnL
-        # at its system's cost (evaluation fraction taken as uniform)
+        # algorithmic flops of one sweep: every evaluated lnL at its system's
+        # cost (the evaluated fraction taken as uniform over systems)
         flops = float(flops_sys.sum()) * T * W * nsteps * frac_eval
-        achieved = flops / (ev_ms * 1e-3) / 1e12
         out[prec] = {"sweeps_per_s": round(steps / wall, 3), "ms_per_sweep": round(1e3 * wall / steps, 4),
                      "ms_per_sweep_events": round(ev_ms, 4),
                      "lnl_evals_per_s": round(props / wall, 1), "lnl_evaluated_per_s": round((n1 - n0) / wall, 1),
-                     "tuning": tune, "create_s": round(create_s, 2),
-                     "roofline": {"bound": "valu", "kernel": f"k_sys_sweep<{prec}>", "achieved": round(achieved, 3),
-                                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                                  "frac": round(achieved / PEAK_FP64_TFLOPS, 4)}}
+                     "in_support_fraction": round(frac_eval, 4), "tuning": tune, "create_s": round(create_s, 2),
+                     "roofline": roofline("k_sys_sweep", prec, flops, ev_ms)}
         print(f"[bench] systems {prec}: {steps / wall:.2f} sweeps/s ({props / wall / 1e6:.1f} M lnL/s), "
               f"tuning {tune}", file=sys.stderr, flush=True)
         states[prec] = s
     if "mixed" in states and "fp64" in states:
         out["mixed_over_fp64"] = round(out["mixed"]["sweeps_per_s"] / out["fp64"]["sweeps_per_s"], 3)
-        # |dlnL| on the states the mixed run visited, a spread of systems
+        # |dlnL| on the states the mixed run visited, a spread of systems;
+        # the oracle on one walker of six of them
         s = states["mixed"]
-        dmax, n = 0.0, 0
-        for i in np.linspace(0, len(models) - 1, 24).astype(int):
-            x, _, _ = s.state(int(i))
+        dmax, n, omax, orel = 0.0, 0, 0.0, 0.0
+        for c, i in enumerate(np.linspace(0, len(models) - 1, 24).astype(int)):
+            x, ll, _ = s.state(int(i))
             th = x.reshape(-1, x.shape[-1])
             a = models[i].lnl_batch(th, "fp64")
             b = models[i].lnl_batch(th, "mixed")
@@ -172,22 +342,19 @@ def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), tr
             if fin.any():
                 dmax = max(dmax, float(np.abs(a[fin] - b[fin]).max()))
                 n += int(fin.sum())
+            if c % 4 == 0:
+                d = models[i].data
+                o = OracleModel(models[i].spec(), d.X, d.Y, d.YERR, d.FLAG)
+                ref = o.my_likelihood(x[0, 0])
+                if np.isfinite(ref):
+                    omax = max(omax, abs(ll[0, 0] - ref))
+                    orel = max(orel, abs(a[0] - ref) / abs(ref))
             models[i]._device = None
         out["dlnl_mixed_vs_fp64"] = {"n": n, "max": dmax}
+        out["dlnl_vs_oracle"] = {"n": 6, "stored_mixed_max_abs": omax, "fp64_max_rel": orel}
     for s in states.values():
         s.close()
     return out
-
-
-def _state_dlnl(model, coords):
-    """|lnL_mixed - lnL_fp64| over a set of positions [..., D]."""
-    th = coords.reshape(-1, coords.shape[-1])
-    a = model.lnl_batch(th, "fp64")
-    b = model.lnl_batch(th, "mixed")
-    fin = np.isfinite(a) & np.isfinite(b)
-    d = np.abs(a[fin] - b[fin])
-    return {"n": int(fin.sum()), "max": float(d.max()) if d.size else None,
-            "p99": float(np.percentile(d, 99)) if d.size else None}
 
 
 def run_c1(seed, cpu_cores, with_cpu):
@@ -209,9 +376,11 @@ def run_c1(seed, cpu_cores, with_cpu):
         wall = time.perf_counter() - t0
         out[prec] = {"sweeps_per_s": round((S - 1) / wall, 1), "lnl_evals_per_s": round((S - 1) * T * W / wall, 1),
                      "seconds": round(wall, 3)}
-        last = s.get_last_sample().coords
+        st = s.get_last_sample()
+        last = st
         s.close()
-    out["dlnl_mixed_vs_fp64"] = _state_dlnl(m, last)
+    out["dlnl_mixed_vs_fp64"] = _state_dlnl(m, last.coords)
+    out["dlnl_vs_oracle"] = _oracle_dlnl(m, last.coords, last.log_like, 64, np.random.default_rng(seed))
     if with_cpu:
         from oracle import cpu_reference
         from astroemperor_amd.sampler import default_ladder
@@ -219,16 +388,16 @@ def run_c1(seed, cpu_cores, with_cpu):
         r = cpu_reference.run(m.spec(), d.X, d.Y, d.YERR, d.FLAG, p0, default_ladder(T, m.ndim__), S,
                               ncores=cpu_cores, seed=seed)
         out["cpu"] = {"sweeps_per_s": round(r["sweeps_per_s"], 2), "lnl_evals_per_s": round(r["lnl_evals_per_s"], 1),
-                      "cores": r["cores"], "seconds": round(r["seconds"], 2), "kind": "port",
-                      "sample": f"the full run ({S} sweeps)"}
+                      "cores": r["cores"], "os_cpu_count": r["os_cpu_count"], "seconds": round(r["seconds"], 2),
+                      "kind": "port", "sample": f"the full run ({S} sweeps)"}
         out["fp64_over_cpu"] = round(out["fp64"]["sweeps_per_s"] / r["sweeps_per_s"], 1)
     return out
 
 
-def run_c4(seed, batch=65536, reps=5, map_starts=1024):
+def run_c4(seed, batch=65536, reps=5, map_starts=1024, fd_vectors=64):
     """Adjoint gradients: 5 Keplerians, N = 2000, 2 instruments, a batch of
     parameter vectors drawn from the prior; throughput of lnL + gradient,
-    a central finite-difference check and the batched L-BFGS MAP search."""
+    the finite-difference check and the batched L-BFGS MAP search."""
     import torch
     from astroemperor_amd import synth
     from astroemperor_amd.autograd import map_search
@@ -237,126 +406,103 @@ def run_c4(seed, batch=65536, reps=5, map_starts=1024):
     rng = np.random.default_rng(seed)
     th = synth.initial_positions(m, 1, batch, rng)[0]
     tt = torch.tensor(th, device="cuda")
-    m.lnl_grad_batch(tt)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        m.lnl_grad_batch(tt)
-    torch.cuda.synchronize()
-    tg = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        m.lnl_batch(tt, "fp64")
-    torch.cuda.synchronize()
-    tl = (time.perf_counter() - t0) / reps
-    # FD check (FP64, h = 1e-6 max(|theta|, 1)) on 8 vectors.  The central
-    # difference carries a rounding error ~ 2^-52 |lnL| / h, large for draws
-    # far from the data; entries are judged where that error is below 1e-7 |fd|.
-    # Richardson's extrapolation of the h and h/2 differences removes their
-    # O(h^2) truncation error, which h = 1e-6 |theta| leaves near 1e-5 on the
-    # periods (the phase derivative grows as t / P^2).
-    sub = th[:8]
-    f0, g = m.lnl_grad_batch(sub)
-    errs, errs_r, noisy = [], [], 0
-
-    def cdiff(i, j, h):
-        xp, xm = sub[i].copy(), sub[i].copy()
-        xp[j] += h
-        xm[j] -= h
-        return (m.lnl_batch(xp[None])[0] - m.lnl_batch(xm[None])[0]) / (2 * h)
-
-    for i in range(len(sub)):
-        for j in range(sub.shape[1]):
-            h = 1e-6 * max(abs(sub[i, j]), 1.0)
-            fd = cdiff(i, j, h)
-            if 2.0 ** -52 * abs(f0[i]) / h > 1e-7 * abs(fd):
-                noisy += 1
-                continue
-            fr = (4.0 * cdiff(i, j, 0.5 * h) - fd) / 3.0
-            errs.append(abs(g[i, j] - fd) / abs(fd))
-            errs_r.append(abs(g[i, j] - fr) / abs(fr))
-    t0 = time.perf_counter()
-    _, fbest = map_search(m, th[:map_starts], iters=50)
-    tmap = time.perf_counter() - t0
-    return {"workload": f"5 Keplerians, 2 instruments, {m.data.ndat} epochs, batch {batch} from the prior",
-            "lnl_grad_evals_per_s": round(batch / tg, 1), "lnl_evals_per_s": round(batch / tl, 1),
-            "grad_over_lnl_time": round(tg / tl, 2),
-            "fd_check": {"entries": len(errs), "skipped_fd_rounding": noisy, "max_rel": float(np.max(errs)),
-                         "median_rel": float(np.median(errs)), "richardson_max_rel": float(np.max(errs_r)),
-                         "richardson_median_rel": float(np.median(errs_r))},
-            "map_search": {"starts": map_starts, "iters": 50, "seconds": round(tmap, 2),
-                           "best_lnpost": float(np.max(fbest))}}
-
-
-def run_c5(seed, steps, warmup):
-    """Large-scale mixed: 7 Keplerians, N = 10000 over 5 instruments with
-    MA(1), 16 temps x 1024 walkers, gradient sweeps mixed 1:4 with stretch
-    sweeps; a shortened timed run.  Each sweep kind is also timed alone."""
-    from astroemperor_amd import GradientMove, PTSampler, StretchMove, synth
-    from astroemperor_amd.simulation import test_init
-    work = synth.c5()
-    m = work["model"]
-    T, W, _, _ = work["setup"]
-    p0 = test_init(m, T, W, np.random.default_rng(seed))
-    mix = [(StretchMove(), 4.0), (GradientMove(), 1.0)]
-    out = {"workload": f"7 Keplerians, 5 instruments, {m.data.ndat} epochs, MA(1), {T} temps x {W} walkers, "
-                       f"{steps} timed sweeps, gradient (Langevin) sweeps mixed 1:4 with stretch sweeps",
-           "gradient_step": GradientMove().scale(m.ndim__)}
-    last = None
-    for prec in ("mixed", "fp64"):
-        extra = 2 * 6 + 3          # the per-move timings and the timed half-steps
-        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=seed, precision=prec, record=False, moves=mix)
-        s.run_mcmc(p0, nsweeps=max(warmup, 1), capacity=warmup + steps + extra + 1)
-        wpt, _ = s.autotune(3)
-        n0 = s.n_lnl_evaluated
-        t0 = time.perf_counter()
-        s.run_mcmc(None, nsweeps=steps)
-        wall = time.perf_counter() - t0
-        n1 = s.n_lnl_evaluated
-        h = s.time_halfstep(6)
-        n2 = s.n_lnl_evaluated
-        flops = flops_per_theta(m, prec) * (n2 - n1) / 6
-        ach = flops / (h * 1e-3) / 1e12
-        kinds = {}
-        for name, mv in (("stretch", [(StretchMove(), 1.0)]), ("gradient", [(GradientMove(), 1.0)])):
-            s.set_moves(mv)
-            s.run_mcmc(None, nsweeps=1)            # captures this kind's graph
-            a1 = s._get(6, np.int64).sum()
-            s.run_mcmc(None, nsweeps=5)
-            kinds[name] = {"ms_per_sweep": round(s.last_timing()[0], 3),
-                           "acceptance": round(float(s._get(6, np.int64).sum() - a1) / (5 * T * W), 4)}
-        s.set_moves(mix)
-        out[prec] = {"sweeps_per_s": round(steps / wall, 2), "lnl_evals_per_s": round(steps * T * W / wall, 1),
-                     "lnl_evaluated_per_s": round((n1 - n0) / wall, 1), "waves_per_walker": wpt,
-                     "by_move": kinds,
-                     "roofline": {"bound": "valu", "kernel": f"k_pt_half<{prec}>", "achieved": round(ach, 3),
-                                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4)}}
-        if prec == "mixed":
-            last = s.get_last_sample().coords
-        s.close()
-    out["mixed_over_fp64"] = round(out["mixed"]["sweeps_per_s"] / out["fp64"]["sweeps_per_s"], 3)
-    out["dlnl_mixed_vs_fp64"] = _state_dlnl(m, last)
-    return out
-
-
-def run_lnl(n, reps, seed):
-    """Batched lnL alone (emp_lnl_batch, device pointers) on n prior draws of
-    the default workload's model, reps times per precision: the subject of
-    the instruction-mix counter runs."""
-    import torch
-    from astroemperor_amd import synth
-    m = synth.c2()["model"]
-    th = torch.tensor(synth.initial_positions(m, 1, n, np.random.default_rng(seed))[0], device="cuda")
-    out = {}
-    for prec in ("mixed", "fp64"):
-        m.lnl_batch(th, prec)
+    out = {"workload": f"5 Keplerians, 2 instruments, {m.data.ndat} epochs, batch {batch} from the prior"}
+    for prec in ("fp64", "mixed"):
+        m.lnl_grad_batch(tt, prec)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            m.lnl_batch(th, prec)
+            m.lnl_grad_batch(tt, prec)
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
-        out[prec] = {"batch": n, "ms_per_batch": round(1e3 * dt, 4), "lnl_evals_per_s": round(n / dt, 1)}
+        tg = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            m.lnl_batch(tt, prec)
+        torch.cuda.synchronize()
+        tl = (time.perf_counter() - t0) / reps
+        out[prec] = {"lnl_grad_evals_per_s": round(batch / tg, 1), "lnl_evals_per_s": round(batch / tl, 1),
+                     "grad_over_lnl_time": round(tg / tl, 2),
+                     "roofline_k_lnl_grad": roofline("k_lnl_grad", prec, flops_grad_per_theta(m, prec) * batch,
+                                                     1e3 * tg, {"vectors_per_launch": batch,
+                                                                "timing": "wall clock of the launch, synchronised"})}
+    out["fd_check"] = fd_check(m, th[:fd_vectors])
+    t0 = time.perf_counter()
+    _, fbest = map_search(m, th[:map_starts], iters=50)
+    out["map_search"] = {"starts": map_starts, "iters": 50, "seconds": round(time.perf_counter() - t0, 2),
+                         "best_lnpost": float(np.max(fbest))}
+    return out
+
+
+def fd_check(m, sub, h_rel=1e-6):
+    """FP64 adjoint gradient against finite differences of the FP64 lnL, on
+    every entry of `sub`.  The reference difference is Richardson's
+    extrapolation of the central differences at h and h/2,
+    h = 1e-6 max(|theta_j|, 1) (its O(h^4) truncation is far below the plain
+    difference's O(h^2), ~1e-5 on the periods of long-baseline data).  An
+    entry is judged where the differences' rounding error, 4 x 2^-52 |lnL| / h,
+    is at most 1e-7 max(|fd|, 1); the others are counted as skipped.  Pass:
+    relative error <= 1e-6 of max(|fd|, 1)."""
+    f0, g = m.lnl_grad_batch(sub)
+    B, D = sub.shape
+    rows = []
+    hs = h_rel * np.maximum(np.abs(sub), 1.0)
+    for i in range(B):
+        for j in range(D):
+            for s_ in (hs[i, j], -hs[i, j], hs[i, j] / 2, -hs[i, j] / 2):
+                t = sub[i].copy()
+                t[j] += s_
+                rows.append(t)
+    v = m.lnl_batch(np.array(rows)).reshape(B, D, 4)
+    plain = (v[..., 0] - v[..., 1]) / (2 * hs)
+    rich = (4 * (v[..., 2] - v[..., 3]) / hs - plain) / 3
+    scale = np.maximum(np.abs(rich), 1.0)
+    judged = 4 * 2.0 ** -52 * np.abs(f0)[:, None] / hs <= 1e-7 * scale
+    err = np.abs(g - rich) / scale
+    names = m.names_free()
+    per = {}
+    for j in range(D):
+        sel = judged[:, j]
+        per[names[j]] = {"judged": int(sel.sum()), "max_rel": float(err[sel, j].max()) if sel.any() else None,
+                         "pass": bool(sel.any() and err[sel, j].max() <= 1e-6)}
+    plain_err = np.abs(g - plain) / np.maximum(np.abs(plain), 1.0)
+    return {"vectors": B, "entries": int(judged.sum()), "skipped_fd_rounding": int((~judged).sum()),
+            "reference": "Richardson-extrapolated central differences (h, h/2), h = 1e-6 max(|theta|, 1)",
+            "max_rel": float(err[judged].max()), "median_rel": float(np.median(err[judged])),
+            "plain_central_max_rel": float(plain_err[judged].max()),
+            "all_pass": bool(all(p["pass"] for p in per.values() if p["judged"])), "per_parameter": per}
+
+
+def run_grad(n, reps, seed, wpts=(0,), precisions=("mixed", "fp64")):
+    """The adjoint alone (emp_lnl_grad_batch, device pointers) on n prior
+    draws of the headline model, against the batched lnL on the same draws:
+    the subject of the adjoint's counter runs."""
+    import torch
+    from astroemperor_amd import synth
+    from astroemperor_amd.simulation import test_init
+    m = synth.c5()["model"]
+    th = torch.tensor(test_init(m, 1, n, np.random.default_rng(seed))[0], device="cuda")
+    out = {}
+    for w in wpts:
+        m.set_waves_per_theta(w)
+        for prec in precisions:
+            m.lnl_grad_batch(th, prec)
+            m.lnl_batch(th, prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                m.lnl_grad_batch(th, prec)
+            torch.cuda.synchronize()
+            tg = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                m.lnl_batch(th, prec)
+            torch.cuda.synchronize()
+            tl = (time.perf_counter() - t0) / reps
+            out[f"{prec}_wpt{w}"] = {"batch": n, "ms_grad": round(1e3 * tg, 4), "ms_lnl": round(1e3 * tl, 4),
+                                    "grad_over_lnl": round(tg / tl, 3),
+                                    "frac_grad": round(flops_grad_per_theta(m, prec) * n / tg / 1e12
+                                                       / PEAK_FP64_TFLOPS, 4)}
+    m.set_waves_per_theta(0)
     return out
 
 
@@ -373,39 +519,25 @@ def run_copy(mib, reps):
     return {"copy_bytes_each_way": n * 8, "reps": reps}
 
 
-def run_cpu(work, sweeps, cores, seed):
-    from oracle import cpu_reference
-    from astroemperor_amd.simulation import test_init
-    from astroemperor_amd.sampler import default_ladder
-    m = work["model"]
-    T, W, _, _ = work["setup"]
-    p0 = test_init(m, T, W, np.random.default_rng(seed))
-    betas = default_ladder(T, m.ndim__)
-    d = m.data
-    r = cpu_reference.run(m.spec(), d.X, d.Y, d.YERR, d.FLAG, p0, betas, sweeps, ncores=cores, seed=seed)
-    r.pop("final")
-    return r
-
-
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000, help="timed PT sweeps")
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=200, help="timed PT sweeps of the headline workload")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--adapt-sweeps", type=int, default=16, help="gradient-step adaptation sweeps (burn-in)")
     ap.add_argument("--cpu-sweeps", type=int, default=20)
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--half-launches", type=int, default=40,
-                    help="likelihood-kernel launches timed with events (an even number: whole sweeps)")
+    ap.add_argument("--c2-steps", type=int, default=2000)
+    ap.add_argument("--c2-warmup", type=int, default=200)
     ap.add_argument("--systems", type=int, default=4096, help="batched-systems leg (0 = skip)")
     ap.add_argument("--systems-steps", type=int, default=30)
     ap.add_argument("--systems-warmup", type=int, default=5)
-    ap.add_argument("--no-configs", action="store_true", help="skip the other workloads (c1, c4, c5)")
-    ap.add_argument("--c5-steps", type=int, default=20)
-    ap.add_argument("--only", choices=["all", "c2", "systems", "configs", "c5", "lnl", "copy"], default="all",
-                    help="run one leg only (profiling); 'lnl': batched lnL alone on 20480 prior draws; "
-                         "'copy': the HBM-counter calibration copy")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other workloads (c1, c2, c3, c4)")
+    ap.add_argument("--only", choices=["all", "c5", "c5-mixed", "c2", "systems", "configs", "grad", "copy"],
+                    default="all", help="run one leg only (profiling); 'grad': the adjoint alone on 8192 prior "
+                                        "draws of the headline model; 'copy': the HBM-counter calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -413,95 +545,78 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from astroemperor_amd import synth, device_info
+    from astroemperor_amd import device_info
     info = device_info()
+    cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
     if args.only == "copy":
         print(json.dumps({"calibration": run_copy(256, 10), "device": info}), flush=True)
         return
-    if args.only == "lnl":
-        print(json.dumps({"lnl_batch": run_lnl(20480, 8, args.seed), "device": info}), flush=True)
+    if args.only == "grad":
+        print(json.dumps({"grad": run_grad(8192, 4, args.seed, wpts=(0, 2, 4, 8)), "device": info}), flush=True)
         return
     if args.only == "systems":
         print(json.dumps({"batched_systems": run_systems(args.systems, args.systems_steps, args.systems_warmup,
                                                          args.seed), "device": info}), flush=True)
         return
-    if args.only == "c5":
-        print(json.dumps({"c5": run_c5(args.seed, args.c5_steps, 3), "device": info}), flush=True)
+    if args.only == "c2":
+        print(json.dumps({"c2": run_c2(args.seed, args.c2_steps, args.c2_warmup), "device": info}), flush=True)
         return
-    cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
     if args.only == "configs":
         print(json.dumps({"c1": run_c1(args.seed, cores, not args.no_cpu), "c4": run_c4(args.seed),
-                          "c5": run_c5(args.seed, args.c5_steps, 3), "device": info}), flush=True)
+                          "device": info}), flush=True)
         return
 
-    work = synth.c2()
-    m = work["model"]
-    res = {}
-    for prec in ("fp64", "mixed"):
-        res[prec] = run_gpu(work, prec, args.steps, args.warmup, args.seed, args.half_launches)
-        print(f"[bench] {prec}: {res[prec]['sweeps_per_s']:.1f} sweeps/s, "
-              f"half-step kernel {res[prec]['ms_halfstep']:.4f} ms", file=sys.stderr, flush=True)
-    dl = dlnl_report(work, [res["fp64"]["sampler"], res["mixed"]["sampler"]])
-    for r in res.values():
-        r["sampler"].close()
-    full = args.only == "all"
-    systems = run_systems(args.systems, args.systems_steps, args.systems_warmup, args.seed) \
-        if full and args.systems else None
-    configs = None
-    if full and not args.no_configs:
-        configs = {"c1": run_c1(args.seed, cores, not args.no_cpu)}
-        print(f"[bench] c1: {configs['c1']}", file=sys.stderr, flush=True)
-        configs["c4"] = run_c4(args.seed)
-        print(f"[bench] c4: {configs['c4']}", file=sys.stderr, flush=True)
-        configs["c5"] = run_c5(args.seed, args.c5_steps, 3)
-        print(f"[bench] c5: {configs['c5']}", file=sys.stderr, flush=True)
-
-    mixed, fp64 = res["mixed"], res["fp64"]
-    flops = flops_per_theta(m, "mixed") * mixed["evals_per_halfstep"]
-    achieved = flops / (mixed["ms_halfstep"] * 1e-3) / 1e12
-    roof = {"bound": "valu", "kernel": "k_pt_half<mixed>", "achieved": round(achieved, 3),
-            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": traffic_bytes(mixed["waves_per_walker"]),
-            "convention": "FP64-equivalent flops per lnL eval = 2 x (planet-epochs x ops_kep + epochs x 46"
-                          " + MA epochs x 23); ops_kep = 127 (fp64) / 51 (mixed)",
-            "flops_per_launch": flops, "launch_ms": mixed["ms_halfstep"],
-            "fp64_path": {"achieved": round(flops_per_theta(m, "fp64") * fp64["evals_per_halfstep"]
-                                            / (fp64["ms_halfstep"] * 1e-3) / 1e12, 3)}}
-    roof["fp64_path"]["frac"] = round(roof["fp64_path"]["achieved"] / PEAK_FP64_TFLOPS, 4)
+    precisions = ("mixed",) if args.only == "c5-mixed" else ("mixed", "fp64")
+    c5, m = run_c5(args.seed, args.steps, args.warmup, args.adapt_sweeps, precisions)
+    if args.only == "c5-mixed":
+        print(json.dumps({"c5": c5, "device": info}), flush=True)
+        return
+    mixed, fp64 = c5["mixed"], c5["fp64"]
+    # the dominant kernel of the mixed run
+    share = mixed["kernel_time_share"]
+    dom = "k_lnl_grad" if share["k_lnl_grad"] >= share["k_pt_half"] else "k_pt_half"
+    roof = dict(mixed[f"roofline_{dom}"])
+    roof["time_share"] = share[dom]
+    roof["convention"] = ("FP64-equivalent flops = 2 x ops; lnL per vector: planet-epochs x ops_kep (127 FP64 / 51 "
+                          "mixed) + epochs x 46 + MA epochs x 23; adjoint: the same + planet-epochs x 24 + epochs x 4 "
+                          "+ MA epochs x 4")
+    roof["fp64_path"] = {k: fp64[f"roofline_{dom}"][k] for k in ("achieved", "frac", "launch_ms")}
 
     cpu = None
     if not args.no_cpu:
-        r = run_cpu(work, args.cpu_sweeps, cores, args.seed)
-        cpu = {"value": round(r["sweeps_per_s"], 4), "unit": "PT sweeps/s", "cores": r["cores"],
-               "kind": "port",
-               "sample": f"{args.cpu_sweeps} sweeps of the same workload (10 temps x 256 walkers, "
-                         f"{r['sweeps'] * 2560} proposals) on a {r['cores']}-process pool",
-               "lnl_evals_per_s": round(r["lnl_evals_per_s"], 1), "cpu_model": r["cpu_model"],
-               "seconds": round(r["seconds"], 2),
-               "label": "reconstructed reference CPU path (numpy + oracle C Kepler + multiprocessing)"}
+        cpu = run_cpu_c5(m, args.seed, args.cpu_sweeps, cores)
+        print(f"[bench] cpu: {cpu}", file=sys.stderr, flush=True)
 
-    T, W = mixed["T"], mixed["W"]
+    configs = {}
+    if args.only == "all" and not args.no_configs:
+        configs["c1"] = run_c1(args.seed, cores, not args.no_cpu)
+        print(f"[bench] c1: {configs['c1']}", file=sys.stderr, flush=True)
+        configs["c2"] = run_c2(args.seed, args.c2_steps, args.c2_warmup)
+        if args.systems:
+            configs["c3"] = run_systems(args.systems, args.systems_steps, args.systems_warmup, args.seed)
+        configs["c4"] = run_c4(args.seed)
+        print(f"[bench] c4: {configs['c4']}", file=sys.stderr, flush=True)
+
     line = {
-        "metric": METRIC, "value": round(mixed["sweeps_per_s"], 2), "unit": "PT sweeps/s",
+        "metric": METRIC, "value": mixed["sweeps_per_s"], "unit": "PT sweeps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(mixed["ms_per_sweep"], 5), "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": mixed["ms_per_sweep"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64 (mixed)", "data": "synthetic",
-        "config": {"workload": "3 Keplerians, 4 instruments (80/150/400/900 epochs), MA(1), "
-                               f"{T} temps x {W} walkers, 1 stretch step per sweep",
-                   "epochs": int(m.data.ndat), "planets": int(m.kplan__), "instruments": int(m.nins__),
-                   "ndim": int(m.ndim__), "ntemps": T, "nwalkers": W},
-        "lnl_evals_per_s": round(mixed["proposals_per_s"], 1),
-        "lnl_evaluated_per_s": round(mixed["lnl_evaluated_per_s"], 1),
-        "waves_per_walker": mixed["waves_per_walker"],
-        "fp64": {"value": round(fp64["sweeps_per_s"], 2), "lnl_evals_per_s": round(fp64["proposals_per_s"], 1),
-                 "ms_per_step": round(fp64["ms_per_sweep"], 5), "halfstep_ms": round(fp64["ms_halfstep"], 5),
-                 "waves_per_walker": fp64["waves_per_walker"]},
-        "mixed_over_fp64": round(mixed["sweeps_per_s"] / fp64["sweeps_per_s"], 3),
-        "dlnl_mixed_vs_fp64": dl,
+        "config": {"workload": c5["workload"], "epochs": int(m.data.ndat), "planets": int(m.kplan__),
+                   "instruments": int(m.nins__), "ndim": c5["ndim"], "ntemps": c5["ntemps"],
+                   "nwalkers": c5["nwalkers"], "moves": "stretch 4 : gradient 1",
+                   "gradient_step_adaptation_sweeps": args.adapt_sweeps},
+        "lnl_evals_per_s": mixed["lnl_evals_per_s"],
+        "lnl_evaluated_per_s": mixed["lnl_evaluated_per_s"],
+        "fp64": {"value": fp64["sweeps_per_s"], "ms_per_step": fp64["ms_per_sweep"],
+                 "lnl_evals_per_s": fp64["lnl_evals_per_s"]},
+        "mixed_over_fp64": c5.get("mixed_over_fp64"),
+        "dlnl_vs_oracle": c5["dlnl_vs_oracle"],
+        "dlnl_mixed_vs_fp64": c5["dlnl_mixed_vs_fp64"],
         "roofline": roof,
         "cpu_baseline": cpu,
-        "batched_systems": systems,
-        "configs": configs,
+        "headline": c5,
+        "configs": configs or None,
         "device": info,
     }
     if cpu:
