@@ -57,6 +57,7 @@ class EmpMoveDesc(C.Structure):
 
 EMP_MOVE_STRETCH, EMP_MOVE_GRADIENT = 0, 1
 EMP_GRAD_MIXED = 2
+EMP_NOALLOC_SCRATCH_DOUBLES = 1 << 23     # doubles of the allocation-free entry point's static scratch
 
 
 class EmpPTDesc(C.Structure):
@@ -87,6 +88,7 @@ _SIGS = {
     "emp_lnprior_batch": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, C.c_int32, _P]),
     "emp_model_eval": (C.c_int, [_P, _P, _P, _P]),
     "emp_lnl_grad_batch": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P, C.c_int32, _P]),
+    "emp_lnl_grad_batch_noalloc": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P, C.c_int32, _P]),
     "emp_model_set_waves_per_theta": (C.c_int, [_P, C.c_int32]),
     "emp_model_get_waves_per_theta": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "emp_model_nonfinite_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),

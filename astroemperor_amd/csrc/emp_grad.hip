@@ -32,6 +32,7 @@
 // The natural-parameter gradients are then mapped to the block's
 // parameterisation (block_repo.py:23-31) once per vector.
 #include <cmath>
+#include <mutex>
 
 #include "emp_kernels.h"
 
@@ -765,4 +766,66 @@ extern "C" int emp_lnl_grad_batch(EmpModel* m, const double* theta, int64_t B, i
     (void)hipFree(dl);
     (void)hipFree(dg);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of emp_lnl_grad_batch (device pointers).  The
+// moving-average scratch is `scratch`, a static device array that every
+// device the library runs on holds until the process ends, instead of model
+// memory grown with hipMalloc.  Host code never passes the array's name to a
+// kernel: it stands for the host-side shadow, not the device address.  The
+// device address comes from hipGetSymbolAddress, taken once per device with
+// that device current, and is kept in scratch_dev[].
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[EMP_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_dev[MAX_DEVICES] = {};
+}  // namespace
+
+// Device address of this device's copy of `scratch` (the current device).
+static int scratch_address(int dev, double** out) {
+    if (dev < 0 || dev >= MAX_DEVICES) { emp_set_error("device %d outside [0, %d)", dev, MAX_DEVICES); return EMP_EINVAL; }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return emp_hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        if (!p) { emp_set_error("hipGetSymbolAddress(scratch) returned a null address"); return EMP_EHIP; }
+        scratch_dev[dev] = (double*)p;
+    }
+    *out = scratch_dev[dev];
+    return EMP_OK;
+}
+
+extern "C" int emp_lnl_grad_batch_noalloc(EmpModel* m, const double* theta, int64_t B, int64_t ld, double* lnl,
+                                          double* grad, int32_t flags, void* stream) {
+    if (!m) { emp_set_error("null model"); return EMP_EINVAL; }
+    if (B < 0 || (B > 0 && (!theta || !lnl || !grad))) { emp_set_error("bad batch arguments"); return EMP_EINVAL; }
+    if (ld < m->desc.ndim) { emp_set_error("ld < ndim"); return EMP_EINVAL; }
+    if (flags & EMP_HOST_POINTERS) {
+        emp_set_error("the allocation-free entry point takes device pointers only (EMP_HOST_POINTERS stages through allocated memory)");
+        return EMP_EINVAL;
+    }
+    if (flags & ~EMP_GRAD_MIXED) { emp_set_error("unknown flags 0x%x", (unsigned)flags); return EMP_EINVAL; }
+    const size_t need = emp_grad_scratch_doubles(m, B);
+    if (need > EMP_NOALLOC_SCRATCH_DOUBLES) {
+        emp_set_error("batch of %lld vectors needs %zu doubles of scratch; the allocation-free entry point holds %zu",
+                      (long long)B, need, (size_t)EMP_NOALLOC_SCRATCH_DOUBLES);
+        return EMP_EINVAL;
+    }
+    if (emp_fault_injected()) return EMP_EFAULT_INJECTED;
+    if (B == 0) return EMP_OK;
+    int dev = -1;
+    EMP_HIP(hipGetDevice(&dev));
+    if (dev != m->device) { emp_set_error("model lives on device %d, device %d is current", m->device, dev); return EMP_EINVAL; }
+    double* ws = nullptr;
+    if (need) {
+        const int rc = scratch_address(dev, &ws);
+        if (rc) return rc;
+    }
+    const int prec = (flags & EMP_GRAD_MIXED) ? EMP_MIXED : EMP_FP64;
+    return emp_launch_grad_ws(m, theta, B, ld, lnl, grad, ws, (hipStream_t)stream, prec);
 }

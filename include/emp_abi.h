@@ -144,6 +144,22 @@ int emp_model_eval(EmpModel* model, const double* theta, double* model_out, doub
 int emp_lnl_grad_batch(EmpModel* model, const double* theta, int64_t B, int64_t ld,
                        double* lnl, double* grad, int32_t flags, void* stream);
 
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): emp_lnl_grad_batch with device pointers that allocates nothing.
+ * Its moving-average scratch is a static device array of
+ * EMP_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB) that each device the library
+ * runs on holds until the process ends; a batch needing more (with an active
+ * MA term, B x n_epochs doubles) is refused with EMP_EINVAL.  Same results as
+ * emp_lnl_grad_batch, bit for bit.  flags: 0 or EMP_GRAD_MIXED
+ * (EMP_HOST_POINTERS is refused).  Call with the model's device current.
+ * Every call on one device shares that device's array, and each launch
+ * writes every element it reads, so nothing needs clearing between calls;
+ * calls that overlap on one device (other streams or threads) race. */
+#define EMP_NOALLOC_SCRATCH_DOUBLES ((int64_t)1 << 23)
+int emp_lnl_grad_batch_noalloc(EmpModel* model, const double* theta, int64_t B, int64_t ld,
+                               double* lnl, double* grad, int32_t flags, void* stream);
+
 /* Launch shape: waves per parameter vector (1, 2, 4 or 8); 0 = automatic. */
 int emp_model_set_waves_per_theta(EmpModel* model, int32_t waves_per_theta);
 int emp_model_get_waves_per_theta(EmpModel* model, int32_t precision, int32_t* out);
