@@ -433,6 +433,116 @@ __device__ __forceinline__ double plain_w(const ModelDev& md, const SlotLDS& sl,
     return w;
 }
 
+// Adds the per-wave partial sums of a slot in wave order, maps the natural
+// gradients to the parameterisation and writes lnL and the free gradient of
+// vector b.  All threads of the workgroup call this (it holds barriers).
+template <int EXT, int WPT>
+__device__ __forceinline__ void grad_finish(const ModelDev& md, const SlotLDS& sl, GradLDS& gl,
+                                            const GradWaveLDS<EXT>& s_gw, bool live, int wave, int wsub, int lane,
+                                            double lnl_v, int64_t b, double* __restrict__ out_lnl,
+                                            double* __restrict__ out_grad, unsigned long long* nonfinite, int* stale) {
+    using GW = GradWave<EXT>;
+    constexpr bool POST = EXT >= 1;
+    __syncthreads();
+    if (live && wsub == 0) {
+        const int w0 = wave;
+        for (int e = lane; e < GW::N; e += 64) {
+            double v = 0.0;
+            for (int w = 0; w < WPT; ++w) v += s_gw.acc[w0 + w][e];
+            if (POST && e >= GW::SAI) {
+                if (md.sai_w && e - GW::SAI < md.sai_base[md.I]) gl.g[md.off_sai + (e - GW::SAI)] = v;
+            } else if (POST && e >= GW::COS) {
+                (&gl.cosg[0][0])[e - GW::COS] = v;
+            } else if (e >= GW::MAC) {
+                const bool tau = e >= GW::MAT;
+                const int r = e - (tau ? GW::MAT : GW::MAC), set = r / GW::QM, c = r - set * GW::QM;
+                if (md.ma_active && set < (md.ma_global ? 1 : md.I) && c < md.ma_q)
+                    gl.g[md.off_ma + 2 * md.ma_q * set + 2 * c + (tau ? 1 : 0)] = v;
+            } else if (e >= GW::JIT) {
+                if (md.has_jit && e - GW::JIT < md.I) gl.g[md.off_jit + (e - GW::JIT)] = v;
+            } else if (e < md.I) {
+                gl.g[md.off_off + e] = v;
+            }
+        }
+        for (int e = lane; e < WN_N; e += 64) {
+            double v = 0.0;
+            for (int w = 0; w < WPT; ++w) v += s_gw.nat[w0 + w][e];
+            if (e < WN_ACC) {
+                if (e < md.K * 5) (&gl.nat[0][0])[e] = v;
+            } else if (e - WN_ACC < md.acc) {
+                gl.g[md.off_acc + (e - WN_ACC)] = v;
+            }
+        }
+    }
+    __syncthreads();
+    if (live && wsub == 0) {
+        // Map natural gradients to the parameterisation (one lane per planet).
+        if (lane < md.K) {
+            const int p = lane;
+            const PlanetC& pc = sl.planet[p];
+            const double* th = sl.theta + md.off_kep + 5 * p;
+            double* gp = gl.g + md.off_kep + 5 * p;
+            const double per = 2.0 * PI_D / pc.freq;
+            const double gP = gl.nat[p][0] * (-pc.freq / per);
+            const double gph = gl.nat[p][1];
+            const double ge = gl.nat[p][2], gw = gl.nat[p][3], gA = gl.nat[p][4];
+            const int prm = md.param;
+            const bool lnP = (prm == 2 || prm == 6 || prm == 7);
+            const bool sc = (prm == 1 || prm == 2 || prm == 4 || prm == 7);
+            gp[0] = lnP ? gP * per : gP;
+            if (prm == 2) {
+                const double As = th[1], Ac = th[2], A = pc.K;
+                gp[1] = gA * 2.0 * As + gph * (Ac / A);
+                gp[2] = gA * 2.0 * Ac - gph * (As / A);
+            } else {
+                gp[1] = gA;
+                gp[2] = pc.tp_mode ? -pc.freq * gph : gph;
+            }
+            if (sc) {
+                const double S = th[3], C = th[4], e = pc.e;
+                const double thr = (prm == 2 || prm == 4) ? 1e-5 : 1e-6;
+                const bool free_w = md.use_c || e >= thr;
+                gp[3] = ge * 2.0 * S + (free_w ? gw * (C / e) : 0.0);
+                gp[4] = ge * 2.0 * C - (free_w ? gw * (S / e) : 0.0);
+            } else {
+                gp[3] = ge;
+                gp[4] = gw;
+            }
+        }
+        if (POST && lane == 63) {
+            for (int c = 0; c < md.n_cos; ++c) {
+                const int m = md.off_sin >= 0 ? c - 1 : c;    // -1: the sinusoid
+                const int o = m < 0 ? md.off_sin : md.off_mag;
+                const double gP = gl.cosg[c][2] * -(sl.cosw[c][1] / sl.theta[o]);
+                if (m < 0) {
+                    gl.g[o] = gP;
+                    gl.g[o + 1] = gl.cosg[c][0];
+                    gl.g[o + 2] = gl.cosg[c][1];
+                } else {
+                    gl.g[o] += gP;
+                    gl.g[o + 1 + m] = gl.cosg[c][0];
+                    gl.g[o + 3 + m] = gl.cosg[c][1];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (live && wsub == 0) {
+        double v = lnl_v;
+        bool bad = !(v == v) || v == __builtin_inf();
+        for (int d = lane; d < md.D; d += 64) {
+            double gv = gl.g[md.free_idx[d]];
+            if (bad) gv = 0.0;
+            out_grad[b * md.D + d] = gv;
+        }
+        if (lane == 0) {
+            if (bad) { v = -__builtin_inf(); atomicAdd(nonfinite, 1ull); }
+            out_lnl[b] = v;
+            if (stale) stale[b] = 0;
+        }
+    }
+}
+
 // Adjoint kernel.  Per tile (forward without the MA, in reverse with it):
 // stage 1 finds w = dlnL/dmu of each lane's epochs (CPT chunks of 64 WPT
 // epochs, held in registers), stage 2 then takes the planets one at a time,
@@ -562,104 +672,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
         for (int w = 0; w < WPT; ++w) { c2 += s_scan.red[w0 + w][0]; l2 += s_scan.red[w0 + w][1]; }
         lnl_v = -0.5 * (c2 + l2) + md.lnl_const;
     }
-    __syncthreads();
-    if (live && wsub == 0) {
-        const int w0 = wave;
-        for (int e = lane; e < GW::N; e += 64) {
-            double v = 0.0;
-            for (int w = 0; w < WPT; ++w) v += s_gw.acc[w0 + w][e];
-            if (POST && e >= GW::SAI) {
-                if (md.sai_w && e - GW::SAI < md.sai_base[md.I]) gl.g[md.off_sai + (e - GW::SAI)] = v;
-            } else if (POST && e >= GW::COS) {
-                (&gl.cosg[0][0])[e - GW::COS] = v;
-            } else if (e >= GW::MAC) {
-                const bool tau = e >= GW::MAT;
-                const int r = e - (tau ? GW::MAT : GW::MAC), set = r / GW::QM, c = r - set * GW::QM;
-                if (md.ma_active && set < (md.ma_global ? 1 : md.I) && c < md.ma_q)
-                    gl.g[md.off_ma + 2 * md.ma_q * set + 2 * c + (tau ? 1 : 0)] = v;
-            } else if (e >= GW::JIT) {
-                if (md.has_jit && e - GW::JIT < md.I) gl.g[md.off_jit + (e - GW::JIT)] = v;
-            } else if (e < md.I) {
-                gl.g[md.off_off + e] = v;
-            }
-        }
-        for (int e = lane; e < WN_N; e += 64) {
-            double v = 0.0;
-            for (int w = 0; w < WPT; ++w) v += s_gw.nat[w0 + w][e];
-            if (e < WN_ACC) {
-                if (e < md.K * 5) (&gl.nat[0][0])[e] = v;
-            } else if (e - WN_ACC < md.acc) {
-                gl.g[md.off_acc + (e - WN_ACC)] = v;
-            }
-        }
-    }
-    __syncthreads();
-    if (live && wsub == 0) {
-        // Map natural gradients to the parameterisation (one lane per planet).
-        if (lane < md.K) {
-            const int p = lane;
-            const PlanetC& pc = sl.planet[p];
-            const double* th = sl.theta + md.off_kep + 5 * p;
-            double* gp = gl.g + md.off_kep + 5 * p;
-            const double per = 2.0 * PI_D / pc.freq;
-            const double gP = gl.nat[p][0] * (-pc.freq / per);
-            const double gph = gl.nat[p][1];
-            const double ge = gl.nat[p][2], gw = gl.nat[p][3], gA = gl.nat[p][4];
-            const int prm = md.param;
-            const bool lnP = (prm == 2 || prm == 6 || prm == 7);
-            const bool sc = (prm == 1 || prm == 2 || prm == 4 || prm == 7);
-            gp[0] = lnP ? gP * per : gP;
-            if (prm == 2) {
-                const double As = th[1], Ac = th[2], A = pc.K;
-                gp[1] = gA * 2.0 * As + gph * (Ac / A);
-                gp[2] = gA * 2.0 * Ac - gph * (As / A);
-            } else {
-                gp[1] = gA;
-                gp[2] = pc.tp_mode ? -pc.freq * gph : gph;
-            }
-            if (sc) {
-                const double S = th[3], C = th[4], e = pc.e;
-                const double thr = (prm == 2 || prm == 4) ? 1e-5 : 1e-6;
-                const bool free_w = md.use_c || e >= thr;
-                gp[3] = ge * 2.0 * S + (free_w ? gw * (C / e) : 0.0);
-                gp[4] = ge * 2.0 * C - (free_w ? gw * (S / e) : 0.0);
-            } else {
-                gp[3] = ge;
-                gp[4] = gw;
-            }
-        }
-        if (POST && lane == 63) {
-            for (int c = 0; c < md.n_cos; ++c) {
-                const int m = md.off_sin >= 0 ? c - 1 : c;    // -1: the sinusoid
-                const int o = m < 0 ? md.off_sin : md.off_mag;
-                const double gP = gl.cosg[c][2] * -(sl.cosw[c][1] / sl.theta[o]);
-                if (m < 0) {
-                    gl.g[o] = gP;
-                    gl.g[o + 1] = gl.cosg[c][0];
-                    gl.g[o + 2] = gl.cosg[c][1];
-                } else {
-                    gl.g[o] += gP;
-                    gl.g[o + 1 + m] = gl.cosg[c][0];
-                    gl.g[o + 3 + m] = gl.cosg[c][1];
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (live && wsub == 0) {
-        double v = lnl_v;
-        bool bad = !(v == v) || v == __builtin_inf();
-        for (int d = lane; d < md.D; d += 64) {
-            double gv = gl.g[md.free_idx[d]];
-            if (bad) gv = 0.0;
-            out_grad[b * md.D + d] = gv;
-        }
-        if (lane == 0) {
-            if (bad) { v = -__builtin_inf(); atomicAdd(nonfinite, 1ull); }
-            out_lnl[b] = v;
-            if (stale) stale[b] = 0;
-        }
-    }
+    grad_finish<EXT, WPT>(md, sl, gl, s_gw, live, wave, wsub, lane, lnl_v, b, out_lnl, out_grad, nonfinite, stale);
 }
 
 // Each model runs the instance with only the features it uses: the terms

@@ -436,9 +436,11 @@ __global__ __launch_bounds__(MOM_BLOCK) void k_pt_lnl_moments(PTDev pd, long lon
 // ---------------------------------------------------------------------------
 constexpr uint32_t LG_TAG = 0x20000000u;
 constexpr uint32_t MOVE_TAG = 0x10000000u;
-constexpr int LG_BLOCK = 256;
-constexpr int LG_CHUNK = 32;
-constexpr int LG_PAIRS = (MAXD * (MAXD + 1) / 2 + LG_BLOCK - 1) / LG_BLOCK;
+constexpr int LG_BLOCK = 1024;
+constexpr int LG_CHUNK = 64;
+constexpr int LG_NP = MAXD * (MAXD + 1) / 2;
+constexpr int LG_PAIRS = (LG_NP + LG_BLOCK - 1) / LG_BLOCK;
+constexpr int LG_STAGE = (LG_CHUNK * MAXD + LG_BLOCK - 1) / LG_BLOCK;
 constexpr double LG_RIDGE = 1e-6;
 
 __device__ __forceinline__ U4 lg_draw(const PTDev& pd, long long sweep, int half, int step, int t, int k, uint32_t r) {
@@ -478,118 +480,148 @@ __device__ inline void lg_lt_times(const double* L2, const double g[2], int D, i
 }
 
 // Covariance of the complementary half at each temperature and its Cholesky
-// factor; one workgroup per temperature.  Deterministic: every sum runs in a
-// fixed order.  Block 0 of the first half-step of the sweep also counts the
-// likelihoods of the current-position gradient launch.
+// factor; one workgroup of 1024 threads per temperature.  The walkers are
+// staged through LDS in chunks (the next chunk's rows are loaded into
+// registers while the current one is summed); the covariance and its
+// factorisation stay in LDS as a packed lower triangle (row i starts at
+// i (i + 1) / 2).  Each thread owns the same entries in both phases.  The
+// factorisation is right-looking on the unscaled columns,
+// A_ik -= A_ij A_kj / A_jj for j < k <= i, with one barrier per column; the
+// columns are scaled by 1 / sqrt(A_jj) at the end.  Deterministic: every sum
+// runs in a fixed order.  Block 0 of the first half-step of the sweep also
+// counts the likelihoods of the current-position gradient launch.
 __global__ __launch_bounds__(LG_BLOCK) void k_lg_precond(PTDev pd, int half, int step) {
     __shared__ double s_x[LG_CHUNK][MAXD];
+    __shared__ double s_a[LG_NP];
     __shared__ double s_mean[MAXD], s_diag[MAXD];
-    __shared__ int s_fail;
     const int t = blockIdx.x, D = pd.D, W2 = pd.W2, tid = threadIdx.x;
     const int* cmap = pd.map + t * pd.W + (1 - half) * W2;
     double* A = pd.lg_L2 + (size_t)t * D * D;
     double* L1 = pd.lg_L1 + (size_t)t * D * D;
     if (t == 0 && tid == 0 && half == 0 && step == 0)
         atomicAdd((unsigned long long*)(pd.sweep + 1), (unsigned long long)pd.T * pd.W);
-    // mean
+    const int np = D * (D + 1) / 2;
+    // this thread's entries p = tid + m LG_BLOCK of the packed triangle, as i << 16 | j
+    int pij[LG_PAIRS];
+    {
+        int i = 0, j = tid;
+        while (j > i) { j -= i + 1; ++i; }
+#pragma unroll
+        for (int m = 0; m < LG_PAIRS; ++m) {
+            pij[m] = i << 16 | j;
+            j += LG_BLOCK;
+            while (j > i) { j -= i + 1; ++i; }
+        }
+    }
+    double pre[LG_STAGE];
+    auto fetch = [&](int w0) {
+        const int nw = min(LG_CHUNK, W2 - w0);
+#pragma unroll
+        for (int r = 0; r < LG_STAGE; ++r) {
+            const int e = tid + r * LG_BLOCK;
+            if (e < nw * D) {
+                const int w = e / D, j = e - w * D;
+                pre[r] = pd.X[(size_t)cmap[w0 + w] * D + j];
+            }
+        }
+    };
+    auto stage = [&](int w0, bool centre) {
+        const int nw = min(LG_CHUNK, W2 - w0);
+#pragma unroll
+        for (int r = 0; r < LG_STAGE; ++r) {
+            const int e = tid + r * LG_BLOCK;
+            if (e < nw * D) {
+                const int w = e / D, j = e - w * D;
+                s_x[w][j] = centre ? pre[r] - s_mean[j] : pre[r];
+            }
+        }
+    };
+    // mean (thread j < D sums column j over the walkers in order)
     double sum = 0.0;
+    fetch(0);
     for (int w0 = 0; w0 < W2; w0 += LG_CHUNK) {
         const int nw = min(LG_CHUNK, W2 - w0);
         __syncthreads();
-        for (int e = tid; e < nw * D; e += LG_BLOCK) {
-            const int w = e / D, j = e - w * D;
-            s_x[w][j] = pd.X[(size_t)cmap[w0 + w] * D + j];
-        }
+        stage(w0, false);
+        if (w0 + LG_CHUNK < W2) fetch(w0 + LG_CHUNK);
         __syncthreads();
         if (tid < D)
             for (int w = 0; w < nw; ++w) sum += s_x[w][tid];
     }
     if (tid < D) s_mean[tid] = sum / (double)W2;
-    // lower triangle of the covariance: this thread's pairs p = tid + m LG_BLOCK,
-    // p = i (i + 1) / 2 + j, j <= i
-    const int np = D * (D + 1) / 2;
+    // lower triangle of the covariance
     double acc[LG_PAIRS];
+#pragma unroll
     for (int m = 0; m < LG_PAIRS; ++m) acc[m] = 0.0;
+    fetch(0);
     for (int w0 = 0; w0 < W2; w0 += LG_CHUNK) {
         const int nw = min(LG_CHUNK, W2 - w0);
         __syncthreads();
-        for (int e = tid; e < nw * D; e += LG_BLOCK) {
-            const int w = e / D, j = e - w * D;
-            s_x[w][j] = pd.X[(size_t)cmap[w0 + w] * D + j] - s_mean[j];
-        }
+        stage(w0, true);
+        if (w0 + LG_CHUNK < W2) fetch(w0 + LG_CHUNK);
         __syncthreads();
-        int i = 0, j = tid;
-        while (j > i) { j -= i + 1; ++i; }
 #pragma unroll
         for (int m = 0; m < LG_PAIRS; ++m) {
             if (tid + m * LG_BLOCK < np) {
+                const int i = pij[m] >> 16, j = pij[m] & 0xffff;
                 double a = acc[m];
                 for (int w = 0; w < nw; ++w) a = __builtin_fma(s_x[w][i], s_x[w][j], a);
                 acc[m] = a;
             }
-            j += LG_BLOCK;
-            while (j > i) { j -= i + 1; ++i; }
         }
     }
-    {
-        int i = 0, j = tid;
-        while (j > i) { j -= i + 1; ++i; }
-        for (int m = 0; m < LG_PAIRS; ++m) {
-            if (tid + m * LG_BLOCK < np) {
-                const double c = acc[m] / (double)W2;
-                A[(size_t)i * D + j] = c;
-                if (i == j) s_diag[i] = c;
-            }
-            j += LG_BLOCK;
-            while (j > i) { j -= i + 1; ++i; }
+#pragma unroll
+    for (int m = 0; m < LG_PAIRS; ++m) {
+        const int p = tid + m * LG_BLOCK;
+        if (p < np) {
+            const double c = acc[m] / (double)W2;
+            s_a[p] = c;
+            if ((pij[m] >> 16) == (pij[m] & 0xffff)) s_diag[pij[m] >> 16] = c;
         }
     }
     __syncthreads();
     // parameters without spread: row and column cleared, unit pivot, inactive
     if (tid < D) {
-        const bool act = s_diag[tid] > 0.0;
-        pd.lg_act[t * D + tid] = act ? 1 : 0;
+        const int i = tid;
+        const bool act = s_diag[i] > 0.0;
+        pd.lg_act[t * D + i] = act ? 1 : 0;
+        const int ri = i * (i + 1) / 2;
         if (act) {
-            A[(size_t)tid * D + tid] = s_diag[tid] * (1.0 + LG_RIDGE);
+            s_a[ri + i] = s_diag[i] * (1.0 + LG_RIDGE);
         } else {
-            for (int k = 0; k < tid; ++k) A[(size_t)tid * D + k] = 0.0;
-            for (int i = tid + 1; i < D; ++i) A[(size_t)i * D + tid] = 0.0;
-            A[(size_t)tid * D + tid] = 1.0;
+            for (int k = 0; k < i; ++k) s_a[ri + k] = 0.0;
+            for (int r = i + 1; r < D; ++r) s_a[r * (r + 1) / 2 + i] = 0.0;
+            s_a[ri + i] = 1.0;
         }
     }
-    if (tid == 0) s_fail = 0;
-    // right-looking Cholesky on the lower triangle
+    bool fail = false;
     for (int j = 0; j < D; ++j) {
         __syncthreads();
-        if (tid == 0) {
-            const double piv = A[(size_t)j * D + j];
-            if (!(piv > 0.0) || !(piv < __builtin_inf())) s_fail = 1;
-            else A[(size_t)j * D + j] = sqrt(piv);
+        const double piv = s_a[j * (j + 1) / 2 + j];    // final since step j - 1; same value in every thread
+        if (!(piv > 0.0) || !(piv < __builtin_inf())) {
+            fail = true;
+            break;
         }
-        __syncthreads();
-        if (s_fail) break;
-        const double d = A[(size_t)j * D + j];
-        for (int i = j + 1 + tid; i < D; i += LG_BLOCK) A[(size_t)i * D + j] /= d;
-        __syncthreads();
-        const int m = D - j - 1, nq = m * (m + 1) / 2;
-        int ii = 0, kk = tid;
-        while (kk > ii) { kk -= ii + 1; ++ii; }
-        for (int q = tid; q < nq; q += LG_BLOCK) {
-            const int i = j + 1 + ii, k = j + 1 + kk;
-            A[(size_t)i * D + k] = __builtin_fma(-A[(size_t)i * D + j], A[(size_t)k * D + j], A[(size_t)i * D + k]);
-            kk += LG_BLOCK;
-            while (kk > ii) { kk -= ii + 1; ++ii; }
+        const double inv = 1.0 / piv;
+#pragma unroll
+        for (int m = 0; m < LG_PAIRS; ++m) {
+            const int p = tid + m * LG_BLOCK;
+            const int i = pij[m] >> 16, k = pij[m] & 0xffff;
+            if (p < np && k > j) {
+                const double lij = s_a[i * (i + 1) / 2 + j] * inv;
+                s_a[p] = __builtin_fma(-lij, s_a[k * (k + 1) / 2 + j], s_a[p]);
+            }
         }
     }
     __syncthreads();
-    const bool fail = s_fail != 0;
     for (int e = tid; e < D * D; e += LG_BLOCK) {
         const int i = e / D, k = e - i * D;
         double v = 0.0;
         if (fail) {
             if (i == k) v = s_diag[i] > 0.0 ? sqrt(s_diag[i]) : 1.0;
         } else if (k <= i) {
-            v = A[e];
+            const double dk = sqrt(s_a[k * (k + 1) / 2 + k]);
+            v = i == k ? dk : s_a[i * (i + 1) / 2 + k] / dk;
         }
         A[e] = v;
         L1[(size_t)k * D + i] = v;

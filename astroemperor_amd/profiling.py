@@ -1,6 +1,6 @@
 """Text summaries of rocprofv3 result databases (rocpd SQLite).
 
-    python -m astroemperor_amd.profiling RESULTS.db [--counters | --gaps] [--top N]
+    python -m astroemperor_amd.profiling RESULTS.db [--counters [--by-grid] | --gaps] [--top N]
 
 Without options: one row per kernel (dispatches, total and mean device
 time, VGPRs, LDS) from a ``--kernel-trace`` run.  With --gaps: dispatches in
@@ -8,7 +8,9 @@ time order grouped by (previous kernel, kernel), with the mean duration and
 the mean idle time between the previous kernel's end and this one's start:
 the launch sequence of a captured sweep and what each boundary costs.  With
 --counters: the mean value per dispatch of every collected counter, per
-kernel, from a ``--pmc`` run (FETCH_SIZE / WRITE_SIZE are in KiB).
+kernel (or per kernel and grid size with --by-grid, which separates launches
+of one kernel over different batch sizes), from a ``--pmc`` run (FETCH_SIZE /
+WRITE_SIZE are in KiB).
 """
 from __future__ import annotations
 
@@ -52,8 +54,17 @@ def gap_table(db, top=12, min_count=50):
     return "\n".join(out)
 
 
-def counter_table(db, top=12):
+def counter_table(db, top=12, by_grid=False):
     c = sqlite3.connect(db)
+    if by_grid:
+        rows = c.execute(
+            "select kernel_name, grid_size, counter_name, count(*), avg(value), min(value), max(value) "
+            "from counters_collection group by kernel_name, grid_size, counter_name order by avg(value) desc "
+            "limit ?", (top,)).fetchall()
+        out = ["| kernel | grid | counter | dispatches | mean | min | max |", "|---|---|---|---|---|---|---|"]
+        for name, grid, cn, n, avg, mn, mx in rows:
+            out.append(f"| `{name[:80]}` | {grid} | {cn} | {n} | {avg:.1f} | {mn:.1f} | {mx:.1f} |")
+        return "\n".join(out)
     rows = c.execute(
         "select kernel_name, counter_name, count(*), avg(value), min(value), max(value) from counters_collection "
         "group by kernel_name, counter_name order by avg(value) desc limit ?", (top,)).fetchall()
@@ -68,10 +79,11 @@ def main():
     ap.add_argument("db")
     ap.add_argument("--counters", action="store_true")
     ap.add_argument("--gaps", action="store_true")
+    ap.add_argument("--by-grid", action="store_true", help="with --counters: one row per kernel and grid size")
     ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args()
     if a.counters:
-        print(counter_table(a.db, a.top))
+        print(counter_table(a.db, a.top, a.by_grid))
     elif a.gaps:
         print(gap_table(a.db, a.top))
     else:

@@ -54,18 +54,26 @@ OPS_DEPOCH = 4
 OPS_DMA = 4
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
 # HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB per
-# dispatch) on the headline workload, each scaled by what the same counter
-# reads for a plain 256 MiB device copy (FETCH 2.00, WRITE 1.00).  Keys:
-# (kernel, precision).  Filled from the committed counter summaries.
-TRAFFIC_KIB = {}
+# dispatch, mean over the dispatches of the launch shape the tuner chose),
+# each scaled by what the same counter reads for a plain 256 MiB device copy
+# (FETCH_SIZE 131,084 KiB: factor 2.00; WRITE_SIZE 262,144 KiB: 1.00).  Keys:
+# (workload, kernel, precision); values (FETCH_SIZE, WRITE_SIZE).  Sources:
+# profiles/traffic_k_pt_half_k_lnl_grad_c5_c2.md.
+TRAFFIC_KIB = {
+    ("c5", "k_pt_half", "mixed"): (3383.0, 1446.3),       # k_pt_half<1, 4>, 8192 proposals
+    ("c5", "k_lnl_grad", "mixed"): (327892.4, 643328.0),  # k_lnl_grad<1, 4, 0>, 8192 vectors (eps scratch)
+    ("c2", "k_pt_half", "mixed"): (577.1, 84.5),          # k_pt_half<1, 8>, 1280 proposals
+    ("c2", "k_pt_half", "fp64"): (583.5, 84.5),           # k_pt_half<0, 8>
+}
+FETCH_SCALE, WRITE_SCALE = 2.00, 1.00
 
 
-def traffic_bytes(kernel, prec):
-    v = TRAFFIC_KIB.get((kernel, prec))
+def traffic_bytes(workload, kernel, prec):
+    v = TRAFFIC_KIB.get((workload, kernel, prec))
     if v is None:
         return None
     f, w = v
-    return int((2.0 * f + 1.0 * w) * 1024)
+    return int((FETCH_SCALE * f + WRITE_SCALE * w) * 1024)
 
 
 def _ma_epochs(model):
@@ -88,10 +96,11 @@ def flops_grad_per_theta(model, precision):
     return 2.0 * ops
 
 
-def roofline(kernel, prec, flops, ms, extra=None):
+def roofline(kernel, prec, flops, ms, extra=None, workload=None):
     ach = flops / (ms * 1e-3) / 1e12
     out = {"bound": "valu", "kernel": f"{kernel}<{prec}>", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
-           "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": traffic_bytes(kernel, prec),
+           "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4),
+           "traffic": traffic_bytes(workload, kernel, prec),
            "flops_per_launch": flops, "launch_ms": round(ms, 5)}
     if extra:
         out.update(extra)
@@ -182,8 +191,8 @@ def run_c5(seed, steps, warmup, adapt_sweeps=16, precisions=("mixed", "fp64"), o
         states[prec] = st
         props = T * W // 2
         r_half = roofline("k_pt_half", prec, flops_l[prec] * ev_half, ms_half,
-                          {"lnl_evaluated_per_launch": round(ev_half, 1), "proposals_per_launch": props})
-        r_grad = roofline("k_lnl_grad", prec, flops_g[prec] * props, ms_grad, {"vectors_per_launch": props})
+                          {"lnl_evaluated_per_launch": round(ev_half, 1), "proposals_per_launch": props}, "c5")
+        r_grad = roofline("k_lnl_grad", prec, flops_g[prec] * props, ms_grad, {"vectors_per_launch": props}, "c5")
         # share of the mix's time: 4 stretch sweeps (2 half-step launches each) per gradient sweep (2 proposal
         # adjoints plus the one over the slots a stretch move moved)
         t_mix = 4 * kinds["stretch"]["ms_per_sweep"] + kinds["gradient"]["ms_per_sweep"]
@@ -267,7 +276,7 @@ def run_c2(seed, steps, warmup, half_launches=40):
                      "waves_per_walker": wpt,
                      "roofline": roofline("k_pt_half", prec, flops_per_theta(m, prec) * ev, ms_half,
                                           {"lnl_evaluated_per_launch": round(ev, 1),
-                                           "proposals_per_launch": T * W // 2})}
+                                           "proposals_per_launch": T * W // 2}, "c2")}
         if prec == "mixed":
             last = s.get_last_sample().coords
         s.close()
