@@ -58,6 +58,7 @@ class EmpMoveDesc(C.Structure):
 EMP_MOVE_STRETCH, EMP_MOVE_GRADIENT = 0, 1
 EMP_GRAD_MIXED = 2
 EMP_NOALLOC_SCRATCH_DOUBLES = 1 << 23     # doubles of the allocation-free entry point's static scratch
+EMP_GRAD_KERNEL_AUTO, EMP_GRAD_KERNEL_TWO_PASS, EMP_GRAD_KERNEL_SINGLE_PASS = 0, 1, 2
 
 
 class EmpPTDesc(C.Structure):
@@ -90,6 +91,7 @@ _SIGS = {
     "emp_lnl_grad_batch": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P, C.c_int32, _P]),
     "emp_lnl_grad_batch_noalloc": (C.c_int, [_P, _P, C.c_int64, C.c_int64, _P, _P, C.c_int32, _P]),
     "emp_model_set_waves_per_theta": (C.c_int, [_P, C.c_int32]),
+    "emp_model_set_grad_kernel": (C.c_int, [_P, C.c_int32]),
     "emp_model_get_waves_per_theta": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "emp_model_nonfinite_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "emp_pt_create": (C.c_int, [_P, C.POINTER(EmpPTDesc), _P, _P, C.POINTER(_P)]),

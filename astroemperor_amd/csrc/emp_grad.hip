@@ -32,6 +32,7 @@
 // The natural-parameter gradients are then mapped to the block's
 // parameterisation (block_repo.py:23-31) once per vector.
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 
 #include "emp_kernels.h"
@@ -675,6 +676,434 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     grad_finish<EXT, WPT>(md, sl, gl, s_gw, live, wave, wsub, lane, lnl_v, b, out_lnl, out_grad, nonfinite, stale);
 }
 
+// ---------------------------------------------------------------------------
+// Single-pass adjoint for models whose moving average has order 1 or is
+// absent: eight waves per vector, one vector per workgroup, one epoch per
+// lane per tile.  Each planet's Kepler solve is done once per epoch; its
+// (sin E, cos E) stay in registers from the model to the derivatives.
+//
+// The MA(1) adjoint is carried forward.  Within a tile the reverse
+// recurrence lambda_i = u_i + a_{i+1} lambda_{i+1} composes (a suffix scan
+// started from zero) into lambda_i = B_i + A_i Lambda, with Lambda the value
+// at the next tile's first epoch, unknown while the tile is processed.  Every
+// gradient sum is linear in lambda: sum_i lambda_i h_i = S_B + Lambda S_A,
+// S_B = sum B_i h_i, S_A = sum A_i h_i.  With (A^n, B^n) the map of tile n as
+// a whole (that of its first epoch), Lambda_{n-1} = B^n + A^n Lambda_n and
+// Lambda = 0 after the last tile, so
+//   sum_n (S_B^n + Lambda_n S_A^n) = sum_n (S_B^n + B^n R_n),
+//   R_0 = 0, R_{n+1} = A^n R_n + S_A^n.
+// Every such sum keeps a total and an R state and is complete after one pass
+// over the epochs; no eps row is stored.
+// ---------------------------------------------------------------------------
+
+// (x, y) exchanged between lanes 32 apart by v_permlane32_swap, which swaps
+// the upper half of its first operand with the lower half of its second:
+// lanes 0-31 end with (x, x of lane + 32), lanes 32-63 with (y of lane - 32,
+// y), so the sum holds x summed over the pair below and y above.
+__device__ __forceinline__ double swap_add32(double x, double y) {
+    const long long xi = __double_as_longlong(x), yi = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)xi, (unsigned)yi, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(xi >> 32), (unsigned)(yi >> 32), false, false);
+    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
+           __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+
+// The same between lanes 16 apart within each half (v_permlane16_swap):
+// x summed over the pair where lane bit 4 is 0, y where it is 1.
+__device__ __forceinline__ double swap_add16(double x, double y) {
+    const long long xi = __double_as_longlong(x), yi = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)xi, (unsigned)yi, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(xi >> 32), (unsigned)(yi >> 32), false, false);
+    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
+           __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+
+// Wave sums of two groups of five per-lane values, b[0..4] and a[0..4] (the
+// adjoint's B and A parts), as one scattered reduction of sixteen values in
+// which slots 5-7 and 13-15 are zero: halves are exchanged across lanes 32
+// and 16 apart (permlane swaps: each lane keeps the half its lane bit
+// selects and adds its partner's), then across lanes 8 apart and with the
+// lane mirrored in its group of eight (lane ^ 7, which flips bit 2) by DPP,
+// and the last value is summed over each quad.  Lane 4 i then holds the
+// total of b[i] and lane 32 + 4 i that of a[i].  Whole wave active.
+__device__ __forceinline__ double wave_reduce_ba5(const double (&bv)[5], const double (&av)[5], int lane) {
+    const bool h3 = lane & 8, h2 = lane & 4;
+    double a[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) a[i] = swap_add32(bv[i], av[i]);     // slot i below, i + 8 above
+    double b[4];
+    b[0] = swap_add16(a[0], a[4]);
+#pragma unroll
+    for (int i = 1; i < 4; ++i) b[i] = swap_add16(a[i], 0.0);          // slot i + 4 is zero
+    double c[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double keep = h3 ? b[i + 2] : b[i], send = h3 ? b[i] : b[i + 2];
+        c[i] = keep + dpp_f64<0x128, 0xf>(0.0, send);   // row_ror 8: lane ^ 8
+    }
+    double d = (h2 ? c[1] : c[0]) + dpp_f64<0x141, 0xf>(0.0, h2 ? c[0] : c[1]);   // row_half_mirror
+    d += dpp_f64<0xB1, 0xf>(0.0, d);    // quad_perm [1, 0, 3, 2]
+    d += dpp_f64<0x4E, 0xf>(0.0, d);    // quad_perm [2, 3, 0, 1]
+    return d;
+}
+
+// Per-lane sums of the per-instrument terms (offset, jitter, MA(1)
+// coefficient and timescale) of the lane's epochs, the offset and MA sums
+// with their R states; added to the wave's LDS totals and R states when the
+// lane's instrument changes (the MA sums are keyed by instrument, or by 0 for
+// a global MA).
+struct InsAcc1 {
+    int k = -1, km = -1;
+    double off = 0.0, offR = 0.0, jit = 0.0, mac = 0.0, macR = 0.0, mat = 0.0, matR = 0.0;
+    template <int EXT>
+    __device__ __forceinline__ void flush_ins(double* tot, double* rst) {
+        using GW = GradWave<EXT>;
+        if (k < 0) return;
+        atomicAdd(&tot[GW::OFF + k], off);
+        atomicAdd(&rst[GW::OFF + k], offR);
+        atomicAdd(&tot[GW::JIT + k], jit);
+        off = offR = jit = 0.0;
+    }
+    template <int EXT>
+    __device__ __forceinline__ void flush_ma(double* tot, double* rst) {
+        using GW = GradWave<EXT>;
+        if (km < 0) return;
+        atomicAdd(&tot[GW::MAC + km * GW::QM], mac);
+        atomicAdd(&rst[GW::MAC + km * GW::QM], macR);
+        atomicAdd(&tot[GW::MAT + km * GW::QM], mat);
+        atomicAdd(&rst[GW::MAT + km * GW::QM], matR);
+        mac = macR = mat = matR = 0.0;
+    }
+    template <int EXT>
+    __device__ __forceinline__ void to(int kk, int kkm, double* tot, double* rst) {
+        if (kk != k) {
+            flush_ins<EXT>(tot, rst);
+            k = kk;
+        }
+        if (kkm != km) {
+            flush_ma<EXT>(tot, rst);
+            km = kkm;
+        }
+    }
+};
+
+// The epochs of one tile of the single-pass kernel (NT of them) and the
+// sin/cos table; thread j stages epoch tb + j, its loads issued a tile ahead.
+template <int NT>
+struct Tile1LDS {
+    double t[NT], y[NT], s2[NT], dt[NT];
+    int ins[NT];
+    SinCosTab tab;
+};
+
+template <int NT>
+__device__ __forceinline__ void tile1_prefetch(const ModelDev& md, int tb, EpochRegs& r) {
+    const int n = tb + (int)threadIdx.x;
+    if (n < md.N) {
+        r.t = md.t[n];
+        r.y = md.y[n];
+        r.s2 = md.s2[n];
+        r.ins = md.ins[n];
+        r.dt = md.ma_active ? md.dt[n] : 0.0;
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void tile1_store(const ModelDev& md, Tile1LDS<NT>& tl, int tb, const EpochRegs& r) {
+    const int j = threadIdx.x;
+    if (tb + j < md.N) {
+        tl.t[j] = r.t;
+        tl.y[j] = r.y;
+        tl.s2[j] = r.s2;
+        tl.ins[j] = r.ins;
+        tl.dt[j] = r.dt;
+    }
+}
+
+template <int EXT>
+struct Grad1LDS {
+    double rst[NWAVES][GradWave<EXT>::N];    // R states of the per-instrument sums (per wave)
+    double natR[NWAVES][WN_N];               // R states of the planet and acceleration sums (per wave)
+    double suf[2][NWAVES][2];                // suffix maps of the waves (double-buffered)
+};
+
+// Single-pass adjoint (above).  EXT: 0 plain models, 1 with the terms added
+// after the MA.  NW waves per workgroup (one vector), tiles of 64 NW epochs.
+// `stale` as in k_lnl_grad.
+template <int PREC, int EXT, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_lnl_grad1(ModelDev md, const double* __restrict__ theta, int64_t B,
+                                                       int64_t ld, double* __restrict__ out_lnl,
+                                                       double* __restrict__ out_grad, unsigned long long* nonfinite,
+                                                       int* stale) {
+    constexpr int WPT = NW, NT = 64 * NW, TILE1 = NT;
+    constexpr bool POST = EXT >= 1;
+    using GW = GradWave<EXT>;
+    __shared__ SlotLDS sl;
+    __shared__ GradLDS gl;
+    __shared__ GradWaveLDS<EXT> s_gw;
+    __shared__ Grad1LDS<EXT> s_g1;
+    __shared__ Tile1LDS<TILE1> s_tile;
+    __shared__ ScanLDS s_scan;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    if (b >= B || (stale && stale[b] == 0)) return;     // the whole workgroup: one vector
+    expand_theta(md, sl, theta + b * ld, g, NT);
+    for (int j = g; j < MAXD; j += NT) gl.g[j] = 0.0;
+    for (int j = g; j < KMAX * 5; j += NT) (&gl.nat[0][0])[j] = 0.0;
+    double* wacc = s_gw.acc[wave];
+    double* wnat = s_gw.nat[wave];
+    double* wrst = s_g1.rst[wave];
+    double* wnatR = s_g1.natR[wave];
+    for (int j = lane; j < GW::N; j += 64) wacc[j] = wrst[j] = 0.0;
+    for (int j = lane; j < WN_N; j += 64) wnat[j] = wnatR[j] = 0.0;
+    for (int j = g; j <= SC_N; j += NT) s_tile.tab.v[j] = md.sctab[j];
+    if (wave == 0) slot_setup(md, sl, lane);
+    __syncthreads();
+    if (PREC == 1)      // the mixed path's per-vector Kepler tables, one planet per wave
+        for (int p = wave; p < md.K; p += NW)
+            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
+    __syncthreads();
+
+    const bool ma = md.ma_active;
+    double chi = 0.0, prod = 1.0;
+    int pexp = 0;
+    double carry = 0.0;                 // eps of the last epoch so far
+    int parity = 0, sparity = 0;
+    InsAcc1 ia;
+    EpochRegs nx{};
+    tile1_prefetch<TILE1>(md, 0, nx);
+    for (int tb = 0; tb < md.N; tb += TILE1) {
+        const int tn = min(TILE1, md.N - tb);
+        __syncthreads();                // the previous tile's readers are done
+        tile1_store<TILE1>(md, s_tile, tb, nx);
+        __syncthreads();
+        if (tb + TILE1 < md.N) tile1_prefetch<TILE1>(md, tb + TILE1, nx);
+        const int j = g;
+        const bool valid = j < tn;
+        // Stage 1: every planet's (sin E, cos E), the model and the residual.
+        double sE[KMAX], cE[KMAX];
+        double t = 0.0, r = 0.0, err2 = 1.0;
+        int k = 0;
+        if (valid) {
+            t = s_tile.t[j];
+            k = s_tile.ins[j];
+            double mu = sl.gam[k];
+            if (md.acc) {
+                const double x = t - md.t0;
+                double v = 0.0;
+                for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
+                mu += v * x;
+            }
+#pragma unroll
+            for (int p = 0; p < KMAX; ++p) {
+                sE[p] = cE[p] = 0.0;
+                if (p < md.K) {
+                    const PlanetC& pc = sl.planet[p];
+                    planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p]);
+                    mu += __builtin_fma(pc.c1, cE[p], pc.c2 * sE[p])
+                          * rcp_path<PREC>(__builtin_fma(-pc.e, cE[p], 1.0));
+                }
+            }
+            err2 = s_tile.s2[j] + sl.jit2[k];
+            r = s_tile.y[j] - mu;
+            ia.to<EXT>(k, md.ma_global ? 0 : k, wacc, wrst);
+        }
+        // The MA filter, forward (the same scan and composition as block_lnl).
+        double eps = r, em1 = 0.0;      // eps of this epoch and of the previous one
+        if (ma) {
+            double a = 1.0, bb = 0.0;
+            if (valid) {
+                const double dtj = s_tile.dt[j];
+                if (md.ma_wfixed) a = -(sl.ma_c[k] * dtj);
+                else a = dtj < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-dtj / sl.ma_tau[k]));
+                bb = r;
+            }
+            wave_affine_scan(a, bb, lane);
+            if (lane == 63) { s_scan.tot[parity][wave][0] = a; s_scan.tot[parity][wave][1] = bb; }
+            __syncthreads();
+            double pa = 1.0, pb = 0.0, e0 = 0.0;
+            for (int w = 0; w < WPT; ++w) {
+                if (w == wave) {
+                    e0 = __builtin_fma(pa, carry, pb);
+                    eps = __builtin_fma(a, e0, bb);
+                }
+                const double ta = s_scan.tot[parity][w][0], tbv = s_scan.tot[parity][w][1];
+                pb = __builtin_fma(ta, pb, tbv);
+                pa = ta * pa;
+            }
+            carry = __builtin_fma(pa, carry, pb);
+            parity ^= 1;
+            em1 = __shfl_up(eps, 1, 64);
+            if (lane == 0) em1 = e0;
+        }
+        // The likelihood's terms.
+        double ef = 0.0, ie = 0.0;
+        if (valid) {
+            const double cs = (POST && has_post(md)) ? epoch_post(md, sl, s_tile.tab, t, tb + j, k) : 0.0;
+            ef = eps - cs;
+            ie = rcp_path<PREC>(err2);
+            chi = __builtin_fma(ef * ef, ie, chi);
+            prod *= err2;
+            pexp += __builtin_amdgcn_frexp_exp(prod);
+            prod = __builtin_amdgcn_frexp_mant(prod);
+            if (md.has_jit) ia.jit = __builtin_fma(sl.jit[k], ef * ef * ie * ie - ie, ia.jit);
+            if (POST && has_post(md)) post_accumulate<EXT>(md, sl, s_tile.tab, t, tb + j, k, ef * ie, wacc);
+        }
+        // dlnL/dmu = wB + wA Lambda; lambda = lB + lA Lambda; (tA, tB) the tile's map.
+        double wB = 0.0, wA = 0.0, tA = 0.0, tB = 0.0;
+        if (!ma) {
+            wB = ef * ie;
+            ia.off += wB;
+        } else {
+            double u = 0.0, amul = 1.0;
+            if (valid) {
+                u = -ef * ie;
+                const int n1 = tb + j + 1;     // the next epoch's multiplier
+                amul = 0.0;
+                if (n1 < md.N) {
+                    const bool in_tile = j + 1 < tn;
+                    const int k1 = in_tile ? s_tile.ins[j + 1] : md.ins[n1];
+                    const double c1 = in_tile ? s_tile.dt[j + 1] : md.dt[n1];
+                    amul = ma_mult(md, sl, k1, c1);
+                }
+            }
+            double sa = amul, sb = u;
+            wave_affine_suffix_scan(sa, sb, lane);
+            if (lane == 0) { s_g1.suf[sparity][wave][0] = sa; s_g1.suf[sparity][wave][1] = sb; }
+            __syncthreads();
+            double pa = 1.0, pb = 0.0, lB = 0.0, lA = 0.0;
+            for (int w = WPT - 1; w >= 0; --w) {
+                if (w == wave) {
+                    lB = __builtin_fma(sa, pb, sb);
+                    lA = sa * pa;
+                }
+                const double ta = s_g1.suf[sparity][w][0], tbv = s_g1.suf[sparity][w][1];
+                pb = __builtin_fma(ta, pb, tbv);
+                pa = ta * pa;
+            }
+            tA = pa;
+            tB = pb;
+            sparity ^= 1;
+            if (valid) {
+                wB = -lB;
+                wA = -lA;
+            }
+            // the sums this wave already holds in LDS move to the next tile
+            for (int e = lane; e < GW::COS; e += 64) {
+                const double R0 = wrst[e];
+                wacc[e] = __builtin_fma(tB, R0, wacc[e]);
+                wrst[e] = tA * R0;
+            }
+            // this lane's sums: the epoch's terms, then the move to the next tile
+            double hc = 0.0, ht = 0.0;      // d a_j / d (c, tau) times eps_{j-1}
+            if (valid) {
+                const double col = s_tile.dt[j];
+                const bool start = md.ma_wfixed ? col == 0.0 : col < 0.0;
+                if (!start) {
+                    const double wgt = md.ma_wfixed ? col : exp(-col / sl.ma_tau[k]);
+                    hc = em1 * -wgt;
+                    if (!md.ma_wfixed) {
+                        const double tau = sl.ma_tau[k];
+                        ht = em1 * -(sl.ma_c[k] * wgt * col / (tau * tau));
+                    }
+                }
+            }
+            ia.off += __builtin_fma(tB, ia.offR, wB);
+            ia.offR = __builtin_fma(tA, ia.offR, wA);
+            ia.mac += __builtin_fma(tB, ia.macR, lB * hc);
+            ia.macR = __builtin_fma(tA, ia.macR, lA * hc);
+            ia.mat += __builtin_fma(tB, ia.matR, lB * ht);
+            ia.matR = __builtin_fma(tA, ia.matR, lA * ht);
+        }
+        // Stage 2: the planets one at a time, their derivatives from the
+        // stage-1 (sin E, cos E); one wave reduction per planet per tile.
+#pragma unroll
+        for (int p = 0; p < KMAX; ++p) {
+            if (p < md.K) {
+                const PlanetC& pc = sl.planet[p];
+                double f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                if (valid) {
+                    const double e = pc.e;
+                    const double rD = rcp_path<PREC>(__builtin_fma(-e, cE[p], 1.0));
+                    const double uu = cE[p] - e, vv = pc.bq * sE[p];
+                    const double sfw = __builtin_fma(vv, pc.cw, uu * pc.sw) * rD;    // sin(f + w)
+                    const double cfw = __builtin_fma(uu, pc.cw, -(vv * pc.sw)) * rD; // cos(f + w)
+                    const double Ks = pc.K * sfw;
+                    const double fM = -(Ks * pc.bq) * (rD * rD);
+                    f[0] = fM * (pc.tp_mode ? t - pc.tp : t);
+                    f[1] = fM;
+                    f[2] = __builtin_fma(-Ks, sE[p] * rD * __builtin_fma(pc.bq, rD, pc.ibq), pc.K * pc.cw);
+                    f[3] = -pc.K * __builtin_fma(e, pc.sw, sfw);
+                    f[4] = __builtin_fma(e, pc.cw, cfw);
+                }
+                if (!ma) {
+                    double a8[8] = {wB * f[0], wB * f[1], wB * f[2], wB * f[3], wB * f[4], 0.0, 0.0, 0.0};
+                    wave_reduce_add8(a8, 5, wnat + 5 * p, lane);
+                } else {
+                    double bv[5], av[5];
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        bv[i] = wB * f[i];
+                        av[i] = wA * f[i];
+                    }
+                    const double s = wave_reduce_ba5(bv, av, lane);
+                    const double sA = __shfl_down(s, 32, 64);
+                    if (lane < 20 && (lane & 3) == 0) {
+                        const int e = 5 * p + (lane >> 2);
+                        const double R0 = wnatR[e];
+                        wnat[e] += __builtin_fma(tB, R0, s);
+                        wnatR[e] = __builtin_fma(tA, R0, sA);
+                    }
+                }
+            }
+        }
+        if (md.acc) {
+            double xq[AMAX] = {0.0, 0.0, 0.0, 0.0};
+            if (valid) {
+                const double x = t - md.t0;
+                double xp = x;
+#pragma unroll
+                for (int q = AMAX - 1; q >= 0; --q) {
+                    if (q < md.acc) {
+                        xq[q] = xp;
+                        xp *= x;
+                    }
+                }
+            }
+            if (!ma) {
+                double a8[8] = {wB * xq[0], wB * xq[1], wB * xq[2], wB * xq[3], 0.0, 0.0, 0.0, 0.0};
+                wave_reduce_add8(a8, md.acc, wnat + WN_ACC, lane);
+            } else {
+                double bv[5], av[5];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    bv[q] = q < AMAX ? wB * xq[q] : 0.0;
+                    av[q] = q < AMAX ? wA * xq[q] : 0.0;
+                }
+                const double s = wave_reduce_ba5(bv, av, lane);
+                const double sA = __shfl_down(s, 32, 64);
+                if ((lane & 3) == 0 && (lane >> 2) < md.acc) {
+                    const int e = WN_ACC + (lane >> 2);
+                    const double R0 = wnatR[e];
+                    wnat[e] += __builtin_fma(tB, R0, s);
+                    wnatR[e] = __builtin_fma(tA, R0, sA);
+                }
+            }
+        }
+    }
+    ia.flush_ins<EXT>(wacc, wrst);
+    ia.flush_ma<EXT>(wacc, wrst);
+    double ldet = log(prod) + (double)pexp * LN2;
+    chi = wave_sum(chi);
+    ldet = wave_sum(ldet);
+    if (lane == 0) { s_scan.red[wave][0] = chi; s_scan.red[wave][1] = ldet; }
+    __syncthreads();
+    double c2 = 0.0, l2 = 0.0;
+    for (int w = 0; w < WPT; ++w) { c2 += s_scan.red[w][0]; l2 += s_scan.red[w][1]; }
+    const double lnl_v = -0.5 * (c2 + l2) + md.lnl_const;
+    grad_finish<EXT, WPT>(md, sl, gl, s_gw, true, wave, wave, lane, lnl_v, b, out_lnl, out_grad, nonfinite, stale);
+}
+
 // Each model runs the instance with only the features it uses: the terms
 // added after the MA cost scalar registers, MA orders above 1 vector
 // registers.
@@ -694,10 +1123,41 @@ static void launch_grad_t(const ModelDev& md, const double* th, int64_t B, int64
                            lnl, grad, nf, scratch, stale);
 }
 
+// The single-pass kernel runs models whose MA order is at most 1; by
+// default only those with at least GRAD1_MIN_N epochs: it keeps one epoch
+// per lane, so shorter series would leave lanes idle where the two-pass
+// kernel takes fewer waves per vector.  emp_model_set_grad_kernel overrides
+// the choice per model.
+constexpr int GRAD1_NW = 4;
+constexpr int GRAD1_MIN_N = 4 * 64 * GRAD1_NW;
+
+static bool use_grad1(const EmpModel* m) {
+    const ModelDev& md = m->dev;
+    if (md.ma_active && md.ma_q > 1) return false;
+    if (m->grad_kernel == EMP_GRAD_KERNEL_SINGLE_PASS) return true;
+    if (m->grad_kernel == EMP_GRAD_KERNEL_TWO_PASS) return false;
+    return md.N >= GRAD1_MIN_N;
+}
+
+extern "C" int emp_model_set_grad_kernel(EmpModel* m, int32_t which) {
+    if (!m) { emp_set_error("null model"); return EMP_EINVAL; }
+    if (which != EMP_GRAD_KERNEL_AUTO && which != EMP_GRAD_KERNEL_TWO_PASS && which != EMP_GRAD_KERNEL_SINGLE_PASS) {
+        emp_set_error("adjoint kernel must be EMP_GRAD_KERNEL_AUTO, _TWO_PASS or _SINGLE_PASS");
+        return EMP_EINVAL;
+    }
+    if (which == EMP_GRAD_KERNEL_SINGLE_PASS && m->dev.ma_active && m->dev.ma_q > 1) {
+        emp_set_error("the single-pass adjoint handles MA orders up to 1 (this model: %d)", m->dev.ma_q);
+        return EMP_EUNSUP;
+    }
+    m->grad_kernel = which;
+    return EMP_OK;
+}
+
 // Doubles of scratch one launch over B vectors needs: with the moving
-// average, one row of eps per vector (B x N); none without it.
+// average in the two-pass kernel, one row of eps per vector (B x N); none
+// otherwise.
 size_t emp_grad_scratch_doubles(const EmpModel* m, int64_t B) {
-    return m->dev.ma_active ? (size_t)B * (size_t)m->dev.N : 0;
+    return (m->dev.ma_active && !use_grad1(m)) ? (size_t)B * (size_t)m->dev.N : 0;
 }
 
 // The MA adjoint's scratch, model-owned, grown (never shrunk) on demand.
@@ -735,9 +1195,19 @@ int emp_pick_grad_wpt(const EmpModel* m, int64_t B) {
 int emp_launch_grad_ws(const EmpModel* m, const double* th, int64_t B, int64_t ld, double* lnl, double* grad,
                        double* eps_scratch, hipStream_t s, int precision, int* stale, int wpt) {
     if (B <= 0) return EMP_OK;
+    unsigned long long* nf = m->d_nonfinite;
+    if (use_grad1(m)) {
+        const ModelDev& md = m->dev;
+        if (precision != 0 && precision != 1) { emp_set_error("bad precision"); return EMP_EINVAL; }
+#define EMP_CASE1(P, E) hipLaunchKernelGGL((k_lnl_grad1<P, E, GRAD1_NW>), dim3((unsigned)B), dim3(64 * GRAD1_NW), 0, s, md, th, B, ld, lnl, grad, nf, stale)
+        if (precision == 0) { if (has_post(md)) EMP_CASE1(0, 1); else EMP_CASE1(0, 0); }
+        else { if (has_post(md)) EMP_CASE1(1, 1); else EMP_CASE1(1, 0); }
+#undef EMP_CASE1
+        EMP_HIP(hipGetLastError());
+        return EMP_OK;
+    }
     if (m->dev.ma_active && !eps_scratch) { emp_set_error("moving-average gradient needs a scratch buffer"); return EMP_EINVAL; }
     if (wpt <= 0) wpt = emp_pick_grad_wpt(m, B);
-    unsigned long long* nf = m->d_nonfinite;
 #define EMP_CASE(P, W) if (precision == P && wpt == W) launch_grad_t<P, W>(m->dev, th, B, ld, lnl, grad, nf, eps_scratch, stale, s); else
     EMP_CASE(0, 1) EMP_CASE(0, 2) EMP_CASE(0, 4) EMP_CASE(0, 8)
     EMP_CASE(1, 1) EMP_CASE(1, 2) EMP_CASE(1, 4) EMP_CASE(1, 8) { emp_set_error("bad precision/wpt"); return EMP_EINVAL; }

@@ -92,6 +92,7 @@ struct EmpModel {
     double* d_stage_out = nullptr;
     size_t stage_in_cap = 0, stage_out_cap = 0;
     int wpt_override = 0;
+    int grad_kernel = 0;      // EMP_GRAD_KERNEL_*
     int device = 0;
     int n_cu = 256;
     double* d_grad_scratch = nullptr;   // MA adjoint: eps rows [B][N]

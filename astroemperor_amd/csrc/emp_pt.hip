@@ -1402,7 +1402,7 @@ extern "C" int emp_pt_set_moves(EmpPT* p, const EmpMoveDesc* moves, int32_t n) {
             (rc = alloc(p, &v.lg_act, (size_t)v.T * D)) || (rc = alloc(p, &v.lg_Xp, T2 * D)) ||
             (rc = alloc(p, &v.lg_gp, T2 * D)) || (rc = alloc(p, &v.lg_lp, T2)) || (rc = alloc(p, &v.lg_lq, T2)) ||
             (rc = alloc(p, &v.lg_eps, (size_t)v.T)) || (rc = alloc(p, &v.lg_stale, TW)) ||
-            (m->dev.ma_active && (rc = alloc(p, &v.lg_scr, TW * (size_t)m->dev.N)))) {
+            (emp_grad_scratch_doubles(m, (int64_t)TW) && (rc = alloc(p, &v.lg_scr, emp_grad_scratch_doubles(m, (int64_t)TW))))) {
             v.lg_g = nullptr;     // the allocations stay with the engine; a later call retries
             v.lg_stale = nullptr;
             return rc;

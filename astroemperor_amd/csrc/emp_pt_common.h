@@ -281,86 +281,35 @@ __device__ inline void pt_swap_stage(const PTView& pv, long long sweep, double* 
 // the decision and the swap.  On return s_red[c] = (accepted, sum |log
 // ratio|) of pair (c, c+1).  Walker states never move; only map entries do.
 // bet: the ladder (LDS copy).
-//
-// In LDS the rounds are made by the first wave alone: the LDS accesses of one
-// wave execute in program order, so a round reads what the previous round
-// wrote without a workgroup barrier (a wavefront-scope fence keeps the
-// compiler from moving accesses across rounds).  The pairs of one round are
-// disjoint (the partners are a permutation), so each lane decides up to
-// SWAP_LANE_BATCH pairs with all their loads issued together.
-constexpr int SWAP_LANE_BATCH = 4;
-
-__device__ __forceinline__ void swap_decide(int h, int W, double dbeta, int e, double lu, int jp, double* L, int* Mp,
-                                            double* LU, int* JP, double lh, double lc, int mh, int mc) {
-    const int c = h - 1, i = e - (h - 1) * W;
-    const int ih = h * W + i, jc = c * W + jp;
-    double lr = dbeta * (lh - lc);
-    if (!(lr == lr)) lr = -__builtin_inf();
-    const bool acc = lu < lr;
-    if (acc) {
-        L[ih] = lc;
-        L[jc] = lh;
-        Mp[ih] = mc;
-        Mp[jc] = mh;
-    }
-    LU[e] = (lr > -__builtin_inf() && lr < __builtin_inf()) ? fabs(lr) : 0.0;
-    JP[e] = acc ? 1 : 0;
-}
-
 __device__ inline void pt_swap_round(const PTView& pv, long long sweep, double* L, int* Mp, double* LU, int* JP,
                                      bool global_mem, double (*s_red)[2], const double* bet, int nthr) {
     (void)sweep;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
     const int T = pv.T, W = pv.W;
-    if (!global_mem) {
-        if (wave == 0) {
-            for (int h = T - 1; h >= 1; --h) {
-                const int c = h - 1;
-                const double dbeta = bet[c] - bet[h];
-                for (int i0 = 0; i0 < W; i0 += 64 * SWAP_LANE_BATCH) {
-                    int e[SWAP_LANE_BATCH], jp[SWAP_LANE_BATCH], mh[SWAP_LANE_BATCH], mc[SWAP_LANE_BATCH];
-                    double lu[SWAP_LANE_BATCH], lh[SWAP_LANE_BATCH], lc[SWAP_LANE_BATCH];
-#pragma unroll
-                    for (int r = 0; r < SWAP_LANE_BATCH; ++r) {
-                        const int i = i0 + r * 64 + lane;
-                        e[r] = i < W ? (h - 1) * W + i : -1;
-                        if (e[r] >= 0) {
-                            jp[r] = JP[e[r]];
-                            lu[r] = LU[e[r]];
-                        }
-                    }
-#pragma unroll
-                    for (int r = 0; r < SWAP_LANE_BATCH; ++r) {
-                        if (e[r] >= 0) {
-                            const int ih = h * W + (i0 + r * 64 + lane), jc = c * W + jp[r];
-                            lh[r] = L[ih];
-                            lc[r] = L[jc];
-                            mh[r] = Mp[ih];
-                            mc[r] = Mp[jc];
-                        }
-                    }
-#pragma unroll
-                    for (int r = 0; r < SWAP_LANE_BATCH; ++r)
-                        if (e[r] >= 0) swap_decide(h, W, dbeta, e[r], lu[r], jp[r], L, Mp, LU, JP, lh[r], lc[r], mh[r], mc[r]);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                __builtin_amdgcn_wave_barrier();
+    for (int h = T - 1; h >= 1; --h) {
+        const int c = h - 1;
+        const double dbeta = bet[c] - bet[h];
+        for (int i = tid; i < W; i += nthr) {
+            const int e = (h - 1) * W + i;
+            const int jp = JP[e];
+            const double lu = LU[e];
+            const int ih = h * W + i, jc = c * W + jp;
+            const double lh = L[ih], lc = L[jc];
+            double lr = dbeta * (lh - lc);
+            if (!(lr == lr)) lr = -__builtin_inf();
+            const bool acc = lu < lr;
+            if (acc) {
+                L[ih] = lc;
+                L[jc] = lh;
+                const int mh = Mp[ih];
+                Mp[ih] = Mp[jc];
+                Mp[jc] = mh;
             }
+            LU[e] = (lr > -__builtin_inf() && lr < __builtin_inf()) ? fabs(lr) : 0.0;
+            JP[e] = acc ? 1 : 0;
         }
+        if (global_mem) __threadfence();
         __syncthreads();
-    } else {
-        for (int h = T - 1; h >= 1; --h) {
-            const int c = h - 1;
-            const double dbeta = bet[c] - bet[h];
-            for (int i = tid; i < W; i += nthr) {
-                const int e = (h - 1) * W + i;
-                const int jp = JP[e];
-                const int ih = h * W + i, jc = c * W + jp;
-                swap_decide(h, W, dbeta, e, LU[e], jp, L, Mp, LU, JP, L[ih], L[jc], Mp[ih], Mp[jc]);
-            }
-            __threadfence();
-            __syncthreads();
-        }
     }
     for (int h = 1 + wave; h < T; h += nw) {      // one wave per temperature pair
         double na = 0.0, di = 0.0;

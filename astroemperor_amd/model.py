@@ -294,6 +294,14 @@ class ReddModel:
     def set_waves_per_theta(self, w):
         _abi.check(_abi.lib().emp_model_set_waves_per_theta(self.device.handle, int(w)))
 
+    def set_grad_kernel(self, which="auto"):
+        """Adjoint kernel for lnl_grad_batch and the gradient move: 'auto'
+        (single-pass for MA order <= 1 and >= 1024 epochs), 'two_pass' or
+        'single_pass' (MA order <= 1 only)."""
+        code = {"auto": _abi.EMP_GRAD_KERNEL_AUTO, "two_pass": _abi.EMP_GRAD_KERNEL_TWO_PASS,
+                "single_pass": _abi.EMP_GRAD_KERNEL_SINGLE_PASS}[which]
+        _abi.check(_abi.lib().emp_model_set_grad_kernel(self.device.handle, code))
+
     def nonfinite_count(self):
         v = C.c_int64()
         _abi.check(_abi.lib().emp_model_nonfinite_count(self.device.handle, C.byref(v)))

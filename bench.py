@@ -481,37 +481,46 @@ def fd_check(m, sub, h_rel=1e-6):
             "all_pass": bool(all(p["pass"] for p in per.values() if p["judged"])), "per_parameter": per}
 
 
-def run_grad(n, reps, seed, wpts=(0,), precisions=("mixed", "fp64")):
+def run_grad(n, reps, seed, kernels=("single_pass", "two_pass"), precisions=("mixed", "fp64")):
     """The adjoint alone (emp_lnl_grad_batch, device pointers) on n prior
-    draws of the headline model, against the batched lnL on the same draws:
-    the subject of the adjoint's counter runs."""
+    draws of the headline model with each adjoint kernel, against the
+    batched lnL on the same draws, and the largest difference between the
+    kernels' gradients (relative to max(|g|, 1)): the subject of the
+    adjoint's counter runs."""
     import torch
     from astroemperor_amd import synth
     from astroemperor_amd.simulation import test_init
     m = synth.c5()["model"]
     th = torch.tensor(test_init(m, 1, n, np.random.default_rng(seed))[0], device="cuda")
-    out = {}
-    for w in wpts:
-        m.set_waves_per_theta(w)
+    out, grads = {}, {}
+    for kern in kernels:
+        m.set_grad_kernel(kern)
         for prec in precisions:
             m.lnl_grad_batch(th, prec)
             m.lnl_batch(th, prec)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
-                m.lnl_grad_batch(th, prec)
+                lnl, g = m.lnl_grad_batch(th, prec)
             torch.cuda.synchronize()
             tg = (time.perf_counter() - t0) / reps
+            grads[(kern, prec)] = g
             t0 = time.perf_counter()
             for _ in range(reps):
                 m.lnl_batch(th, prec)
             torch.cuda.synchronize()
             tl = (time.perf_counter() - t0) / reps
-            out[f"{prec}_wpt{w}"] = {"batch": n, "ms_grad": round(1e3 * tg, 4), "ms_lnl": round(1e3 * tl, 4),
+            out[f"{prec}_{kern}"] = {"batch": n, "ms_grad": round(1e3 * tg, 4), "ms_lnl": round(1e3 * tl, 4),
                                     "grad_over_lnl": round(tg / tl, 3),
                                     "frac_grad": round(flops_grad_per_theta(m, prec) * n / tg / 1e12
                                                        / PEAK_FP64_TFLOPS, 4)}
-    m.set_waves_per_theta(0)
+    m.set_grad_kernel("auto")
+    if len(kernels) == 2:
+        for prec in precisions:
+            a, b = grads[(kernels[0], prec)], grads[(kernels[1], prec)]
+            fin = torch.isfinite(a).all(1) & torch.isfinite(b).all(1)
+            d = ((a[fin] - b[fin]).abs() / b[fin].abs().clamp(min=1.0)).max().item() if fin.any() else None
+            out[f"{prec}_max_rel_grad_difference"] = d
     return out
 
 
@@ -561,7 +570,7 @@ def main():
         print(json.dumps({"calibration": run_copy(256, 10), "device": info}), flush=True)
         return
     if args.only == "grad":
-        print(json.dumps({"grad": run_grad(8192, 4, args.seed, wpts=(0, 2, 4, 8)), "device": info}), flush=True)
+        print(json.dumps({"grad": run_grad(8192, 4, args.seed), "device": info}), flush=True)
         return
     if args.only == "systems":
         print(json.dumps({"batched_systems": run_systems(args.systems, args.systems_steps, args.systems_warmup,

@@ -139,18 +139,21 @@ int emp_model_eval(EmpModel* model, const double* theta, double* model_out, doub
 /* No reference counterpart: lnL and dlnL/dtheta[free]
  * for a batch, by a hand-written adjoint kernel (FP64; with EMP_GRAD_MIXED the
  * Kepler solve of the mixed path, FP32 start and one FP64 step). grad is [B, ndim].
- * With an active MA(1) term the model keeps a device scratch of B x n_epochs
- * doubles (grown on demand, freed with the model). */
+ * The two-pass adjoint kernel (emp_model_set_grad_kernel: MA orders above 1,
+ * series shorter than 1024 epochs by default) keeps, with an active MA term,
+ * a device scratch of B x n_epochs doubles in the model (grown on demand,
+ * freed with the model); the single-pass kernel needs none. */
 int emp_lnl_grad_batch(EmpModel* model, const double* theta, int64_t B, int64_t ld,
                        double* lnl, double* grad, int32_t flags, void* stream);
 
 /* No reference counterpart.  For callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): emp_lnl_grad_batch with device pointers that allocates nothing.
- * Its moving-average scratch is a static device array of
+ * The two-pass kernel's moving-average scratch is a static device array of
  * EMP_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB) that each device the library
  * runs on holds until the process ends; a batch needing more (with an active
- * MA term, B x n_epochs doubles) is refused with EMP_EINVAL.  Same results as
+ * MA term in the two-pass kernel, B x n_epochs doubles) is refused with
+ * EMP_EINVAL.  The single-pass kernel uses no scratch.  Same results as
  * emp_lnl_grad_batch, bit for bit.  flags: 0 or EMP_GRAD_MIXED
  * (EMP_HOST_POINTERS is refused).  Call with the model's device current.
  * Every call on one device shares that device's array, and each launch
@@ -162,6 +165,23 @@ int emp_lnl_grad_batch_noalloc(EmpModel* model, const double* theta, int64_t B, 
 
 /* Launch shape: waves per parameter vector (1, 2, 4 or 8); 0 = automatic. */
 int emp_model_set_waves_per_theta(EmpModel* model, int32_t waves_per_theta);
+
+/* Adjoint kernel of emp_lnl_grad_batch (and of the gradient move) for this
+ * model.  No counterpart in the reference (its gradients, where a caller
+ * wants them, are finite differences of my_likelihood); the two kernels
+ * give the same lnL and gradients to rounding.
+ *   EMP_GRAD_KERNEL_AUTO: single-pass when the MA order is at most 1 and the
+ *     series has at least 1024 epochs, else two-pass;
+ *   EMP_GRAD_KERNEL_TWO_PASS: likelihood with the MA residuals stored
+ *     (B x N doubles of scratch), then the epochs in reverse;
+ *   EMP_GRAD_KERNEL_SINGLE_PASS: one pass, no scratch; EMP_EUNSUP for MA
+ *     orders above 1.
+ * A gradient move enabled before a change keeps the scratch it allocated:
+ * switch before emp_pt_set_moves. */
+#define EMP_GRAD_KERNEL_AUTO 0
+#define EMP_GRAD_KERNEL_TWO_PASS 1
+#define EMP_GRAD_KERNEL_SINGLE_PASS 2
+int emp_model_set_grad_kernel(EmpModel* model, int32_t which);
 int emp_model_get_waves_per_theta(EmpModel* model, int32_t precision, int32_t* out);
 
 /* Number of non-finite likelihoods mapped to -inf since creation. */
