@@ -91,25 +91,41 @@ __device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab&
     a[4] = __builtin_fma(w, __builtin_fma(e, pc.cw, cfw), a[4]);
 }
 
+// (x, y) exchanged between lanes 32 apart by v_permlane32_swap, which swaps
+// the upper half of its first operand with the lower half of its second:
+// lanes 0-31 end with (x, x of lane + 32), lanes 32-63 with (y of lane - 32,
+// y), so the sum holds x summed over the pair below and y above.
+__device__ __forceinline__ double swap_add32(double x, double y) {
+    const long long xi = __double_as_longlong(x), yi = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)xi, (unsigned)yi, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(xi >> 32), (unsigned)(yi >> 32), false, false);
+    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
+           __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+
+// The same between lanes 16 apart within each half (v_permlane16_swap):
+// x summed over the pair where lane bit 4 is 0, y where it is 1.
+__device__ __forceinline__ double swap_add16(double x, double y) {
+    const long long xi = __double_as_longlong(x), yi = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)xi, (unsigned)yi, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(xi >> 32), (unsigned)(yi >> 32), false, false);
+    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
+           __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+
 // Sum of eight per-lane values over the wave, scattered: halves of the
-// values are exchanged across lanes 32 apart, then 16, then 8 (each lane
-// keeps the half its lane bit selects and adds its partner's), and the
-// remaining value is summed over each aligned group of eight lanes.  Lane
-// 8 i then holds the total of value i; the first n are added to dst[i].
-// 10 exchanges instead of 48 for eight separate wave sums.  Whole wave active.
+// values are exchanged across lanes 32 apart, then 16 (permlane swaps: each
+// lane keeps the half its lane bit selects and adds its partner's), then 8
+// (DPP), and the remaining value is summed over each aligned group of eight
+// lanes.  Lane 8 i then holds the total of value i; the first n are added to
+// dst[i].  Whole wave active.
 __device__ __forceinline__ void wave_reduce_add8(double (&v)[8], int n, double* dst, int lane) {
     double a[4], b[2];
-    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+    const bool h3 = lane & 8;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const double keep = h5 ? v[i + 4] : v[i], send = h5 ? v[i] : v[i + 4];
-        a[i] = keep + __shfl_xor(send, 32, 64);
-    }
+    for (int i = 0; i < 4; ++i) a[i] = swap_add32(v[i], v[i + 4]);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const double keep = h4 ? a[i + 2] : a[i], send = h4 ? a[i] : a[i + 2];
-        b[i] = keep + __shfl_xor(send, 16, 64);
-    }
+    for (int i = 0; i < 2; ++i) b[i] = swap_add16(a[i], a[i + 2]);
     double c = (h3 ? b[1] : b[0]) + dpp_f64<0x128, 0xf>(0.0, h3 ? b[0] : b[1]);   // row_ror 8: lane ^ 8
     c += dpp_f64<0x141, 0xf>(0.0, c);   // row_half_mirror: lane i with 7 - i of its group of 8
     c += dpp_f64<0xB1, 0xf>(0.0, c);    // quad_perm [1, 0, 3, 2]
@@ -695,28 +711,6 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
 // Every such sum keeps a total and an R state and is complete after one pass
 // over the epochs; no eps row is stored.
 // ---------------------------------------------------------------------------
-
-// (x, y) exchanged between lanes 32 apart by v_permlane32_swap, which swaps
-// the upper half of its first operand with the lower half of its second:
-// lanes 0-31 end with (x, x of lane + 32), lanes 32-63 with (y of lane - 32,
-// y), so the sum holds x summed over the pair below and y above.
-__device__ __forceinline__ double swap_add32(double x, double y) {
-    const long long xi = __double_as_longlong(x), yi = __double_as_longlong(y);
-    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)xi, (unsigned)yi, false, false);
-    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(xi >> 32), (unsigned)(yi >> 32), false, false);
-    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
-           __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
-}
-
-// The same between lanes 16 apart within each half (v_permlane16_swap):
-// x summed over the pair where lane bit 4 is 0, y where it is 1.
-__device__ __forceinline__ double swap_add16(double x, double y) {
-    const long long xi = __double_as_longlong(x), yi = __double_as_longlong(y);
-    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)xi, (unsigned)yi, false, false);
-    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(xi >> 32), (unsigned)(yi >> 32), false, false);
-    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) +
-           __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
-}
 
 // Wave sums of two groups of five per-lane values, b[0..4] and a[0..4] (the
 // adjoint's B and A parts), as one scattered reduction of sixteen values in

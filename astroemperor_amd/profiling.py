@@ -1,9 +1,10 @@
 """Text summaries of rocprofv3 result databases (rocpd SQLite).
 
-    python -m astroemperor_amd.profiling RESULTS.db [--counters [--by-grid] | --gaps] [--top N]
+    python -m astroemperor_amd.profiling RESULTS.db [--counters | --gaps] [--by-grid] [--top N]
 
 Without options: one row per kernel (dispatches, total and mean device
-time, VGPRs, LDS) from a ``--kernel-trace`` run.  With --gaps: dispatches in
+time, VGPRs, LDS) from a ``--kernel-trace`` run (per kernel and grid size
+with --by-grid).  With --gaps: dispatches in
 time order grouped by (previous kernel, kernel), with the mean duration and
 the mean idle time between the previous kernel's end and this one's start:
 the launch sequence of a captured sweep and what each boundary costs.  With
@@ -18,11 +19,12 @@ import argparse
 import sqlite3
 
 
-def kernel_table(db, top=12):
+def kernel_table(db, top=12, by_grid=False):
     c = sqlite3.connect(db)
+    group = "name, grid_x" if by_grid else "name"
     rows = c.execute(
         "select name, count(*), sum(duration), avg(duration), avg(vgpr_count), avg(lds_size), avg(grid_x), "
-        "avg(workgroup_x) from kernels group by name order by sum(duration) desc limit ?", (top,)).fetchall()
+        f"avg(workgroup_x) from kernels group by {group} order by sum(duration) desc limit ?", (top,)).fetchall()
     total = c.execute("select sum(duration) from kernels").fetchone()[0] or 1
     out = ["| kernel | dispatches | total ms | mean us | share | VGPRs | LDS B | grid | WG |",
            "|---|---|---|---|---|---|---|---|---|"]
@@ -79,7 +81,8 @@ def main():
     ap.add_argument("db")
     ap.add_argument("--counters", action="store_true")
     ap.add_argument("--gaps", action="store_true")
-    ap.add_argument("--by-grid", action="store_true", help="with --counters: one row per kernel and grid size")
+    ap.add_argument("--by-grid", action="store_true",
+                    help="one row per kernel and grid size (launches of one kernel over different batch sizes)")
     ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args()
     if a.counters:
@@ -87,7 +90,7 @@ def main():
     elif a.gaps:
         print(gap_table(a.db, a.top))
     else:
-        print(kernel_table(a.db, a.top))
+        print(kernel_table(a.db, a.top, a.by_grid))
 
 
 if __name__ == "__main__":

@@ -57,13 +57,16 @@ PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
 # dispatch, mean over the dispatches of the launch shape the tuner chose),
 # each scaled by what the same counter reads for a plain 256 MiB device copy
 # (FETCH_SIZE 131,084 KiB: factor 2.00; WRITE_SIZE 262,144 KiB: 1.00).  Keys:
-# (workload, kernel, precision); values (FETCH_SIZE, WRITE_SIZE).  Sources:
-# profiles/traffic_k_pt_half_k_lnl_grad_c5_c2.md.
+# (workload, kernel, precision); values (FETCH_SIZE, WRITE_SIZE).  Source:
+# profiles/hbm_traffic_k_pt_half_k_lnl_grad1_k_sys_sweep.md.
 TRAFFIC_KIB = {
     ("c5", "k_pt_half", "mixed"): (3383.0, 1446.3),       # k_pt_half<1, 4>, 8192 proposals
-    ("c5", "k_lnl_grad", "mixed"): (327892.4, 643328.0),  # k_lnl_grad<1, 4, 0>, 8192 vectors (eps scratch)
+    ("c5", "k_lnl_grad", "mixed"): (3775.1, 3584.0),      # k_lnl_grad1<1, 0, 4>, 8192 vectors
+    ("c5", "k_lnl_grad", "fp64"): (3786.6, 3584.0),       # k_lnl_grad1<0, 0, 4>
     ("c2", "k_pt_half", "mixed"): (577.1, 84.5),          # k_pt_half<1, 8>, 1280 proposals
     ("c2", "k_pt_half", "fp64"): (583.5, 84.5),           # k_pt_half<0, 8>
+    ("c3", "k_sys_sweep", "mixed"): (194014.1, 139825.4),  # k_sys_sweep<1, 1, 16>, 4096 systems
+    ("c3", "k_sys_sweep", "fp64"): (195433.0, 139722.6),   # k_sys_sweep<0, 1, 16>
 }
 FETCH_SCALE, WRITE_SCALE = 2.00, 1.00
 
@@ -332,7 +335,7 @@ def run_systems(n_systems, steps, warmup, seed, precisions=("mixed", "fp64"), tr
                      "ms_per_sweep_events": round(ev_ms, 4),
                      "lnl_evals_per_s": round(props / wall, 1), "lnl_evaluated_per_s": round((n1 - n0) / wall, 1),
                      "in_support_fraction": round(frac_eval, 4), "tuning": tune, "create_s": round(create_s, 2),
-                     "roofline": roofline("k_sys_sweep", prec, flops, ev_ms)}
+                     "roofline": roofline("k_sys_sweep", prec, flops, ev_ms, None, "c3")}
         print(f"[bench] systems {prec}: {steps / wall:.2f} sweeps/s ({props / wall / 1e6:.1f} M lnL/s), "
               f"tuning {tune}", file=sys.stderr, flush=True)
         states[prec] = s
