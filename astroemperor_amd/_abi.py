@@ -59,6 +59,7 @@ EMP_MOVE_STRETCH, EMP_MOVE_GRADIENT = 0, 1
 EMP_GRAD_MIXED = 2
 EMP_NOALLOC_SCRATCH_DOUBLES = 1 << 23     # doubles of the allocation-free entry point's static scratch
 EMP_GRAD_KERNEL_AUTO, EMP_GRAD_KERNEL_TWO_PASS, EMP_GRAD_KERNEL_SINGLE_PASS = 0, 1, 2
+EMP_SWAP_AUTO, EMP_SWAP_LDS, EMP_SWAP_STREAM, EMP_SWAP_GLOBAL = 0, 1, 2, 3
 
 
 class EmpPTDesc(C.Structure):
@@ -100,6 +101,7 @@ _SIGS = {
     "emp_pt_get": (C.c_int, [_P, C.c_int32, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "emp_pt_set_state": (C.c_int, [_P, _P, _P]),
     "emp_pt_set_adaptation": (C.c_int, [_P, C.c_int32, C.c_double, C.c_double]),
+    "emp_pt_set_swap_path": (C.c_int, [_P, C.c_int32]),
     "emp_pt_set_moves": (C.c_int, [_P, _P, C.c_int32]),
     "emp_pt_set_gradient_steps": (C.c_int, [_P, _P]),
     "emp_pt_lnl_moments": (C.c_int, [_P, C.c_int64, _P, _P, _P]),

@@ -35,6 +35,7 @@ struct PTDev {
     int nsteps;         // stretch steps per sweep
     int64_t max_rows;   // history rows: one per step (max_sweeps x nsteps)
     int record;
+    int swap_stream;    // the swap rounds stream rows through LDS (the global-memory instance, W <= 1024)
     double* X;          // [T*W slots][D]
     double* lnl;        // [slots]
     double* lnp;        // [slots]
@@ -85,7 +86,8 @@ struct EmpPT {
     int wpt = 1;
     int grad_wpt = 2;            // waves per vector of the adjoint launches (one shape for both, so a
                                  // slot's gradient does not depend on which launch computed it)
-    size_t swap_lds = 0;         // dynamic LDS of the swap kernel, 0 = global path
+    size_t swap_lds = 0;         // dynamic LDS of the swap kernel, 0 = global-memory instance
+    size_t swap_lds_fit = 0;     // what the LDS instance needs when the ensemble fits, else 0
     long long host_sweeps = 0;   // mirror of the device counter
     long long hist_base = 0;     // first sweep whose history the engine holds (a restore from a
                                  // checkpoint without chain rows starts the history there)
@@ -223,11 +225,107 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
 // Adjacent-temperature swaps (hottest pair first), swap statistics, ladder
 // adaptation and the sweep counter.  One workgroup.  The pairings and
 // log-uniforms were drawn by the sweep's last prior pass.  With USE_LDS the
-// map, the per-position likelihoods and those randoms are staged in LDS;
-// otherwise the round works on them in global memory.
+// map, the per-position likelihoods and those randoms are staged in LDS.
+// Otherwise, for W up to SWAP_BLOCK, the rows stream through LDS
+// (swap_rounds_stream); for wider ensembles the round works in global memory.
 // ---------------------------------------------------------------------------
 constexpr int SWAP_BLOCK = 1024;
 constexpr int SWAP_UNROLL = 4;
+
+// The swap rounds of an ensemble whose positions do not fit LDS, W <=
+// SWAP_BLOCK.  Round h reads only temperatures h and h - 1, and temperature
+// h is final after it, so three rows of (lnL, map entry) rotate through LDS,
+// row t in buffer t mod 3: the pair's two rows and the next row, staged
+// while the pair is decided from loads issued a round earlier (as are the
+// pair's partner and log-uniform).  One barrier per round.  Thread i owns
+// position i of every row and pair i of every round; the decisions, the
+// pair statistics left in swap_lu / swap_jp and the final map equal those of
+// pt_swap_round.  Final rows go to the map and their likelihoods by position
+// to swap_l (the specific-heat spread reads them).
+__device__ inline void swap_rounds_stream(const PTDev& pd, const PTView& pv, double (*s_l)[SWAP_BLOCK],
+                                          int (*s_m)[SWAP_BLOCK], double (*s_red)[2], const double* bet) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = SWAP_BLOCK >> 6;
+    const int T = pd.T, W = pd.W;
+    const bool own = tid < W;
+    if (own) {
+        for (int t = T - 1; t >= 0 && t >= T - 2; --t) {
+            const int m = pd.map[(size_t)t * W + tid];
+            s_m[t % 3][tid] = m;
+            s_l[t % 3][tid] = pd.lnl[m];
+        }
+    }
+    int mreg = 0, jreg = 0;
+    double lreg = 0.0, ureg = 0.0;
+    if (own && T >= 3) {
+        mreg = pd.map[(size_t)(T - 3) * W + tid];
+        lreg = pd.lnl[mreg];
+    }
+    if (own && T >= 2) {
+        jreg = pd.swap_jp[(size_t)(T - 2) * W + tid];
+        ureg = pd.swap_lu[(size_t)(T - 2) * W + tid];
+    }
+    __syncthreads();
+    for (int h = T - 1; h >= 1; --h) {
+        const int c = h - 1, bh = h % 3, bc = c % 3;
+        int mnext = 0, jnext = 0;
+        double lnext = 0.0, unext = 0.0;
+        if (own) {
+            if (h >= 2) {                 // stage row h - 2, start loading row h - 3 and pair h - 1
+                s_m[(h - 2) % 3][tid] = mreg;
+                s_l[(h - 2) % 3][tid] = lreg;
+                jnext = pd.swap_jp[(size_t)(h - 2) * W + tid];
+                unext = pd.swap_lu[(size_t)(h - 2) * W + tid];
+            }
+            if (h >= 3) mnext = pd.map[(size_t)(h - 3) * W + tid];
+            const size_t e = (size_t)(h - 1) * W + tid;
+            const int jp = jreg;
+            const double lh = s_l[bh][tid], lc = s_l[bc][jp];
+            double lr = (bet[c] - bet[h]) * (lh - lc);
+            if (!(lr == lr)) lr = -__builtin_inf();
+            const bool acc = ureg < lr;
+            if (acc) {
+                s_l[bh][tid] = lc;
+                s_l[bc][jp] = lh;
+                const int mh = s_m[bh][tid];
+                s_m[bh][tid] = s_m[bc][jp];
+                s_m[bc][jp] = mh;
+            }
+            pd.swap_lu[e] = (lr > -__builtin_inf() && lr < __builtin_inf()) ? fabs(lr) : 0.0;
+            pd.swap_jp[e] = acc ? 1 : 0;
+            if (h >= 3) lnext = pd.lnl[mnext];
+        }
+        __syncthreads();
+        if (own) {                        // temperature h is final
+            pd.map[(size_t)h * W + tid] = s_m[bh][tid];
+            pd.swap_l[(size_t)h * W + tid] = s_l[bh][tid];
+        }
+        mreg = mnext;
+        lreg = lnext;
+        jreg = jnext;
+        ureg = unext;
+    }
+    if (own) {
+        pd.map[tid] = s_m[0][tid];
+        pd.swap_l[tid] = s_l[0][tid];
+    }
+    __threadfence();
+    __syncthreads();
+    for (int h = 1 + wave; h < T; h += nw) {      // one wave per temperature pair
+        double na = 0.0, di = 0.0;
+        for (int i = lane; i < W; i += 64) {
+            const size_t e = (size_t)(h - 1) * W + i;
+            na += (double)pd.swap_jp[e];
+            di += pd.swap_lu[e];
+        }
+        na = wave_sum(na);
+        di = wave_sum(di);
+        if (lane == 0) {
+            s_red[h - 1][0] = na;
+            s_red[h - 1][1] = di;
+        }
+    }
+    __syncthreads();
+}
 
 template <bool USE_LDS>
 __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
@@ -276,13 +374,23 @@ __global__ __launch_bounds__(SWAP_BLOCK) void k_pt_swap(PTDev pd) {
                 }
             }
         }
-    } else {
-        for (int e = tid; e < n; e += SWAP_BLOCK) L[e] = pd.lnl[pd.map[e]];
-        __threadfence();
     }
-    __syncthreads();
-    pt_swap_round(pv, sweep, L, Mp, USE_LDS ? LU : pd.swap_lu, USE_LDS ? JP : pd.swap_jp, !USE_LDS, s_red, s_bet,
-                  SWAP_BLOCK);
+    if constexpr (!USE_LDS) {
+        if (pd.swap_stream) {
+            __shared__ double s_sl[3][SWAP_BLOCK];
+            __shared__ int s_sm[3][SWAP_BLOCK];
+            __syncthreads();
+            swap_rounds_stream(pd, pv, s_sl, s_sm, s_red, s_bet);
+        } else {
+            for (int e = tid; e < n; e += SWAP_BLOCK) L[e] = pd.lnl[pd.map[e]];
+            __threadfence();
+            __syncthreads();
+            pt_swap_round(pv, sweep, L, Mp, pd.swap_lu, pd.swap_jp, true, s_red, s_bet, SWAP_BLOCK);
+        }
+    } else {
+        __syncthreads();
+        pt_swap_round(pv, sweep, L, Mp, LU, JP, false, s_red, s_bet, SWAP_BLOCK);
+    }
     if (tid < T - 1) {
         const double na = s_red[tid][0], di = s_red[tid][1];
         s_rate[tid] = na / W;
@@ -1040,7 +1148,8 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     // inside the default 64 KiB workgroup allocation); otherwise the same
     // kernel works in global memory and draws the randoms inline.
     const size_t lds = (TW + (size_t)(v.T - 1) * v.W) * (sizeof(double) + sizeof(int));
-    p->swap_lds = lds <= 62 * 1024 ? lds : 0;
+    p->swap_lds = p->swap_lds_fit = lds <= 62 * 1024 ? lds : 0;
+    v.swap_stream = (!p->swap_lds && v.W <= SWAP_BLOCK) ? 1 : 0;
 #define EMP_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { int r_ = emp_hip_fail(e_, #x, __FILE__, __LINE__); emp_pt_destroy(p); return r_; } } while (0)
     EMP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     EMP_TRY(hipMemcpy(v.X, p0, TW * v.D * sizeof(double), hipMemcpyHostToDevice));
@@ -1329,6 +1438,37 @@ extern "C" int emp_pt_get(EmpPT* p, int32_t what, void* host_out, int64_t cap, i
 norec:
     emp_set_error("history was not recorded (record == 0)");
     return EMP_EINVAL;
+}
+
+extern "C" int emp_pt_set_swap_path(EmpPT* p, int32_t path) {
+    if (!p) { emp_set_error("null engine"); return EMP_EINVAL; }
+    PTDev& v = p->dev;
+    size_t lds = 0;
+    int stream = 0;
+    switch (path) {
+        case EMP_SWAP_AUTO:
+            lds = p->swap_lds_fit;
+            stream = (!lds && v.W <= SWAP_BLOCK) ? 1 : 0;
+            break;
+        case EMP_SWAP_LDS:
+            if (!p->swap_lds_fit) { emp_set_error("the ensemble does not fit the LDS swap round"); return EMP_EINVAL; }
+            lds = p->swap_lds_fit;
+            break;
+        case EMP_SWAP_STREAM:
+            if (v.W > SWAP_BLOCK) { emp_set_error("the streamed swap round takes at most %d walkers", SWAP_BLOCK); return EMP_EINVAL; }
+            stream = 1;
+            break;
+        case EMP_SWAP_GLOBAL:
+            break;
+        default:
+            emp_set_error("swap path must be EMP_SWAP_AUTO, _LDS, _STREAM or _GLOBAL");
+            return EMP_EINVAL;
+    }
+    EMP_HIP(hipStreamSynchronize(p->stream));
+    p->swap_lds = lds;
+    v.swap_stream = stream;
+    drop_graphs(p);        // kernel and arguments changed: capture again on the next run
+    return EMP_OK;
 }
 
 extern "C" int emp_pt_set_adaptation(EmpPT* p, int32_t mode, double tau, double nu) {

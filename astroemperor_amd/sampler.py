@@ -414,6 +414,13 @@ class PTSampler:
         _abi.check(_abi.lib().emp_pt_last_timing(self._pt, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def set_swap_path(self, path="auto"):
+        """Swap-round implementation ('auto', 'lds', 'stream' or 'global'; the
+        results are identical): for comparisons and tests."""
+        code = {"auto": _abi.EMP_SWAP_AUTO, "lds": _abi.EMP_SWAP_LDS, "stream": _abi.EMP_SWAP_STREAM,
+                "global": _abi.EMP_SWAP_GLOBAL}[path]
+        _abi.check(_abi.lib().emp_pt_set_swap_path(self._pt, code))
+
     def autotune(self, trials=10):
         """Pick the fastest waves per walker from timed graph-replayed sweeps
         (state left unchanged); returns (waves_per_walker, ms per sweep)."""

@@ -251,6 +251,20 @@ int emp_pt_get(EmpPT* pt, int32_t what, void* host_out, int64_t capacity_bytes, 
  * p: [T, W, ndim] host or NULL; betas: [T] host or NULL. */
 int emp_pt_set_state(EmpPT* pt, const double* p, const double* betas);
 
+/* Swap-round implementation.  No counterpart in the reference; the three
+ * give identical results (same decisions, statistics and map) and are for
+ * comparisons and tests:
+ *   EMP_SWAP_AUTO: in LDS when the map, likelihoods and pre-drawn pairs fit
+ *     (12 B per position and per pair, up to 62 KiB), else streamed rows for
+ *     at most 1024 walkers, else global memory;
+ *   EMP_SWAP_LDS, EMP_SWAP_STREAM (three temperature rows at a time in LDS),
+ *   EMP_SWAP_GLOBAL: that path, EMP_EINVAL when the ensemble does not allow it. */
+#define EMP_SWAP_AUTO 0
+#define EMP_SWAP_LDS 1
+#define EMP_SWAP_STREAM 2
+#define EMP_SWAP_GLOBAL 3
+int emp_pt_set_swap_path(EmpPT* pt, int32_t path);
+
 /* Replaces: sampler.select_adjustment('00') (support/endit_freeze1.scr:11),
  * which freezes the ladder; mode -1 = frozen, 0/1/2 as EmpPTDesc.adapt_mode. */
 int emp_pt_set_adaptation(EmpPT* pt, int32_t adapt_mode, double adapt_tau, double adapt_nu);

@@ -256,6 +256,36 @@ def test_graph_equals_direct():
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("mode,T,W", [(2, 6, 64), (0, 4, 1024)])
+def test_swap_paths_agree(mode, T, W):
+    """The swap round in LDS, with the temperature rows streamed through LDS,
+    and in global memory: the same decisions, statistics, ladder and map,
+    bit for bit (the specific-heat ladder reads the post-swap likelihoods
+    each path leaves by position)."""
+    from astroemperor_amd import PTSampler, _abi
+    from astroemperor_amd.simulation import test_init
+    m = make_sim("51Peg", 1, parameterisation=1).model
+    p0 = test_init(m, T, W, np.random.default_rng(8))
+    out = {}
+    for path in ("lds", "stream", "global"):
+        s = PTSampler(W, m.ndim__, m, ntemps=T, seed=21, adapt_tau=50, adapt_nu=2, adapt_mode=mode)
+        s.run_mcmc(p0, nsweeps=1, capacity=13)
+        try:
+            s.set_swap_path(path)
+        except _abi.EmpError:
+            assert path == "lds" and T * W + (T - 1) * W > 62 * 1024 // 12    # does not fit LDS
+            s.close()
+            continue
+        s.run_mcmc(None, nsweeps=12)
+        out[path] = (s.get_chain(), s.get_log_like(), s.get_betas(), s.get_tsw(), s.get_smd())
+        s.close()
+    assert "stream" in out and "global" in out
+    ref = out["global"]
+    for path, got in out.items():
+        for x, y in zip(got, ref):
+            assert np.array_equal(x, y), path
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_ladder_adaptation_matches_oracle(mode):
     """Each sweep's ladder update from that sweep's statistics: the swap
