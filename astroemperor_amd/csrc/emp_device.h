@@ -179,8 +179,10 @@ __device__ inline PlanetC planet_setup(const double* th, int param, int use_c) {
     return pc;
 }
 
-__device__ __forceinline__ double planet_M(const PlanetC& pc, double t) {
-    return pc.tp_mode ? (t - pc.tp) * pc.freq : pc.freq * t + pc.phase;
+// tp: the model's planets are parameterised by the time of periastron
+// (model_tp), a kernel-argument scalar, so the choice is a scalar branch.
+__device__ __forceinline__ double planet_M(const PlanetC& pc, double t, bool tp) {
+    return tp ? (t - pc.tp) * pc.freq : pc.freq * t + pc.phase;
 }
 
 // ----------------------------------------------------------------------------
@@ -393,19 +395,20 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
 }
 
 // (sin E, cos E) of the planet at time t.  PREC 0: FP64 Markley; PREC 1:
-// mixed, with `kt` the planet's per-vector table (read only when pc.ktab is
-// set, a per-vector, hence wave-uniform, branch).
+// mixed, with `kt` the planet's per-vector table, read when `ktab` (the
+// planet's bit of SlotLDS::ktab_mask, passed as a wave-uniform scalar so
+// that the choice is a scalar branch) is set.  tp as in planet_M.
 template <int PREC>
 __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
-                                          double& sE, double& cE) {
-    const double M = reduce_2pi(planet_M(pc, t));
+                                          double& sE, double& cE, bool tp, bool ktab) {
+    const double M = reduce_2pi(planet_M(pc, t, tp));
     if (PREC == 0) {
         kepler_fp64(M, pc, tb, sE, cE);
         return;
     }
     const double Ma = __builtin_fabs(M);
     const float Mf = (float)Ma;
-    if (pc.ktab) kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
+    if (ktab) kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
     else kepler_fp64_step<true>(Ma, kepler_fp32_start(Mf, pc), pc.e, tb, sE, cE);
     if (M < 0.0) sE = -sE;
 }
@@ -418,9 +421,10 @@ __device__ __forceinline__ double rcp_path(double x) {
 
 // Planet term at time t (see planet_sc).
 template <int PREC>
-__device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t) {
+__device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
+                                           bool tp, bool ktab) {
     double sE, cE;
-    planet_sc<PREC>(pc, tb, kt, t, sE, cE);
+    planet_sc<PREC>(pc, tb, kt, t, sE, cE, tp, ktab);
     const double den = __builtin_fma(-pc.e, cE, 1.0);
     return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp_path<PREC>(den);
 }

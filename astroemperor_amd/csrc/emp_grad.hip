@@ -72,9 +72,9 @@ struct GradWaveLDS {
 // D cos f = cos E - e, D sin f = b sin E.
 template <int PREC>
 __device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab& tab, const float2* kt, double t,
-                                             double w, double (&a)[8]) {
+                                             double w, double (&a)[8], bool tp, bool ktab) {
     double sE, cE;
-    planet_sc<PREC>(pc, tab, kt, t, sE, cE);
+    planet_sc<PREC>(pc, tab, kt, t, sE, cE, tp, ktab);
     const double e = pc.e;
     const double rD = rcp_path<PREC>(__builtin_fma(-e, cE, 1.0));
     const double u = cE - e, v = pc.bq * sE;
@@ -83,7 +83,7 @@ __device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab&
     const double wA = w * pc.K;
     const double ws = wA * sfw;
     const double dM = -(ws * pc.bq) * (rD * rD);
-    const double tau = pc.tp_mode ? t - pc.tp : t;
+    const double tau = tp ? t - pc.tp : t;
     a[0] = __builtin_fma(dM, tau, a[0]);
     a[1] += dM;
     a[2] = __builtin_fma(-ws, sE * rD * __builtin_fma(pc.bq, rD, pc.ibq), __builtin_fma(wA, pc.cw, a[2]));
@@ -428,12 +428,13 @@ __device__ __forceinline__ double ma_chunk_w(const ModelDev& md, const SlotLDS& 
 // post-MA terms.
 template <int PREC, int EXT>
 __device__ __forceinline__ double plain_w(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, double* wacc,
-                                          InsAcc& ia, int j, int tb, double& chi, double& prod, int& pexp) {
+                                          InsAcc& ia, int j, int tb, double& chi, double& prod, int& pexp,
+                                          int kmask) {
     using GW = GradWave<EXT>;
     constexpr bool POST = EXT >= 1;
     const double t = tl.t[j];
     const int k = tl.ins[j];
-    const double mu = epoch_mu<PREC>(md, sl, tl.tab, t, k);
+    const double mu = epoch_mu<PREC>(md, sl, tl.tab, t, k, kmask);
     const double err2 = tl.s2[j] + sl.jit2[k];
     const double cs = (POST && has_post(md)) ? epoch_post(md, sl, tl.tab, t, tb + j, k) : 0.0;
     const double r = tl.y[j] - mu - cs;
@@ -605,6 +606,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     tab_load(md, s_tile);
     __syncthreads();
     slot_prepare<PREC, WPT>(md, sl, s_tile, live, wsub, lane);
+    const int kmask = wave_uniform(sl.ktab_mask);
+    const bool tp = model_tp(md);
 
     const bool ma = md.ma_active;
     double* eps = ma ? eps_scratch + (live ? b : 0) * (int64_t)md.N : nullptr;
@@ -633,7 +636,8 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
                 const int j = c * G + g;
-                wv[c] = (live && j < tn) ? plain_w<PREC, EXT>(md, sl, s_tile, wacc, ia, j, tb, chi, prod, pexp) : 0.0;
+                wv[c] = (live && j < tn) ? plain_w<PREC, EXT>(md, sl, s_tile, wacc, ia, j, tb, chi, prod, pexp, kmask)
+                                         : 0.0;
             }
         } else {
 #pragma unroll
@@ -652,7 +656,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
 #pragma unroll
                 for (int c = 0; c < CPT; ++c) {
                     const int j = c * G + g;
-                    if (j < tn) planet_wgrad<PREC>(pc, s_tile.tab, sl.ktab[p], s_tile.t[j], wv[c], a);
+                    if (j < tn) planet_wgrad<PREC>(pc, s_tile.tab, sl.ktab[p], s_tile.t[j], wv[c], a, tp, (kmask >> p) & 1);
                 }
                 wave_reduce_add8(a, 5, wnat + 5 * p, lane);
             }
@@ -857,6 +861,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
         for (int p = wave; p < md.K; p += NW)
             if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
     __syncthreads();
+    const int kmask = wave_uniform(sl.ktab_mask);
+    const bool tp = model_tp(md);
 
     const bool ma = md.ma_active;
     double chi = 0.0, prod = 1.0;
@@ -893,7 +899,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
                 sE[p] = cE[p] = 0.0;
                 if (p < md.K) {
                     const PlanetC& pc = sl.planet[p];
-                    planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p]);
+                    planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p], tp, (kmask >> p) & 1);
                     mu += __builtin_fma(pc.c1, cE[p], pc.c2 * sE[p])
                           * rcp_path<PREC>(__builtin_fma(-pc.e, cE[p], 1.0));
                 }
@@ -1024,7 +1030,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
                     const double cfw = __builtin_fma(uu, pc.cw, -(vv * pc.sw)) * rD; // cos(f + w)
                     const double Ks = pc.K * sfw;
                     const double fM = -(Ks * pc.bq) * (rD * rD);
-                    f[0] = fM * (pc.tp_mode ? t - pc.tp : t);
+                    f[0] = fM * (tp ? t - pc.tp : t);
                     f[1] = fM;
                     f[2] = __builtin_fma(-Ks, sE[p] * rD * __builtin_fma(pc.bq, rD, pc.ibq), pc.K * pc.cw);
                     f[3] = -pc.K * __builtin_fma(e, pc.sw, sfw);

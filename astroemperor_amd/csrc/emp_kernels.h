@@ -61,11 +61,15 @@ __device__ inline double prior_seq(const ModelDev& md, const double* th) {
 // Per-slot constants (planets, instruments, acceleration) into LDS.
 // Called by the slot's first wave; the caller synchronises afterwards.
 __device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
+    bool kt = false;
     if (lane < md.K) {
         PlanetC pc = planet_setup(sl.theta + md.off_kep + 5 * lane, md.param, md.use_c);
         pc.ktab = md.N >= KT_MIN_EPOCHS && pc.e <= KT_EMAX;
+        kt = pc.ktab != 0;
         sl.planet[lane] = pc;
     }
+    const unsigned long long kb = __ballot(kt);       // the whole wave calls this
+    if (lane == 0) sl.ktab_mask = (int)(kb & 0xffu);
     if (lane < md.I) {
         sl.gam[lane] = sl.theta[md.off_off + lane];
         double j = md.has_jit ? sl.theta[md.off_jit + lane] : 0.0;
@@ -164,9 +168,20 @@ __device__ __forceinline__ void slot_prepare(const ModelDev& md, SlotLDS& sl, co
     __syncthreads();
 }
 
-// Model value (planets + offset + acceleration) of one epoch.
+// A value that is the same in every lane of the wave (every kernel keeps
+// one parameter vector per wave), read into a scalar register so that a
+// branch on it is a scalar branch.
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// The model's planets are parameterised by the time of periastron
+// (block_repo.py:23-31, parameterisations 3 and 4).
+__device__ __forceinline__ bool model_tp(const ModelDev& md) { return md.param == 3 || md.param == 4; }
+
+// Model value (planets + offset + acceleration) of one epoch.  kmask: the
+// slot's SlotLDS::ktab_mask as a wave-uniform scalar.
 template <int PREC>
-__device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t, int k) {
+__device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t, int k,
+                                           int kmask) {
     double mu = sl.gam[k];
     if (md.acc) {
         const double x = t - md.t0;
@@ -174,7 +189,8 @@ __device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl
         for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
         mu += v * x;
     }
-    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t);
+    const bool tp = model_tp(md);
+    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t, tp, (kmask >> p) & 1);
     return mu;
 }
 
@@ -423,6 +439,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
     const int wave = threadIdx.x >> 6;
     const int G = 64 * WPT;
     const int g = wsub * 64 + lane;
+    const int kmask = wave_uniform(sl.ktab_mask);
     double chi = 0.0, prod = 1.0;
     int pexp = 0;
     double carry = 0.0;
@@ -461,7 +478,7 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
             if (valid) {
                 k = tl.ins[j];
                 tj = tl.t[j];
-                mu = epoch_mu<PREC>(md, sl, tl.tab, tj, k);
+                mu = epoch_mu<PREC>(md, sl, tl.tab, tj, k, kmask);
                 err2 = tl.s2[j] + sl.jit2[k];
                 r = tl.y[j] - mu;
             }
