@@ -145,18 +145,27 @@ def _state_dlnl(model, coords):
 # ---------------------------------------------------------------------------
 # Headline: the large mixed workload.
 # ---------------------------------------------------------------------------
-def run_c5(seed, steps, warmup, adapt_sweeps=16, precisions=("mixed", "fp64"), oracle_n=16):
+def run_c5(seed, steps, warmup, adapt_sweeps=16, precisions=("mixed", "fp64"), oracle_n=16, init="prior"):
     from astroemperor_amd import GradientMove, PTSampler, StretchMove, synth
     from astroemperor_amd.simulation import test_init
     work = synth.c5()
     m = work["model"]
     T, W, _, _ = work["setup"]
-    p0 = test_init(m, T, W, np.random.default_rng(seed))
+    map_info = None
+    if init == "map":
+        from astroemperor_amd.autograd import map_init
+        t_map = time.perf_counter()
+        p0, _, lp_map = map_init(m, T, W, np.random.default_rng(seed))
+        map_info = {"seconds": round(time.perf_counter() - t_map, 2), "ln_posterior_at_mode": round(float(lp_map), 3)}
+    else:
+        p0 = test_init(m, T, W, np.random.default_rng(seed))
     mix = [(StretchMove(), 4.0), (GradientMove(), 1.0)]
     out = {"workload": f"7 Keplerians, 5 instruments, {m.data.ndat} epochs, per-instrument MA(1), {T} temps x "
                        f"{W} walkers, gradient (Langevin) sweeps mixed 1:4 with stretch sweeps",
            "ndim": int(m.ndim__), "ntemps": T, "nwalkers": W, "adapt_sweeps": adapt_sweeps,
-           "default_gradient_step": GradientMove().scale(m.ndim__)}
+           "default_gradient_step": GradientMove().scale(m.ndim__), "init": init}
+    if map_info:
+        out["map_init"] = map_info
     flops_l = {p: flops_per_theta(m, p) for p in ("fp64", "mixed")}
     flops_g = {p: flops_grad_per_theta(m, p) for p in ("fp64", "mixed")}
     states = {}
@@ -544,9 +553,11 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="timed PT sweeps of the headline workload")
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=100, help="untimed sweeps of the mix before the step adaptation")
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--adapt-sweeps", type=int, default=16, help="gradient-step adaptation sweeps (burn-in)")
+    ap.add_argument("--adapt-sweeps", type=int, default=32, help="gradient-step adaptation sweeps (burn-in)")
+    ap.add_argument("--init", choices=["prior", "map"], default="prior",
+                    help="headline walkers from the reference's prior draws (test_init) or around the MAP optimum")
     ap.add_argument("--cpu-sweeps", type=int, default=20)
     ap.add_argument("--cpu-cores", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -588,7 +599,7 @@ def main():
         return
 
     precisions = ("mixed",) if args.only == "c5-mixed" else ("mixed", "fp64")
-    c5, m = run_c5(args.seed, args.steps, args.warmup, args.adapt_sweeps, precisions)
+    c5, m = run_c5(args.seed, args.steps, args.warmup, args.adapt_sweeps, precisions, init=args.init)
     if args.only == "c5-mixed":
         print(json.dumps({"c5": c5, "device": info}), flush=True)
         return
