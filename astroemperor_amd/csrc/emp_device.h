@@ -302,16 +302,18 @@ constexpr double KT_EMAX = 0.5;
 // about 256 epochs on.
 constexpr int KT_MIN_EPOCHS = 256;
 
+// The cubic Hermite interpolant on [M_j, M_j+1] in Horner form: with
+// d = E_j+1 - E_j and the scaled slopes s0 = h E'_j, s1 = h E'_j+1,
+//   E(x) = E_j + x (s0 + x (3d - 2 s0 - s1 + x (s0 + s1 - 2d))).
 __device__ __forceinline__ float ktab_start(const float2* kt, float Mf) {
     const float u = Mf * (float)((KT_N - 1) / PI_D);
     const int j = min((int)u, KT_N - 2);
     const float x = u - (float)j;
     const float2 a = kt[j], b = kt[j + 1];
-    const float x2 = x * x;
-    const float h01 = x2 * __builtin_fmaf(-2.0f, x, 3.0f);      // 3x^2 - 2x^3
-    const float h11 = x2 * (x - 1.0f);                          // x^3 - x^2
-    const float h10 = __builtin_fmaf(x, (x - 1.0f) * (x - 1.0f), 0.0f);   // x (1 - x)^2
-    return __builtin_fmaf(b.x - a.x, h01, a.x) + __builtin_fmaf(a.y, h10, b.y * h11);
+    const float d = b.x - a.x, ss = a.y + b.y;
+    const float c3 = __builtin_fmaf(-2.0f, d, ss);              // s0 + s1 - 2d
+    const float c2 = __builtin_fmaf(3.0f, d, -(a.y + ss));      // 3d - 2 s0 - s1
+    return __builtin_fmaf(x, __builtin_fmaf(x, __builtin_fmaf(x, c3, c2), a.y), a.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -410,7 +412,9 @@ __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb
     const float Mf = (float)Ma;
     if (ktab) kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
     else kepler_fp64_step<true>(Ma, kepler_fp32_start(Mf, pc), pc.e, tb, sE, cE);
-    if (M < 0.0) sE = -sE;
+    // E(-M) = -E(M), and sin E >= 0 on [0, pi]: the sign of M, one bit insert
+    // (a rounding-level negative sE at E ~ 0 or pi comes back positive).
+    sE = __builtin_copysign(sE, M);
 }
 
 // Reciprocal of the path: two Newton steps (FP64) or one (mixed).

@@ -829,7 +829,7 @@ struct Grad1LDS {
 // after the MA.  NW waves per workgroup (one vector), tiles of 64 NW epochs.
 // `stale` as in k_lnl_grad.
 template <int PREC, int EXT, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_lnl_grad1(ModelDev md, const double* __restrict__ theta, int64_t B,
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 0 ? 4 : 3))) void k_lnl_grad1(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                        int64_t ld, double* __restrict__ out_lnl,
                                                        double* __restrict__ out_grad, unsigned long long* nonfinite,
                                                        int* stale) {
