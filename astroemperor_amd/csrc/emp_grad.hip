@@ -119,7 +119,7 @@ __device__ __forceinline__ double swap_add16(double x, double y) {
 // (DPP), and the remaining value is summed over each aligned group of eight
 // lanes.  Lane 8 i then holds the total of value i; the first n are added to
 // dst[i].  Whole wave active.
-__device__ __forceinline__ void wave_reduce_add8(double (&v)[8], int n, double* dst, int lane) {
+__device__ __forceinline__ double wave_reduce8(const double (&v)[8], int lane) {
     double a[4], b[2];
     const bool h3 = lane & 8;
 #pragma unroll
@@ -130,6 +130,11 @@ __device__ __forceinline__ void wave_reduce_add8(double (&v)[8], int n, double* 
     c += dpp_f64<0x141, 0xf>(0.0, c);   // row_half_mirror: lane i with 7 - i of its group of 8
     c += dpp_f64<0xB1, 0xf>(0.0, c);    // quad_perm [1, 0, 3, 2]
     c += dpp_f64<0x4E, 0xf>(0.0, c);    // quad_perm [2, 3, 0, 1]
+    return c;
+}
+
+__device__ __forceinline__ void wave_reduce_add8(double (&v)[8], int n, double* dst, int lane) {
+    const double c = wave_reduce8(v, lane);
     const int i = lane >> 3;
     if ((lane & 7) == 0 && i < n) dst[i] += c;
 }
@@ -716,35 +721,6 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
 // over the epochs; no eps row is stored.
 // ---------------------------------------------------------------------------
 
-// Wave sums of two groups of five per-lane values, b[0..4] and a[0..4] (the
-// adjoint's B and A parts), as one scattered reduction of sixteen values in
-// which slots 5-7 and 13-15 are zero: halves are exchanged across lanes 32
-// and 16 apart (permlane swaps: each lane keeps the half its lane bit
-// selects and adds its partner's), then across lanes 8 apart and with the
-// lane mirrored in its group of eight (lane ^ 7, which flips bit 2) by DPP,
-// and the last value is summed over each quad.  Lane 4 i then holds the
-// total of b[i] and lane 32 + 4 i that of a[i].  Whole wave active.
-__device__ __forceinline__ double wave_reduce_ba5(const double (&bv)[5], const double (&av)[5], int lane) {
-    const bool h3 = lane & 8, h2 = lane & 4;
-    double a[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) a[i] = swap_add32(bv[i], av[i]);     // slot i below, i + 8 above
-    double b[4];
-    b[0] = swap_add16(a[0], a[4]);
-#pragma unroll
-    for (int i = 1; i < 4; ++i) b[i] = swap_add16(a[i], 0.0);          // slot i + 4 is zero
-    double c[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const double keep = h3 ? b[i + 2] : b[i], send = h3 ? b[i] : b[i + 2];
-        c[i] = keep + dpp_f64<0x128, 0xf>(0.0, send);   // row_ror 8: lane ^ 8
-    }
-    double d = (h2 ? c[1] : c[0]) + dpp_f64<0x141, 0xf>(0.0, h2 ? c[0] : c[1]);   // row_half_mirror
-    d += dpp_f64<0xB1, 0xf>(0.0, d);    // quad_perm [1, 0, 3, 2]
-    d += dpp_f64<0x4E, 0xf>(0.0, d);    // quad_perm [2, 3, 0, 1]
-    return d;
-}
-
 // Per-lane sums of the per-instrument terms (offset, jitter, MA(1)
 // coefficient and timescale) of the lane's epochs, the offset and MA sums
 // with their R states; added to the wave's LDS totals and R states when the
@@ -951,6 +927,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
         }
         // dlnL/dmu = wB + wA Lambda; lambda = lB + lA Lambda; (tA, tB) the tile's map.
         double wB = 0.0, wA = 0.0, tA = 0.0, tB = 0.0;
+        bool needA = false;
         if (!ma) {
             wB = ef * ie;
             ia.off += wB;
@@ -988,6 +965,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                 wB = -lB;
                 wA = -lA;
             }
+            // A wave whose every |A_i| is below 2^-100 adds nothing measurable
+            // through Lambda (the terms are under 1e-17 of any gradient the
+            // data can give): it reduces the B parts alone.  With |a| <= 1/3,
+            // every wave of a tile but the last is such a wave.
+            needA = __any(valid && __builtin_fabs(lA) > 0x1p-100);
             // the sums this wave already holds in LDS move to the next tile
             for (int e = lane; e < GW::COS; e += 64) {
                 const double R0 = wrst[e];
@@ -1040,19 +1022,19 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                     double a8[8] = {wB * f[0], wB * f[1], wB * f[2], wB * f[3], wB * f[4], 0.0, 0.0, 0.0};
                     wave_reduce_add8(a8, 5, wnat + 5 * p, lane);
                 } else {
-                    double bv[5], av[5];
-#pragma unroll
-                    for (int i = 0; i < 5; ++i) {
-                        bv[i] = wB * f[i];
-                        av[i] = wA * f[i];
+                    // the B parts, then the A parts only where some lane's A is not negligible
+                    const double b8[8] = {wB * f[0], wB * f[1], wB * f[2], wB * f[3], wB * f[4], 0.0, 0.0, 0.0};
+                    const double sb = wave_reduce8(b8, lane);
+                    double sa = 0.0;
+                    if (needA) {
+                        const double a8[8] = {wA * f[0], wA * f[1], wA * f[2], wA * f[3], wA * f[4], 0.0, 0.0, 0.0};
+                        sa = wave_reduce8(a8, lane);
                     }
-                    const double s = wave_reduce_ba5(bv, av, lane);
-                    const double sA = __shfl_down(s, 32, 64);
-                    if (lane < 20 && (lane & 3) == 0) {
-                        const int e = 5 * p + (lane >> 2);
+                    if (lane < 40 && (lane & 7) == 0) {
+                        const int e = 5 * p + (lane >> 3);
                         const double R0 = wnatR[e];
-                        wnat[e] += __builtin_fma(tB, R0, s);
-                        wnatR[e] = __builtin_fma(tA, R0, sA);
+                        wnat[e] += __builtin_fma(tB, R0, sb);
+                        wnatR[e] = __builtin_fma(tA, R0, sa);
                     }
                 }
             }
@@ -1074,19 +1056,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                 double a8[8] = {wB * xq[0], wB * xq[1], wB * xq[2], wB * xq[3], 0.0, 0.0, 0.0, 0.0};
                 wave_reduce_add8(a8, md.acc, wnat + WN_ACC, lane);
             } else {
-                double bv[5], av[5];
-#pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    bv[q] = q < AMAX ? wB * xq[q] : 0.0;
-                    av[q] = q < AMAX ? wA * xq[q] : 0.0;
+                const double b8[8] = {wB * xq[0], wB * xq[1], wB * xq[2], wB * xq[3], 0.0, 0.0, 0.0, 0.0};
+                const double sb = wave_reduce8(b8, lane);
+                double sa = 0.0;
+                if (needA) {
+                    const double a8[8] = {wA * xq[0], wA * xq[1], wA * xq[2], wA * xq[3], 0.0, 0.0, 0.0, 0.0};
+                    sa = wave_reduce8(a8, lane);
                 }
-                const double s = wave_reduce_ba5(bv, av, lane);
-                const double sA = __shfl_down(s, 32, 64);
-                if ((lane & 3) == 0 && (lane >> 2) < md.acc) {
-                    const int e = WN_ACC + (lane >> 2);
+                if ((lane & 7) == 0 && (lane >> 3) < md.acc) {
+                    const int e = WN_ACC + (lane >> 3);
                     const double R0 = wnatR[e];
-                    wnat[e] += __builtin_fma(tB, R0, s);
-                    wnatR[e] = __builtin_fma(tA, R0, sA);
+                    wnat[e] += __builtin_fma(tB, R0, sb);
+                    wnatR[e] = __builtin_fma(tA, R0, sa);
                 }
             }
         }
