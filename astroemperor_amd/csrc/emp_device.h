@@ -48,6 +48,18 @@ __device__ __forceinline__ double dpp_f64(double old, double src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// The same for controls under which every lane reads a lane of its own row
+// (row_ror, row_half_mirror = 0x141, quad_perm) with every row enabled: no
+// lane keeps an old value, so none is passed (and no register is cleared
+// for one).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_f64(double src) {
+    const long long s = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_mov_dpp((int)s, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(s >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Value of lane k (uniform k) in every lane.
 __device__ __forceinline__ double readlane_f64(double v, int k) {
     const long long bits = __double_as_longlong(v);

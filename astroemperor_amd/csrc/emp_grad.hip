@@ -72,7 +72,7 @@ struct GradWaveLDS {
 // D cos f = cos E - e, D sin f = b sin E.
 template <int PREC>
 __device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab& tab, const float2* kt, double t,
-                                             double w, double (&a)[8], bool tp, bool ktab) {
+                                             double w, double (&a)[5], bool tp, bool ktab) {
     double sE, cE;
     planet_sc<PREC>(pc, tab, kt, t, sE, cE, tp, ktab);
     const double e = pc.e;
@@ -113,30 +113,61 @@ __device__ __forceinline__ double swap_add16(double x, double y) {
            __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
 }
 
-// Sum of eight per-lane values over the wave, scattered: halves of the
-// values are exchanged across lanes 32 apart, then 16 (permlane swaps: each
-// lane keeps the half its lane bit selects and adds its partner's), then 8
-// (DPP), and the remaining value is summed over each aligned group of eight
-// lanes.  Lane 8 i then holds the total of value i; the first n are added to
-// dst[i].  Whole wave active.
-__device__ __forceinline__ double wave_reduce8(const double (&v)[8], int lane) {
-    double a[4], b[2];
+// Sum of five per-lane values over the wave, scattered: the values are
+// paired across lanes 32 apart as (v0, v4), (v1, v2), (v3, v3) (permlane
+// swaps: each lane keeps the member its lane bit selects, summed with its
+// partner's), those across lanes 16 apart as (v0|v4, v1|v2) and (v3, v3),
+// the two results across lanes 8 apart (DPP), and the remaining value is
+// summed over each aligned group of eight lanes.  The totals land at lanes
+// 0 (v0), 16 (v1), 48 (v2), 8 (v3) and 32 (v4); red5_index(lane) names the
+// value a lane holds, -1 for lanes that hold none.  Whole wave active.
+__device__ __forceinline__ double wave_reduce5(const double (&v)[5], int lane) {
+    const double a0 = swap_add32(v[0], v[4]);
+    const double a1 = swap_add32(v[1], v[2]);
+    const double a2 = swap_add32(v[3], v[3]);
+    const double b0 = swap_add16(a0, a1);
+    const double b1 = swap_add16(a2, a2);
     const bool h3 = lane & 8;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = swap_add32(v[i], v[i + 4]);
-#pragma unroll
-    for (int i = 0; i < 2; ++i) b[i] = swap_add16(a[i], a[i + 2]);
-    double c = (h3 ? b[1] : b[0]) + dpp_f64<0x128, 0xf>(0.0, h3 ? b[0] : b[1]);   // row_ror 8: lane ^ 8
-    c += dpp_f64<0x141, 0xf>(0.0, c);   // row_half_mirror: lane i with 7 - i of its group of 8
-    c += dpp_f64<0xB1, 0xf>(0.0, c);    // quad_perm [1, 0, 3, 2]
-    c += dpp_f64<0x4E, 0xf>(0.0, c);    // quad_perm [2, 3, 0, 1]
+    double c = (h3 ? b1 : b0) + dpp_mov_f64<0x128>(h3 ? b0 : b1);   // row_ror 8: lane ^ 8
+    c += dpp_mov_f64<0x141>(c);    // row_half_mirror: lane i with 7 - i of its group of 8
+    c += dpp_mov_f64<0xB1>(c);     // quad_perm [1, 0, 3, 2]
+    c += dpp_mov_f64<0x4E>(c);     // quad_perm [2, 3, 0, 1]
     return c;
 }
 
-__device__ __forceinline__ void wave_reduce_add8(double (&v)[8], int n, double* dst, int lane) {
-    const double c = wave_reduce8(v, lane);
-    const int i = lane >> 3;
-    if ((lane & 7) == 0 && i < n) dst[i] += c;
+// Nibble g of 0xF2F4F130 is the value lane 8 g holds (F: none).
+__device__ __forceinline__ int red5_index(int lane) {
+    const int i = (int)((0xF2F4F130u >> (4 * (lane >> 3))) & 0xFu);
+    return ((lane & 7) || i == 0xF) ? -1 : i;
+}
+
+// Four values: (v0, v1) and (v2, v3) across lanes 32 apart, the pair across
+// lanes 16 apart, then summed over each row of 16 lanes.  The totals land at
+// lanes 0 (v0), 32 (v1), 16 (v2) and 48 (v3): red4_index(lane).
+__device__ __forceinline__ double wave_reduce4(const double (&v)[4]) {
+    const double a0 = swap_add32(v[0], v[1]);
+    const double a1 = swap_add32(v[2], v[3]);
+    double c = swap_add16(a0, a1);
+    c += dpp_mov_f64<0x128>(c);
+    c += dpp_mov_f64<0x141>(c);
+    c += dpp_mov_f64<0xB1>(c);
+    c += dpp_mov_f64<0x4E>(c);
+    return c;
+}
+
+__device__ __forceinline__ int red4_index(int lane) {
+    return (lane & 15) ? -1 : (int)((0x3120u >> (4 * (lane >> 4))) & 0xFu);
+}
+
+// The five sums added to dst[0..4], the first n of four to dst[0..n).
+__device__ __forceinline__ void wave_reduce_add5(const double (&v)[5], double* dst, int lane, int ri) {
+    const double c = wave_reduce5(v, lane);
+    if (ri >= 0) dst[ri] += c;
+}
+
+__device__ __forceinline__ void wave_reduce_add4(const double (&v)[4], int n, double* dst, int ri4) {
+    const double c = wave_reduce4(v);
+    if (ri4 >= 0 && ri4 < n) dst[ri4] += c;
 }
 
 // Adds w = dlnL/dS_n times the derivatives of the terms added after the MA
@@ -656,17 +687,17 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
         }
         if (live) {
             for (int p = 0; p < md.K; ++p) {
-                double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
                 const PlanetC& pc = sl.planet[p];
 #pragma unroll
                 for (int c = 0; c < CPT; ++c) {
                     const int j = c * G + g;
                     if (j < tn) planet_wgrad<PREC>(pc, s_tile.tab, sl.ktab[p], s_tile.t[j], wv[c], a, tp, (kmask >> p) & 1);
                 }
-                wave_reduce_add8(a, 5, wnat + 5 * p, lane);
+                wave_reduce_add5(a, wnat + 5 * p, lane, red5_index(lane));
             }
             if (md.acc) {
-                double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                double a[AMAX] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int c = 0; c < CPT; ++c) {
                     const int j = c * G + g;
@@ -682,7 +713,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
                         }
                     }
                 }
-                wave_reduce_add8(a, md.acc, wnat + WN_ACC, lane);
+                wave_reduce_add4(a, md.acc, wnat + WN_ACC, red4_index(lane));
             }
         }
     }
@@ -839,6 +870,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
     __syncthreads();
     const int kmask = wave_uniform(sl.ktab_mask);
     const bool tp = model_tp(md);
+    const int ri5 = red5_index(lane), ri4 = red4_index(lane);    // the sums a lane holds after a reduction
 
     const bool ma = md.ma_active;
     double chi = 0.0, prod = 1.0;
@@ -1018,20 +1050,19 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                     f[3] = -pc.K * __builtin_fma(e, pc.sw, sfw);
                     f[4] = __builtin_fma(e, pc.cw, cfw);
                 }
+                const double b5[5] = {wB * f[0], wB * f[1], wB * f[2], wB * f[3], wB * f[4]};
                 if (!ma) {
-                    double a8[8] = {wB * f[0], wB * f[1], wB * f[2], wB * f[3], wB * f[4], 0.0, 0.0, 0.0};
-                    wave_reduce_add8(a8, 5, wnat + 5 * p, lane);
+                    wave_reduce_add5(b5, wnat + 5 * p, lane, ri5);
                 } else {
                     // the B parts, then the A parts only where some lane's A is not negligible
-                    const double b8[8] = {wB * f[0], wB * f[1], wB * f[2], wB * f[3], wB * f[4], 0.0, 0.0, 0.0};
-                    const double sb = wave_reduce8(b8, lane);
+                    const double sb = wave_reduce5(b5, lane);
                     double sa = 0.0;
                     if (needA) {
-                        const double a8[8] = {wA * f[0], wA * f[1], wA * f[2], wA * f[3], wA * f[4], 0.0, 0.0, 0.0};
-                        sa = wave_reduce8(a8, lane);
+                        const double a5[5] = {wA * f[0], wA * f[1], wA * f[2], wA * f[3], wA * f[4]};
+                        sa = wave_reduce5(a5, lane);
                     }
-                    if (lane < 40 && (lane & 7) == 0) {
-                        const int e = 5 * p + (lane >> 3);
+                    if (ri5 >= 0) {
+                        const int e = 5 * p + ri5;
                         const double R0 = wnatR[e];
                         wnat[e] += __builtin_fma(tB, R0, sb);
                         wnatR[e] = __builtin_fma(tA, R0, sa);
@@ -1052,19 +1083,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                     }
                 }
             }
+            const double b4[AMAX] = {wB * xq[0], wB * xq[1], wB * xq[2], wB * xq[3]};
             if (!ma) {
-                double a8[8] = {wB * xq[0], wB * xq[1], wB * xq[2], wB * xq[3], 0.0, 0.0, 0.0, 0.0};
-                wave_reduce_add8(a8, md.acc, wnat + WN_ACC, lane);
+                wave_reduce_add4(b4, md.acc, wnat + WN_ACC, ri4);
             } else {
-                const double b8[8] = {wB * xq[0], wB * xq[1], wB * xq[2], wB * xq[3], 0.0, 0.0, 0.0, 0.0};
-                const double sb = wave_reduce8(b8, lane);
+                const double sb = wave_reduce4(b4);
                 double sa = 0.0;
                 if (needA) {
-                    const double a8[8] = {wA * xq[0], wA * xq[1], wA * xq[2], wA * xq[3], 0.0, 0.0, 0.0, 0.0};
-                    sa = wave_reduce8(a8, lane);
+                    const double a4[AMAX] = {wA * xq[0], wA * xq[1], wA * xq[2], wA * xq[3]};
+                    sa = wave_reduce4(a4);
                 }
-                if ((lane & 7) == 0 && (lane >> 3) < md.acc) {
-                    const int e = WN_ACC + (lane >> 3);
+                if (ri4 >= 0 && ri4 < md.acc) {
+                    const int e = WN_ACC + ri4;
                     const double R0 = wnatR[e];
                     wnat[e] += __builtin_fma(tB, R0, sb);
                     wnatR[e] = __builtin_fma(tA, R0, sa);
