@@ -301,14 +301,25 @@ __device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, const S
 // Per-vector Kepler tables (mixed path).  For one parameter vector the
 // eccentricity of a planet is fixed, so E(M) on [0, pi] is tabulated once at
 // KT_N nodes M_j = j h (h = pi / (KT_N - 1)) as (E_j, h dE/dM_j) in FP32;
-// each epoch then takes a cubic Hermite interpolant (error <= 6e-7 rad for
-// e <= KT_EMAX, 1.4e-5 at e = 0.7) and one FP64 Newton step.  Above KT_EMAX
+// each epoch then takes a cubic Hermite interpolant and one FP64 step: a
+// Newton step up to e = KT_ENEWTON (start error <= 6.1e-7 rad, after the
+// step <= 6.1e-14), a Halley step up to e = KT_EMAX (start error <= 3.5e-5
+// rad, after the step <= 1.5e-14; a Newton step would leave 4.7e-10).
+// Above KT_EMAX the start error grows fast (1.4e-4 at e = 0.8, 4e-3 at 0.9):
 // the FP32 Markley starter (error <= 5e-7 rad to e = 0.99) and one FP64
-// Halley step are used instead.
+// Halley step are used instead.  These bounds are the maxima over 2e6 mean
+// anomalies on [0, pi] of the FP32 interpolant and the FP64 step.
 // ----------------------------------------------------------------------------
 constexpr int KT_N = 64;
 constexpr double KT_H = PI_D / (KT_N - 1);
-constexpr double KT_EMAX = 0.5;
+constexpr double KT_ENEWTON = 0.5;
+constexpr double KT_EMAX = 0.75;
+// A planet's Kepler path on the mixed path (two bits of SlotLDS::ktab_mask):
+// the FP32 Markley start, or the table start followed by a Newton or a
+// Halley step.
+constexpr int KT_MARKLEY = 0, KT_NEWTON = 1, KT_HALLEY = 3;
+// Planet p's path from the mask: bit p (table built), bit 8 + p (Halley).
+__device__ __forceinline__ int kt_mode(int kmask, int p) { return ((kmask >> p) & 1) | ((kmask >> (7 + p)) & 2); }
 // Building a table costs about as much as 64 FP64 planet-epochs, and saves
 // the FP32 Markley starter's ~40 extra operations per epoch: it pays from
 // about 256 epochs on.
@@ -409,12 +420,12 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
 }
 
 // (sin E, cos E) of the planet at time t.  PREC 0: FP64 Markley; PREC 1:
-// mixed, with `kt` the planet's per-vector table, read when `ktab` (the
-// planet's bit of SlotLDS::ktab_mask, passed as a wave-uniform scalar so
-// that the choice is a scalar branch) is set.  tp as in planet_M.
+// mixed, on the path kmode (kt_mode of the slot's mask, a wave-uniform
+// scalar so that the choice is a scalar branch), with `kt` the planet's
+// per-vector table.  tp as in planet_M.
 template <int PREC>
 __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
-                                          double& sE, double& cE, bool tp, bool ktab) {
+                                          double& sE, double& cE, bool tp, int kmode) {
     const double M = reduce_2pi(planet_M(pc, t, tp));
     if (PREC == 0) {
         kepler_fp64(M, pc, tb, sE, cE);
@@ -422,8 +433,14 @@ __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb
     }
     const double Ma = __builtin_fabs(M);
     const float Mf = (float)Ma;
-    if (ktab) kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
-    else kepler_fp64_step<true>(Ma, kepler_fp32_start(Mf, pc), pc.e, tb, sE, cE);
+    if (kmode == KT_NEWTON) {
+        kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
+    } else {
+        float Ef;
+        if (kmode == KT_HALLEY) Ef = ktab_start(kt, Mf);
+        else Ef = kepler_fp32_start(Mf, pc);
+        kepler_fp64_step<true>(Ma, Ef, pc.e, tb, sE, cE);
+    }
     // E(-M) = -E(M), and sin E >= 0 on [0, pi]: the sign of M, one bit insert
     // (a rounding-level negative sE at E ~ 0 or pi comes back positive).
     sE = __builtin_copysign(sE, M);
@@ -438,9 +455,9 @@ __device__ __forceinline__ double rcp_path(double x) {
 // Planet term at time t (see planet_sc).
 template <int PREC>
 __device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
-                                           bool tp, bool ktab) {
+                                           bool tp, int kmode) {
     double sE, cE;
-    planet_sc<PREC>(pc, tb, kt, t, sE, cE, tp, ktab);
+    planet_sc<PREC>(pc, tb, kt, t, sE, cE, tp, kmode);
     const double den = __builtin_fma(-pc.e, cE, 1.0);
     return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp_path<PREC>(den);
 }

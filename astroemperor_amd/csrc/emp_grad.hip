@@ -72,9 +72,9 @@ struct GradWaveLDS {
 // D cos f = cos E - e, D sin f = b sin E.
 template <int PREC>
 __device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab& tab, const float2* kt, double t,
-                                             double w, double (&a)[5], bool tp, bool ktab) {
+                                             double w, double (&a)[5], bool tp, int kmode) {
     double sE, cE;
-    planet_sc<PREC>(pc, tab, kt, t, sE, cE, tp, ktab);
+    planet_sc<PREC>(pc, tab, kt, t, sE, cE, tp, kmode);
     const double e = pc.e;
     const double rD = rcp_path<PREC>(__builtin_fma(-e, cE, 1.0));
     const double u = cE - e, v = pc.bq * sE;
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
 #pragma unroll
                 for (int c = 0; c < CPT; ++c) {
                     const int j = c * G + g;
-                    if (j < tn) planet_wgrad<PREC>(pc, s_tile.tab, sl.ktab[p], s_tile.t[j], wv[c], a, tp, (kmask >> p) & 1);
+                    if (j < tn) planet_wgrad<PREC>(pc, s_tile.tab, sl.ktab[p], s_tile.t[j], wv[c], a, tp, kt_mode(kmask, p));
                 }
                 wave_reduce_add5(a, wnat + 5 * p, lane, red5_index(lane));
             }
@@ -907,7 +907,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                 sE[p] = cE[p] = 0.0;
                 if (p < md.K) {
                     const PlanetC& pc = sl.planet[p];
-                    planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p], tp, (kmask >> p) & 1);
+                    planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p], tp, kt_mode(kmask, p));
                     mu += __builtin_fma(pc.c1, cE[p], pc.c2 * sE[p])
                           * rcp_path<PREC>(__builtin_fma(-pc.e, cE[p], 1.0));
                 }

@@ -61,15 +61,16 @@ __device__ inline double prior_seq(const ModelDev& md, const double* th) {
 // Per-slot constants (planets, instruments, acceleration) into LDS.
 // Called by the slot's first wave; the caller synchronises afterwards.
 __device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
-    bool kt = false;
+    bool kt = false, kh = false;
     if (lane < md.K) {
         PlanetC pc = planet_setup(sl.theta + md.off_kep + 5 * lane, md.param, md.use_c);
         pc.ktab = md.N >= KT_MIN_EPOCHS && pc.e <= KT_EMAX;
         kt = pc.ktab != 0;
+        kh = kt && pc.e > KT_ENEWTON;
         sl.planet[lane] = pc;
     }
-    const unsigned long long kb = __ballot(kt);       // the whole wave calls this
-    if (lane == 0) sl.ktab_mask = (int)(kb & 0xffu);
+    const unsigned long long kb = __ballot(kt), hb = __ballot(kh);     // the whole wave calls this
+    if (lane == 0) sl.ktab_mask = (int)((kb & 0xffu) | ((hb & 0xffu) << 8));
     if (lane < md.I) {
         sl.gam[lane] = sl.theta[md.off_off + lane];
         double j = md.has_jit ? sl.theta[md.off_jit + lane] : 0.0;
@@ -190,7 +191,7 @@ __device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl
         mu += v * x;
     }
     const bool tp = model_tp(md);
-    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t, tp, (kmask >> p) & 1);
+    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t, tp, kt_mode(kmask, p));
     return mu;
 }
 

@@ -195,20 +195,23 @@ def test_large_config_mixed_vs_fp64():
     assert np.max(np.abs(a[idx] - ref) / np.abs(ref)) <= REL
 
 
-def test_high_eccentricity_mixed_vs_fp64():
-    """Every planet at e in [0.9, 0.99] (the FP32 Markley start and the FP64
-    Halley step of the mixed path), on draws that fit the data badly, which
-    magnifies every model error: still within 1e-6 of the FP64 path."""
+@pytest.mark.parametrize("lo,hi", [(0.0, 0.5), (0.5, 0.75), (0.75, 0.9), (0.9, 0.99)])
+def test_eccentricity_bands_mixed_vs_fp64(lo, hi):
+    """Every planet at e in [lo, hi]: one band per Kepler path of the mixed
+    path (the per-vector table with an FP64 Newton step up to e = 0.5, the
+    table with a Halley step up to 0.75, the FP32 Markley start with a
+    Halley step above), on draws that fit the data badly, which magnifies
+    every model error: still within 1e-6 of the FP64 path."""
     from astroemperor_amd import synth
     m = synth.c2()["model"]
     rng = np.random.default_rng(17)
     th = _draws(m, 4096, rng)
     ecol = [i for i, n in enumerate(m.names_free()) if n.startswith("Eccentricity")]
-    th[:, ecol] = rng.uniform(0.9, 0.99, size=(len(th), len(ecol)))
+    th[:, ecol] = rng.uniform(lo, hi, size=(len(th), len(ecol)))
     a = m.lnl_batch(th, "fp64")
     b = m.lnl_batch(th, "mixed")
     fin = np.isfinite(a)
     assert fin.sum() > 1000 and np.array_equal(fin, np.isfinite(b))
     dm = np.abs(a[fin] - b[fin])
-    print(f"high-e mixed vs fp64: max {dm.max():.2e} p99 {np.percentile(dm, 99):.2e}", flush=True)
+    print(f"e in [{lo}, {hi}] mixed vs fp64: max {dm.max():.2e} p99 {np.percentile(dm, 99):.2e}", flush=True)
     assert dm.max() < 1e-6
