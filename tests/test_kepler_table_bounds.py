@@ -1,0 +1,90 @@
+"""Accuracy of the mixed path's per-vector Kepler table (CPU, numpy).
+
+The mixed likelihood starts each planet's Kepler solve from a 64-node cubic
+Hermite table of E(M) on [0, pi] held in FP32 (ktab_start in
+astroemperor_amd/csrc/emp_device.h), then takes one FP64 step: Newton up to
+e = 0.5, Halley up to e = 0.75, and above that it does not use the table.
+This test restates the interpolant with FP32 rounding and the two steps in
+FP64, and checks the error bounds that choice rests on against the exact
+root (Newton iterations to convergence), over a grid of mean anomalies.
+"""
+import numpy as np
+import pytest
+
+f32 = np.float32
+KT_N = 64
+H = np.pi / (KT_N - 1)
+
+
+def exact_E(M, e):
+    # Start above the root (E = M + e is >= the root on [0, pi]); Newton then
+    # decreases monotonically to it, since f = E - e sin E - M is convex there.
+    E = np.minimum(M + e, np.pi)
+    for _ in range(100):
+        E = E - (E - e * np.sin(E) - M) / (1.0 - e * np.cos(E))
+    return E
+
+
+def table(e):
+    Ej = exact_E(np.arange(KT_N) * H, e)
+    return Ej.astype(f32), (H / (1.0 - e * np.cos(Ej))).astype(f32)
+
+
+def table_start(M, tab):
+    Ej, sl = tab
+    u = M.astype(f32) * f32((KT_N - 1) / np.pi)
+    j = np.minimum(u.astype(np.int32), KT_N - 2)
+    x = u - j.astype(f32)
+    ax, ay, bx, by = Ej[j], sl[j], Ej[j + 1], sl[j + 1]
+    d = bx - ax
+    ss = ay + by
+    c3 = f32(-2.0) * d + ss
+    c2 = f32(3.0) * d - (ay + ss)
+    E = x * (x * (x * c3 + c2) + ay) + ax
+    return np.clip(E, f32(0.0), f32(np.pi)).astype(np.float64)
+
+
+def newton(M, e, E0):
+    return E0 - (E0 - e * np.sin(E0) - M) / (1.0 - e * np.cos(E0))
+
+
+def halley(M, e, E0):
+    s, c = np.sin(E0), np.cos(E0)
+    r1 = 1.0 / (1.0 - e * c)
+    dl = -(E0 - e * s - M) * r1
+    return E0 + dl * (1.0 - 0.5 * dl * e * s * r1)
+
+
+@pytest.fixture(scope="module")
+def grid():
+    return np.linspace(0.0, np.pi, 400_001)
+
+
+@pytest.mark.parametrize("e", [0.1, 0.3, 0.5])
+def test_newton_from_table_up_to_half(grid, e):
+    Et = exact_E(grid, e)
+    E0 = table_start(grid, table(e))
+    assert np.max(np.abs(E0 - Et)) < 7e-7
+    assert np.max(np.abs(newton(grid, e, E0) - Et)) < 1e-13
+
+
+@pytest.mark.parametrize("e", [0.6, 0.7, 0.75])
+def test_halley_from_table_up_to_three_quarters(grid, e):
+    Et = exact_E(grid, e)
+    E0 = table_start(grid, table(e))
+    assert np.max(np.abs(E0 - Et)) < 4e-5
+    assert np.max(np.abs(halley(grid, e, E0) - Et)) < 3e-14
+
+
+def test_newton_would_not_do_above_half(grid):
+    e = 0.75
+    Et = exact_E(grid, e)
+    E0 = table_start(grid, table(e))
+    assert np.max(np.abs(newton(grid, e, E0) - Et)) > 1e-10
+
+
+def test_table_stops_at_three_quarters(grid):
+    e = 0.8
+    Et = exact_E(grid, e)
+    E0 = table_start(grid, table(e))
+    assert np.max(np.abs(halley(grid, e, E0) - Et)) > 1e-13
