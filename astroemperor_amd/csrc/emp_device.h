@@ -303,23 +303,33 @@ __device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, const S
 // KT_N nodes M_j = j h (h = pi / (KT_N - 1)) as (E_j, h dE/dM_j) in FP32;
 // each epoch then takes a cubic Hermite interpolant and one FP64 step: a
 // Newton step up to e = KT_ENEWTON (start error <= 6.1e-7 rad, after the
-// step <= 6.1e-14), a Halley step up to e = KT_EMAX (start error <= 3.5e-5
+// step <= 6.1e-14), a Halley step up to e = KT_EH64 (start error <= 3.5e-5
 // rad, after the step <= 1.5e-14; a Newton step would leave 4.7e-10).
-// Above KT_EMAX the start error grows fast (1.4e-4 at e = 0.8, 4e-3 at 0.9):
-// the FP32 Markley starter (error <= 5e-7 rad to e = 0.99) and one FP64
-// Halley step are used instead.  These bounds are the maxima over 2e6 mean
-// anomalies on [0, pi] of the FP32 interpolant and the FP64 step.
+// Above KT_EH64 the start error grows fast (1.4e-4 at e = 0.8, 7e-3 at
+// 0.95): up to KT_EMAX an FP32 Halley step comes first (then <= 7.3e-7 rad,
+// <= 1.3e-15 after the FP64 Halley step).  Above KT_EMAX (start error
+// 1.3e-2 at e = 0.96, 4.6e-2 at 0.97) the FP32 Markley starter (with the
+// FP32 Halley step, error <= 5e-7 rad to e = 0.99) and one FP64 Halley step
+// are used instead.  These bounds are the maxima over 2e6 mean anomalies on
+// [0, pi] of an FP32 emulation of the interpolant and the FP32 step.
 // ----------------------------------------------------------------------------
 constexpr int KT_N = 64;
 constexpr double KT_H = PI_D / (KT_N - 1);
 constexpr double KT_ENEWTON = 0.5;
-constexpr double KT_EMAX = 0.75;
-// A planet's Kepler path on the mixed path (two bits of SlotLDS::ktab_mask):
-// the FP32 Markley start, or the table start followed by a Newton or a
-// Halley step.
-constexpr int KT_MARKLEY = 0, KT_NEWTON = 1, KT_HALLEY = 3;
-// Planet p's path from the mask: bit p (table built), bit 8 + p (Halley).
-__device__ __forceinline__ int kt_mode(int kmask, int p) { return ((kmask >> p) & 1) | ((kmask >> (7 + p)) & 2); }
+constexpr double KT_EH64 = 0.75;
+constexpr double KT_EMAX = 0.95;
+// A planet's Kepler path on the mixed path, as flags: the start from the
+// table (else the FP32 Markley starter), an FP32 Halley step after it, and
+// an FP64 Halley step (else an FP64 Newton step).  Markley: H32 | H64;
+// table: up to KT_ENEWTON TABLE, up to KT_EH64 TABLE | H64, up to KT_EMAX
+// TABLE | H32 | H64.
+constexpr int KT_F_TABLE = 1, KT_F_H32 = 2, KT_F_H64 = 4;
+__device__ __forceinline__ int kt_flags(bool table, double e) {
+    if (!table) return KT_F_H32 | KT_F_H64;
+    return KT_F_TABLE | (e > KT_EH64 ? KT_F_H32 : 0) | (e > KT_ENEWTON ? KT_F_H64 : 0);
+}
+// Planet p's flags from SlotLDS::ktab_mask (four bits per planet).
+__device__ __forceinline__ int kt_mode(int kmask, int p) { return (int)(((unsigned)kmask >> (4 * p)) & 7u); }
 // Building a table costs about as much as 64 FP64 planet-epochs, and saves
 // the FP32 Markley starter's ~40 extra operations per epoch: it pays from
 // about 256 epochs on.
@@ -342,11 +352,12 @@ __device__ __forceinline__ float ktab_start(const float2* kt, float Mf) {
 // ----------------------------------------------------------------------------
 // Mixed-precision Kepler solve.  FP32 buys the iteration count: the starter
 // comes from the per-vector FP32 table (e <= KT_EMAX) or, above it, from
-// Markley's starter in FP32 (hardware log2/exp2/rcp/sqrt) plus one FP32
-// Halley step with the hardware sin/cos.  One FP64 step from that root and
-// the FP64 projection restore the accuracy the likelihood needs.
+// Markley's starter in FP32 (hardware log2/exp2/rcp/sqrt); the Markley start
+// and the table start above KT_EH64 take one FP32 Halley step with the
+// hardware sin/cos.  One FP64 step from that root and the FP64 projection
+// restore the accuracy the likelihood needs.
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) {
+__device__ __forceinline__ float kepler_fp32_markley(float Mf, const PlanetC& pc) {
     const float ef = (float)pc.e;
     const float omef = 1.0f - ef;
     const float alpha = __builtin_fmaf((float)pc.mk_a1, (float)PI_D - Mf, (float)MK_A0);
@@ -359,17 +370,22 @@ __device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) 
     const float x = __builtin_fabsf(r) + __builtin_amdgcn_sqrtf(__builtin_fmaf(q2, q, r * r));
     // w = x^(2/3) = 2^(2/3 log2 x)
     const float w = __builtin_amdgcn_exp2f(0.66666667f * __builtin_amdgcn_logf(x));
-    float E = (2.0f * r * w * __builtin_amdgcn_rcpf(__builtin_fmaf(w, w + q, q2)) + Mf)
-              * __builtin_amdgcn_rcpf(d);
-    {   // FP32 Halley step with hardware sin/cos (argument in revolutions).
-        const float rev = E * 0.15915494f;
-        const float s = __builtin_amdgcn_sinf(rev), c = __builtin_amdgcn_cosf(rev);
-        const float f0 = E - __builtin_fmaf(ef, s, Mf);
-        const float f1 = __builtin_fmaf(-ef, c, 1.0f);
-        const float dn = -f0 * __builtin_amdgcn_rcpf(f1);
-        E += -f0 * __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f * dn, ef * s, f1));
-    }
-    return E;
+    return (2.0f * r * w * __builtin_amdgcn_rcpf(__builtin_fmaf(w, w + q, q2)) + Mf) * __builtin_amdgcn_rcpf(d);
+}
+
+// FP32 Halley step with hardware sin/cos (argument in revolutions).
+__device__ __forceinline__ float kepler_fp32_halley(float E, float Mf, const PlanetC& pc) {
+    const float ef = (float)pc.e;
+    const float rev = E * 0.15915494f;
+    const float s = __builtin_amdgcn_sinf(rev), c = __builtin_amdgcn_cosf(rev);
+    const float f0 = E - __builtin_fmaf(ef, s, Mf);
+    const float f1 = __builtin_fmaf(-ef, c, 1.0f);
+    const float dn = -f0 * __builtin_amdgcn_rcpf(f1);
+    return E - f0 * __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f * dn, ef * s, f1));
+}
+
+__device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) {
+    return kepler_fp32_halley(kepler_fp32_markley(Mf, pc), Mf, pc);
 }
 
 // Node `lane` of planet pc's table (called by one wave: 64 lanes, 64 nodes),
@@ -421,7 +437,7 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
 
 // (sin E, cos E) of the planet at time t.  PREC 0: FP64 Markley; PREC 1:
 // mixed, on the path kmode (kt_mode of the slot's mask, a wave-uniform
-// scalar so that the choice is a scalar branch), with `kt` the planet's
+// scalar so that each choice is a scalar branch), with `kt` the planet's
 // per-vector table.  tp as in planet_M.
 template <int PREC>
 __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
@@ -433,14 +449,12 @@ __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb
     }
     const double Ma = __builtin_fabs(M);
     const float Mf = (float)Ma;
-    if (kmode == KT_NEWTON) {
-        kepler_fp64_step<false>(Ma, ktab_start(kt, Mf), pc.e, tb, sE, cE);
-    } else {
-        float Ef;
-        if (kmode == KT_HALLEY) Ef = ktab_start(kt, Mf);
-        else Ef = kepler_fp32_start(Mf, pc);
-        kepler_fp64_step<true>(Ma, Ef, pc.e, tb, sE, cE);
-    }
+    float Ef;
+    if (kmode & KT_F_TABLE) Ef = ktab_start(kt, Mf);
+    else Ef = kepler_fp32_markley(Mf, pc);
+    if (kmode & KT_F_H32) Ef = kepler_fp32_halley(Ef, Mf, pc);
+    if (kmode & KT_F_H64) kepler_fp64_step<true>(Ma, Ef, pc.e, tb, sE, cE);
+    else kepler_fp64_step<false>(Ma, Ef, pc.e, tb, sE, cE);
     // E(-M) = -E(M), and sin E >= 0 on [0, pi]: the sign of M, one bit insert
     // (a rounding-level negative sE at E ~ 0 or pi comes back positive).
     sE = __builtin_copysign(sE, M);

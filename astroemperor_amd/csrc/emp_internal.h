@@ -78,7 +78,7 @@ struct SlotLDS {
     double acc[AMAX];
     double cosw[CMAX][3];      // per cosine term: (amplitude, angular frequency, phase)
     float2 ktab[KMAX][KT_N];   // mixed path: (E_j, h dE/dM_j) per planet
-    int ktab_mask;             // bit p: planet p's table is built and used (PlanetC::ktab); bit 8 + p: its step is Halley's
+    int ktab_mask;             // bits 4 p .. 4 p + 2: planet p's Kepler path flags (kt_flags)
 };
 
 }  // namespace emp

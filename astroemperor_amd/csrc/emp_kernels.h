@@ -61,16 +61,20 @@ __device__ inline double prior_seq(const ModelDev& md, const double* th) {
 // Per-slot constants (planets, instruments, acceleration) into LDS.
 // Called by the slot's first wave; the caller synchronises afterwards.
 __device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
-    bool kt = false, kh = false;
+    int kf = 0;
     if (lane < md.K) {
         PlanetC pc = planet_setup(sl.theta + md.off_kep + 5 * lane, md.param, md.use_c);
         pc.ktab = md.N >= KT_MIN_EPOCHS && pc.e <= KT_EMAX;
-        kt = pc.ktab != 0;
-        kh = kt && pc.e > KT_ENEWTON;
+        kf = kt_flags(pc.ktab != 0, pc.e);
         sl.planet[lane] = pc;
     }
-    const unsigned long long kb = __ballot(kt), hb = __ballot(kh);     // the whole wave calls this
-    if (lane == 0) sl.ktab_mask = (int)((kb & 0xffu) | ((hb & 0xffu) << 8));
+    // the whole wave calls these: flag bit b of planet p to mask bit 4 p + b
+    unsigned m = 0;
+    for (int b = 0; b < 3; ++b) {
+        const unsigned long long bits = __ballot((kf >> b) & 1);
+        for (int p = 0; p < KMAX; ++p) m |= (unsigned)((bits >> p) & 1ull) << (4 * p + b);
+    }
+    if (lane == 0) sl.ktab_mask = (int)m;
     if (lane < md.I) {
         sl.gam[lane] = sl.theta[md.off_off + lane];
         double j = md.has_jit ? sl.theta[md.off_jit + lane] : 0.0;

@@ -3,9 +3,11 @@
 The mixed likelihood starts each planet's Kepler solve from a 64-node cubic
 Hermite table of E(M) on [0, pi] held in FP32 (ktab_start in
 astroemperor_amd/csrc/emp_device.h), then takes one FP64 step: Newton up to
-e = 0.5, Halley up to e = 0.75, and above that it does not use the table.
-This test restates the interpolant with FP32 rounding and the two steps in
-FP64, and checks the error bounds that choice rests on against the exact
+e = 0.5, Halley up to e = 0.75, an FP32 Halley step and then the FP64 Halley
+step up to e = 0.95, and above that it does not use the table.  This test
+restates the interpolant and the FP32 step with FP32 rounding (sin and cos
+correctly rounded, where the kernel uses the hardware's) and the FP64
+steps, and checks the error bounds that choice rests on against the exact
 root (Newton iterations to convergence), over a grid of mean anomalies.
 """
 import numpy as np
@@ -48,6 +50,18 @@ def newton(M, e, E0):
     return E0 - (E0 - e * np.sin(E0) - M) / (1.0 - e * np.cos(E0))
 
 
+def halley32(M, e, E0):
+    ef, Mf, E = f32(e), M.astype(f32), E0.astype(f32)
+    rev = E * f32(1.0 / (2.0 * np.pi))
+    s = np.sin(2.0 * np.pi * rev.astype(np.float64)).astype(f32)
+    c = np.cos(2.0 * np.pi * rev.astype(np.float64)).astype(f32)
+    f0 = E - (ef * s + Mf)
+    f1 = f32(1.0) - ef * c
+    dn = -f0 * (f32(1.0) / f1)
+    E = E - f0 * (f32(1.0) / (f32(0.5) * dn * (ef * s) + f1))
+    return np.clip(E, f32(0.0), f32(np.pi)).astype(np.float64)
+
+
 def halley(M, e, E0):
     s, c = np.sin(E0), np.cos(E0)
     r1 = 1.0 / (1.0 - e * c)
@@ -83,8 +97,24 @@ def test_newton_would_not_do_above_half(grid):
     assert np.max(np.abs(newton(grid, e, E0) - Et)) > 1e-10
 
 
-def test_table_stops_at_three_quarters(grid):
+def test_fp64_halley_alone_stops_at_three_quarters(grid):
     e = 0.8
     Et = exact_E(grid, e)
     E0 = table_start(grid, table(e))
     assert np.max(np.abs(halley(grid, e, E0) - Et)) > 1e-13
+
+
+@pytest.mark.parametrize("e", [0.8, 0.9, 0.95])
+def test_fp32_then_fp64_halley_from_table_up_to_095(grid, e):
+    Et = exact_E(grid, e)
+    E0 = table_start(grid, table(e))
+    E1 = halley32(grid, e, E0)
+    assert np.max(np.abs(E1 - Et)) < 1e-6
+    assert np.max(np.abs(halley(grid, e, E1) - Et)) < 3e-14
+
+
+def test_table_stops_at_095(grid):
+    e = 0.97
+    Et = exact_E(grid, e)
+    E1 = halley32(grid, e, table_start(grid, table(e)))
+    assert np.max(np.abs(halley(grid, e, E1) - Et)) > 1e-13
