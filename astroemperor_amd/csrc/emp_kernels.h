@@ -194,8 +194,13 @@ __device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl
         for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
         mu += v * x;
     }
-    const bool tp = model_tp(md);
-    for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t, tp, kt_mode(kmask, p));
+    // The planet loop twice, one copy per parameterisation (a model-wide
+    // choice), so that no planet-epoch branches on it.
+    if (model_tp(md)) {
+        for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t, true, kt_mode(kmask, p));
+    } else {
+        for (int p = 0; p < md.K; ++p) mu += planet_mu<PREC>(sl.planet[p], tb, sl.ktab[p], t, false, kt_mode(kmask, p));
+    }
     return mu;
 }
 
