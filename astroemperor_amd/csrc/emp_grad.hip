@@ -834,8 +834,10 @@ struct Grad1LDS {
 
 // Single-pass adjoint (above).  EXT: 0 plain models, 1 with the terms added
 // after the MA.  NW waves per workgroup (one vector), tiles of 64 NW epochs.
+// TP: the model's planets are parameterised by the time of periastron
+// (model_tp), a template parameter so that no planet-epoch branches on it.
 // `stale` as in k_lnl_grad.
-template <int PREC, int EXT, int NW>
+template <int PREC, int EXT, int NW, bool TP>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 0 ? 4 : 3))) void k_lnl_grad1(ModelDev md, const double* __restrict__ theta, int64_t B,
                                                        int64_t ld, double* __restrict__ out_lnl,
                                                        double* __restrict__ out_grad, unsigned long long* nonfinite,
@@ -869,7 +871,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
             if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
     __syncthreads();
     const int kmask = wave_uniform(sl.ktab_mask);
-    const bool tp = model_tp(md);
+    constexpr bool tp = TP;
     const int ri5 = red5_index(lane), ri4 = red4_index(lane);    // the sums a lane holds after a reduction
 
     const bool ma = md.ma_active;
@@ -1210,9 +1212,11 @@ int emp_launch_grad_ws(const EmpModel* m, const double* th, int64_t B, int64_t l
     if (use_grad1(m)) {
         const ModelDev& md = m->dev;
         if (precision != 0 && precision != 1) { emp_set_error("bad precision"); return EMP_EINVAL; }
-#define EMP_CASE1(P, E) hipLaunchKernelGGL((k_lnl_grad1<P, E, GRAD1_NW>), dim3((unsigned)B), dim3(64 * GRAD1_NW), 0, s, md, th, B, ld, lnl, grad, nf, stale)
-        if (precision == 0) { if (has_post(md)) EMP_CASE1(0, 1); else EMP_CASE1(0, 0); }
-        else { if (has_post(md)) EMP_CASE1(1, 1); else EMP_CASE1(1, 0); }
+#define EMP_CASE1(P, E, T) hipLaunchKernelGGL((k_lnl_grad1<P, E, GRAD1_NW, T>), dim3((unsigned)B), dim3(64 * GRAD1_NW), 0, s, md, th, B, ld, lnl, grad, nf, stale)
+#define EMP_CASE1T(P, E) { if (model_tp(md)) EMP_CASE1(P, E, true); else EMP_CASE1(P, E, false); }
+        if (precision == 0) { if (has_post(md)) EMP_CASE1T(0, 1) else EMP_CASE1T(0, 0) }
+        else { if (has_post(md)) EMP_CASE1T(1, 1) else EMP_CASE1T(1, 0) }
+#undef EMP_CASE1T
 #undef EMP_CASE1
         EMP_HIP(hipGetLastError());
         return EMP_OK;

@@ -180,7 +180,7 @@ __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_rea
 
 // The model's planets are parameterised by the time of periastron
 // (block_repo.py:23-31, parameterisations 3 and 4).
-__device__ __forceinline__ bool model_tp(const ModelDev& md) { return md.param == 3 || md.param == 4; }
+__host__ __device__ __forceinline__ bool model_tp(const ModelDev& md) { return md.param == 3 || md.param == 4; }
 
 // Model value (planets + offset + acceleration) of one epoch.  kmask: the
 // slot's SlotLDS::ktab_mask as a wave-uniform scalar.
