@@ -418,7 +418,12 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
     }
     const double f0 = E0 - __builtin_fma(e, s, Ma);
     const double f1 = __builtin_fma(-e, c, 1.0);
-    const double r1 = rcp64_1(f1);
+    // The Newton step follows the table start at e <= KT_ENEWTON, at most
+    // 6.1e-7 rad from the root: the hardware reciprocal's 4.6e-8 relative
+    // error then moves the step by under 3e-14 rad, and its refinement is
+    // left out.  The Halley step may start 1e-4 rad off (the Markley start
+    // near periastron at high e) and keeps it.
+    const double r1 = HALLEY ? rcp64_1(f1) : __builtin_amdgcn_rcp(f1);
     double dl = -f0 * r1;
     if (HALLEY) {
         // The FP32 Markley start (hardware sin/cos) is off by up to ~1e-4 rad

@@ -151,11 +151,17 @@ static void launch_lnl_t(const ModelDev& md, const double* th, int64_t B, int64_
 int emp_pick_wpt(const EmpModel* m, int64_t B, int precision) {
     (void)precision;
     if (m->wpt_override) return m->wpt_override;
-    // Enough waves to give every SIMD (4 per CU) several, but never more lanes
-    // per vector than about half the epochs.
-    const int64_t target_waves = (int64_t)m->n_cu * 4 * 6;
+    // Enough waves to give every SIMD (4 per CU) about 16, at least two waves
+    // per vector for series of 1024 epochs or more, but never more lanes per
+    // vector than about half the epochs.  Measured batched lnL (mixed):
+    // 7 planets x 10^4 epochs at 8192 vectors 1.58 / 1.40 / 1.40 / 1.49 ms at
+    // 1 / 2 / 4 / 8 waves per vector; 5 planets x 2000 epochs at 65,536
+    // vectors 1.47 / 1.32 / 1.43 / 1.62 ms; 3 planets x 1530 epochs at 2560
+    // vectors 77 / 59 / 56 / 63 us.
+    const int64_t target_waves = (int64_t)m->n_cu * 4 * 16;
+    const int wmin = m->dev.N >= 1024 ? 2 : 1;
     int wpt = 1;
-    while (wpt < 8 && B * wpt < target_waves && 64 * (wpt * 2) <= m->dev.N / 2) wpt *= 2;
+    while (wpt < 8 && (B * wpt < target_waves || wpt < wmin) && 64 * (wpt * 2) <= m->dev.N / 2) wpt *= 2;
     return wpt;
 }
 
