@@ -60,6 +60,18 @@ __device__ __forceinline__ double dpp_mov_f64(double src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// row_shr:n (0x110 + n) with every row enabled and zero in the lanes that
+// have no source (the first n of each row): update_dpp with `old` = 0.0,
+// without a register cleared to hold the 0.0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr_zero_f64(double src) {
+    static_assert(CTRL > 0x110 && CTRL < 0x120, "row_shr only");
+    const long long s = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_mov_dpp((int)s, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(s >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Value of lane k (uniform k) in every lane.
 __device__ __forceinline__ double readlane_f64(double v, int k) {
     const long long bits = __double_as_longlong(v);

@@ -252,7 +252,9 @@ __device__ __forceinline__ double epoch_post(const ModelDev& md, const SlotLDS& 
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ void affine_step(double& a, double& b) {
     const double pa = dpp_f64<CTRL, ROW_MASK>(1.0, a);
-    const double pb = dpp_f64<CTRL, ROW_MASK>(0.0, b);
+    double pb;
+    if constexpr (ROW_MASK == 0xf && CTRL > 0x110 && CTRL < 0x120) pb = dpp_shr_zero_f64<CTRL>(b);
+    else pb = dpp_f64<CTRL, ROW_MASK>(0.0, b);
     b = __builtin_fma(a, pb, b);
     a = a * pa;
 }
