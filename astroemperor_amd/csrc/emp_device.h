@@ -236,12 +236,20 @@ __device__ __forceinline__ void sincos_tab(const SinCosTab& tb, double x, double
     c = __builtin_fma(sc.y, cd, -sc.x * sd);
 }
 
-// Mixed path: sin/cos of E = (double)Ef for an FP32 estimate Ef in [0, pi].
-// The table index comes from the FP32 value; d = E - k h in one FMA (the
-// double h = pi / SC_N is exact to 1e-19, k <= SC_N); sin d to d^3
-// (d^5 / 120 < 7e-14 dropped).
+// floor(v + 1/2) as an integer in one instruction (v_cvt_rpi_i32_f32; NaN
+// gives 0).
+__device__ __forceinline__ int round_half_up(float v) {
+    int k;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(k) : "v"(v));
+    return k;
+}
+
+// Mixed path: sin/cos of E = (double)Ef for an FP32 estimate Ef in
+// [-h/2, pi + h/2].  The table index is Ef / h rounded half up (|d| <= h/2
+// either way at a tie); d = E - k h in one FMA (the double h = pi / SC_N is
+// exact to 1e-19, k <= SC_N); sin d to d^3 (d^5 / 120 < 7e-14 dropped).
 __device__ __forceinline__ void sincos_tab_f(const SinCosTab& tb, float Ef, double E, double& s, double& c) {
-    const int k = (int)__builtin_rintf(Ef * (float)SC_INV_H);
+    const int k = round_half_up(Ef * (float)SC_INV_H);
     const double d = __builtin_fma(-(double)k, SC_H, E);
     const double2 sc = tb.v[k];
     const double d2 = d * d;
@@ -313,7 +321,12 @@ __device__ __forceinline__ void kepler_fp64(double M, const PlanetC& pc, const S
 // Per-vector Kepler tables (mixed path).  For one parameter vector the
 // eccentricity of a planet is fixed, so E(M) on [0, pi] is tabulated once at
 // KT_N nodes M_j = j h (h = pi / (KT_N - 1)) as (E_j, h dE/dM_j) in FP32;
-// each epoch then takes a cubic Hermite interpolant and one FP64 step: a
+// each epoch then takes the cubic Hermite interpolant between its two nodes
+// and one FP64 step.  The table is kept in one of two forms: the nodes
+// (float2, 512 B per planet), or the interpolant's coefficients on each of
+// the KT_N - 1 intervals and, as interval KT_N - 1, its linear extension
+// past pi, where rounding can put |M| (float4, 1 KiB per planet, five fewer
+// operations per epoch).  The FP64 step is: a
 // Newton step up to e = KT_ENEWTON (start error <= 6.1e-7 rad, after the
 // step <= 6.1e-14), a Halley step up to e = KT_EH64 (start error <= 3.5e-5
 // rad, after the step <= 1.5e-14; a Newton step would leave 4.7e-10).
@@ -349,16 +362,33 @@ constexpr int KT_MIN_EPOCHS = 256;
 
 // The cubic Hermite interpolant on [M_j, M_j+1] in Horner form: with
 // d = E_j+1 - E_j and the scaled slopes s0 = h E'_j, s1 = h E'_j+1,
-//   E(x) = E_j + x (s0 + x (3d - 2 s0 - s1 + x (s0 + s1 - 2d))).
+//   E(x) = E_j + x (s0 + x (3d - 2 s0 - s1 + x (s0 + s1 - 2d))),
+// x = M / h - j in [0, 1].  Node form: kt[j] = (E_j, s_j).
 __device__ __forceinline__ float ktab_start(const float2* kt, float Mf) {
-    const float u = Mf * (float)((KT_N - 1) / PI_D);
-    const int j = min((int)u, KT_N - 2);
-    const float x = u - (float)j;
+    constexpr float SCALE = (float)((KT_N - 1) / PI_D);
+    const int j = min((int)(Mf * SCALE), KT_N - 2);
+    const float x = __builtin_fmaf(Mf, SCALE, -(float)j);
     const float2 a = kt[j], b = kt[j + 1];
     const float d = b.x - a.x, ss = a.y + b.y;
     const float c3 = __builtin_fmaf(-2.0f, d, ss);              // s0 + s1 - 2d
     const float c2 = __builtin_fmaf(3.0f, d, -(a.y + ss));      // 3d - 2 s0 - s1
     return __builtin_fmaf(x, __builtin_fmaf(x, __builtin_fmaf(x, c3, c2), a.y), a.x);
+}
+
+// Interval form: kt[j] = (E_j, s0, 3d - 2 s0 - s1, s0 + s1 - 2d), the same
+// FP32 values the node form computes per epoch; x is the fraction of the
+// rounded M / h (one instruction; the node form keeps M / h - j unrounded,
+// a difference of at most 9.5e-8 rad in M, below the 1.2e-7 of M's own
+// rounding to FP32).  j is clamped to the table, which only matters for a
+// mean anomaly that is not finite (x is then not M / h - j: the start is
+// wrong, but stays a finite value in range, or NaN).
+__device__ __forceinline__ float ktab_start(const float4* kt, float Mf) {
+    constexpr float SCALE = (float)((KT_N - 1) / PI_D);
+    const float u = Mf * SCALE;
+    const int j = min((int)u, KT_N - 1);
+    const float x = __builtin_amdgcn_fractf(u);
+    const float4 c = kt[j];
+    return __builtin_fmaf(x, __builtin_fmaf(x, __builtin_fmaf(x, c.w, c.z), c.y), c.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -400,23 +430,41 @@ __device__ __forceinline__ float kepler_fp32_start(float Mf, const PlanetC& pc) 
     return kepler_fp32_halley(kepler_fp32_markley(Mf, pc), Mf, pc);
 }
 
-// Node `lane` of planet pc's table (called by one wave: 64 lanes, 64 nodes),
-// solved in FP32: the nodes are stored in FP32, and FP32 roots leave the
-// interpolant's error where exact nodes put it (<= 6.2e-7 rad for e <= 0.5).
+// Node `lane` of planet pc's table, solved in FP32: the nodes are stored in
+// FP32, and FP32 roots leave the interpolant's error where exact nodes put
+// it (<= 6.2e-7 rad for e <= 0.5).
 __device__ __forceinline__ float2 ktab_node(const PlanetC& pc, int lane) {
     const float E = kepler_fp32_start((float)(lane * KT_H), pc);
     const float c = __builtin_amdgcn_cosf(E * 0.15915494f);
     return make_float2(E, (float)KT_H * __builtin_amdgcn_rcpf(__builtin_fmaf(-(float)pc.e, c, 1.0f)));
 }
 
+// Entry `lane` of planet pc's table in either form (see ktab_start): node
+// lane, or interval lane from nodes lane and lane + 1, the last lane holding
+// the linear extension past pi.  Called by a whole wave (64 lanes, 64 nodes).
+__device__ __forceinline__ void ktab_fill(const PlanetC& pc, int lane, float2& out) { out = ktab_node(pc, lane); }
+__device__ __forceinline__ void ktab_fill(const PlanetC& pc, int lane, float4& out) {
+    const float2 a = ktab_node(pc, lane);
+    const float bx = __shfl_down(a.x, 1), by = __shfl_down(a.y, 1);
+    if (lane == KT_N - 1) {
+        out = make_float4(a.x, a.y, 0.0f, 0.0f);
+    } else {
+        const float d = bx - a.x, ss = a.y + by;
+        out = make_float4(a.x, a.y, __builtin_fmaf(3.0f, d, -(a.y + ss)), __builtin_fmaf(-2.0f, d, ss));
+    }
+}
+
 // One FP64 step from the FP32 root Ef (Ma = |M| in [0, pi]): Newton from the
 // table start (<= 6e-7 rad, so sin E, cos E follow from a first-order
 // rotation, neglected d^2 / 2 < 2e-13), Halley with a third-order rotation
 // from the FP32 Markley start.
+// The Newton step's Ef comes from the table, within 6.1e-7 rad of the root
+// in [0, pi] (or NaN), which sincos_tab_f takes as it is; the Halley step's
+// is clamped into [0, pi] first.
 template <bool HALLEY>
 __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, const SinCosTab& tb, double& sE,
                                                  double& cE) {
-    Ef = __builtin_amdgcn_fmed3f(Ef, 0.0f, (float)PI_D);
+    if (HALLEY) Ef = __builtin_amdgcn_fmed3f(Ef, 0.0f, (float)PI_D);
     double E0 = (double)Ef;
     double s, c;
     // Near periastron at high e the step amplifies an error in sin E0 by
@@ -428,8 +476,10 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
     } else {
         sincos_tab_f(tb, Ef, E0, s, c);
     }
+    // 1 - e c as fma(e, -c, 1) (the same value): the negation then folds into
+    // the operand of each use instead of a negated e kept between blocks.
     const double f0 = E0 - __builtin_fma(e, s, Ma);
-    const double f1 = __builtin_fma(-e, c, 1.0);
+    const double f1 = __builtin_fma(e, -c, 1.0);
     // The Newton step follows the table start at e <= KT_ENEWTON, at most
     // 6.1e-7 rad from the root: the hardware reciprocal's 4.6e-8 relative
     // error then moves the step by under 3e-14 rad, and its refinement is
@@ -455,9 +505,9 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
 // (sin E, cos E) of the planet at time t.  PREC 0: FP64 Markley; PREC 1:
 // mixed, on the path kmode (kt_mode of the slot's mask, a wave-uniform
 // scalar so that each choice is a scalar branch), with `kt` the planet's
-// per-vector table.  tp as in planet_M.
-template <int PREC>
-__device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
+// per-vector table (either form).  tp as in planet_M.
+template <int PREC, class KTE>
+__device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const KTE* kt, double t,
                                           double& sE, double& cE, bool tp, int kmode) {
     const double M = reduce_2pi(planet_M(pc, t, tp));
     if (PREC == 0) {
@@ -484,12 +534,12 @@ __device__ __forceinline__ double rcp_path(double x) {
 }
 
 // Planet term at time t (see planet_sc).
-template <int PREC>
-__device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const float2* kt, double t,
+template <int PREC, class KTE>
+__device__ __forceinline__ double planet_mu(const PlanetC& pc, const SinCosTab& tb, const KTE* kt, double t,
                                            bool tp, int kmode) {
     double sE, cE;
     planet_sc<PREC>(pc, tb, kt, t, sE, cE, tp, kmode);
-    const double den = __builtin_fma(-pc.e, cE, 1.0);
+    const double den = __builtin_fma(pc.e, -cE, 1.0);
     return __builtin_fma(pc.c1, cE, pc.c2 * sE) * rcp_path<PREC>(den);
 }
 

@@ -70,13 +70,13 @@ struct GradWaveLDS {
 // natural parameters, added to a = (dmu/dM tau, dmu/dM, dmu/de, dmu/dw,
 // dmu/dA).  With D = 1 - e cos E, b = sqrt(1 - e^2) (per planet, pc.bq):
 // D cos f = cos E - e, D sin f = b sin E.
-template <int PREC>
-__device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab& tab, const float2* kt, double t,
+template <int PREC, class KTE>
+__device__ __forceinline__ void planet_wgrad(const PlanetC& pc, const SinCosTab& tab, const KTE* kt, double t,
                                              double w, double (&a)[5], bool tp, int kmode) {
     double sE, cE;
     planet_sc<PREC>(pc, tab, kt, t, sE, cE, tp, kmode);
     const double e = pc.e;
-    const double rD = rcp_path<PREC>(__builtin_fma(-e, cE, 1.0));
+    const double rD = rcp_path<PREC>(__builtin_fma(e, -cE, 1.0));
     const double u = cE - e, v = pc.bq * sE;
     const double sfw = __builtin_fma(v, pc.cw, u * pc.sw) * rD;     // sin(f + w)
     const double cfw = __builtin_fma(u, pc.cw, -(v * pc.sw)) * rD;  // cos(f + w)
@@ -172,8 +172,8 @@ __device__ __forceinline__ void wave_reduce_add4(const double (&v)[4], int n, do
 
 // Adds w = dlnL/dS_n times the derivatives of the terms added after the MA
 // at epoch n (time t, instrument k).
-template <int EXT>
-__device__ __forceinline__ void post_accumulate(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tab, double t,
+template <int EXT, class SL>
+__device__ __forceinline__ void post_accumulate(const ModelDev& md, const SL& sl, const SinCosTab& tab, double t,
                                                 int n, int k, double w, double* wacc) {
     using GW = GradWave<EXT>;
     if (md.sai_w) {
@@ -194,7 +194,8 @@ __device__ __forceinline__ void post_accumulate(const ModelDev& md, const SlotLD
 
 // MA(1) multiplier of epoch n (device order) from its instrument and its MA
 // column (the fixed-timescale weight, or dt with -1 at a segment start).
-__device__ __forceinline__ double ma_mult(const ModelDev& md, const SlotLDS& sl, int k, double col) {
+template <class SL>
+__device__ __forceinline__ double ma_mult(const ModelDev& md, const SL& sl, int k, double col) {
     if (md.ma_wfixed) return -(sl.ma_c[k] * col);
     return col < 0.0 ? 0.0 : -(sl.ma_c[k] * exp(-col / sl.ma_tau[k]));
 }
@@ -206,8 +207,8 @@ __device__ __forceinline__ double ma_mult(const ModelDev& md, const SlotLDS& sl,
 // are composed from the wave's last lane down (suffix), then across the
 // slot's waves from the last one.  `carry` is the state after the chunk
 // (updated to the state before it).  Reads the tile before the barrier.
-template <int Q, int WPT>
-__device__ __forceinline__ double lam_chunk_q(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, bool valid,
+template <int Q, int WPT, class SL>
+__device__ __forceinline__ double lam_chunk_q(const ModelDev& md, const SL& sl, const TileLDS& tl, bool valid,
                                               int j, int tb, int tn, double t, int k, double u, int lane, int wave,
                                               int wsub, int& parity, double (&carry)[4]) {
     __shared__ double qtot[2][MAX_WAVES][Q * Q + Q];
@@ -359,8 +360,8 @@ struct InsAcc {
 // lane's epoch from eps (the forward pass's scratch row), with the carry from
 // the later epochs; returns w = dlnL/dmu = -lambda (0 for an invalid lane)
 // after adding the epoch's offset, jitter, post-MA and MA-parameter terms.
-template <int PREC, int WPT, int EXT>
-__device__ __forceinline__ double ma_chunk_w(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, ScanLDS& sc,
+template <int PREC, int WPT, int EXT, class SL>
+__device__ __forceinline__ double ma_chunk_w(const ModelDev& md, const SL& sl, const TileLDS& tl, ScanLDS& sc,
                                              const double* eps, double* wacc, InsAcc& ia, bool valid, int j, int tb,
                                              int tn, int lane, int wave, int wsub, int& parity, double& lam_carry,
                                              double (&lq_carry)[4]) {
@@ -462,8 +463,8 @@ __device__ __forceinline__ double ma_chunk_w(const ModelDev& md, const SlotLDS& 
 // Without the MA: the lane's epoch's residual, its share of chi^2 and of the
 // log-determinant, w = dlnL/dmu = r / err2 and the epoch's offset, jitter and
 // post-MA terms.
-template <int PREC, int EXT>
-__device__ __forceinline__ double plain_w(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, double* wacc,
+template <int PREC, int EXT, class SL>
+__device__ __forceinline__ double plain_w(const ModelDev& md, const SL& sl, const TileLDS& tl, double* wacc,
                                           InsAcc& ia, int j, int tb, double& chi, double& prod, int& pexp,
                                           int kmask) {
     using GW = GradWave<EXT>;
@@ -490,8 +491,8 @@ __device__ __forceinline__ double plain_w(const ModelDev& md, const SlotLDS& sl,
 // Adds the per-wave partial sums of a slot in wave order, maps the natural
 // gradients to the parameterisation and writes lnL and the free gradient of
 // vector b.  All threads of the workgroup call this (it holds barriers).
-template <int EXT, int WPT>
-__device__ __forceinline__ void grad_finish(const ModelDev& md, const SlotLDS& sl, GradLDS& gl,
+template <int EXT, int WPT, class SL>
+__device__ __forceinline__ void grad_finish(const ModelDev& md, const SL& sl, GradLDS& gl,
                                             const GradWaveLDS<EXT>& s_gw, bool live, int wave, int wsub, int lane,
                                             double lnl_v, int64_t b, double* __restrict__ out_lnl,
                                             double* __restrict__ out_grad, unsigned long long* nonfinite, int* stale) {
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     constexpr int CPT = TILE / G;             // chunks of a tile per wave
     constexpr bool POST = EXT >= 1;
     using GW = GradWave<EXT>;
-    __shared__ SlotLDS s_slot[SLOTS];
+    __shared__ SlotLDS<SLOTS> s_slot[SLOTS];
     __shared__ GradLDS s_grad[SLOTS];
     __shared__ GradWaveLDS<EXT> s_gw;
     __shared__ TileLDS s_tile;
@@ -630,7 +631,7 @@ __global__ __launch_bounds__(BLOCK) void k_lnl_grad(ModelDev md, const double* _
     const int64_t b = (int64_t)blockIdx.x * SLOTS + slot;
     const bool live = b < B && (!stale || stale[b] != 0);
     if (!__syncthreads_or(live)) return;       // no vector of this workgroup to compute
-    SlotLDS& sl = s_slot[slot];
+    SlotLDS<SLOTS>& sl = s_slot[slot];
     GradLDS& gl = s_grad[slot];
     if (live) expand_theta(md, sl, theta + b * ld, g, G);
     for (int j = g; j < MAXD; j += G) gl.g[j] = 0.0;
@@ -845,7 +846,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
     constexpr int WPT = NW, NT = 64 * NW, TILE1 = NT;
     constexpr bool POST = EXT >= 1;
     using GW = GradWave<EXT>;
-    __shared__ SlotLDS sl;
+    __shared__ SlotLDS<1> sl;
     __shared__ GradLDS gl;
     __shared__ GradWaveLDS<EXT> s_gw;
     __shared__ Grad1LDS<EXT> s_g1;
@@ -868,7 +869,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
     __syncthreads();
     if (PREC == 1)      // the mixed path's per-vector Kepler tables, one planet per wave
         for (int p = wave; p < md.K; p += NW)
-            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
+            if (sl.planet[p].ktab) ktab_fill(sl.planet[p], lane, sl.ktab[p][lane]);
     __syncthreads();
     const int kmask = wave_uniform(sl.ktab_mask);
     constexpr bool tp = TP;
@@ -911,7 +912,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                     const PlanetC& pc = sl.planet[p];
                     planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p], tp, kt_mode(kmask, p));
                     mu += __builtin_fma(pc.c1, cE[p], pc.c2 * sE[p])
-                          * rcp_path<PREC>(__builtin_fma(-pc.e, cE[p], 1.0));
+                          * rcp_path<PREC>(__builtin_fma(pc.e, -cE[p], 1.0));
                 }
             }
             err2 = s_tile.s2[j] + sl.jit2[k];
@@ -1040,7 +1041,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                 double f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
                 if (valid) {
                     const double e = pc.e;
-                    const double rD = rcp_path<PREC>(__builtin_fma(-e, cE[p], 1.0));
+                    const double rD = rcp_path<PREC>(__builtin_fma(e, -cE[p], 1.0));
                     const double uu = cE[p] - e, vv = pc.bq * sE[p];
                     const double sfw = __builtin_fma(vv, pc.cw, uu * pc.sw) * rD;    // sin(f + w)
                     const double cfw = __builtin_fma(uu, pc.cw, -(vv * pc.sw)) * rD; // cos(f + w)

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/emp_abi.h"
@@ -70,16 +71,24 @@ struct ModelDev {
     double t0;                        // X_[0] (acceleration origin)
 };
 
-// Per-slot LDS scratch of the likelihood kernels.
-struct SlotLDS {
+// Per-slot LDS scratch of the likelihood kernels.  KTE: the entry of the
+// mixed path's Kepler tables (ktab_start), float4 intervals or float2 nodes.
+template <class KTE>
+struct SlotLDSt {
     double theta[MAXD];
     PlanetC planet[KMAX];
     double gam[IMAX], jit2[IMAX], ma_c[IMAX], ma_tau[IMAX], jit[IMAX];
     double acc[AMAX];
     double cosw[CMAX][3];      // per cosine term: (amplitude, angular frequency, phase)
-    float2 ktab[KMAX][KT_N];   // mixed path: (E_j, h dE/dM_j) per planet
+    KTE ktab[KMAX][KT_N];      // mixed path: per planet, the Kepler table
     int ktab_mask;             // bits 4 p .. 4 p + 2: planet p's Kepler path flags (kt_flags)
 };
+
+// The slot of a workgroup that holds `SLOTS` of them: the interval tables
+// (8 KiB per slot) where there are at most two, the node tables (4 KiB)
+// where more would cost occupancy or not fit (16 slots: 175 KiB).
+template <int SLOTS>
+using SlotLDS = SlotLDSt<std::conditional_t<(SLOTS <= 2), float4, float2>>;
 
 }  // namespace emp
 

@@ -60,7 +60,8 @@ __device__ inline double prior_seq(const ModelDev& md, const double* th) {
 
 // Per-slot constants (planets, instruments, acceleration) into LDS.
 // Called by the slot's first wave; the caller synchronises afterwards.
-__device__ inline void slot_setup(const ModelDev& md, SlotLDS& sl, int lane) {
+template <class SL>
+__device__ inline void slot_setup(const ModelDev& md, SL& sl, int lane) {
     int kf = 0;
     if (lane < md.K) {
         PlanetC pc = planet_setup(sl.theta + md.off_kep + 5 * lane, md.param, md.use_c);
@@ -161,14 +162,14 @@ __device__ __forceinline__ void epoch_store(const ModelDev& md, TileLDS& tl, int
 // Per-slot constants, then (mixed path) the per-vector Kepler tables, one
 // planet per wave of the slot.  The sin/cos table must already be in LDS.
 // All threads of the workgroup call this (it holds two barriers).
-template <int PREC, int WPT>
-__device__ __forceinline__ void slot_prepare(const ModelDev& md, SlotLDS& sl, const TileLDS& tl, bool live,
+template <int PREC, int WPT, class SL>
+__device__ __forceinline__ void slot_prepare(const ModelDev& md, SL& sl, const TileLDS& tl, bool live,
                                              int wsub, int lane) {
     if (live && wsub == 0) slot_setup(md, sl, lane);
     __syncthreads();
     if (PREC == 1 && live) {
         for (int p = wsub; p < md.K; p += WPT)
-            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
+            if (sl.planet[p].ktab) ktab_fill(sl.planet[p], lane, sl.ktab[p][lane]);
     }
     __syncthreads();
 }
@@ -184,8 +185,8 @@ __host__ __device__ __forceinline__ bool model_tp(const ModelDev& md) { return m
 
 // Model value (planets + offset + acceleration) of one epoch.  kmask: the
 // slot's SlotLDS::ktab_mask as a wave-uniform scalar.
-template <int PREC>
-__device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t, int k,
+template <int PREC, class SL>
+__device__ __forceinline__ double epoch_mu(const ModelDev& md, const SL& sl, const SinCosTab& tb, double t, int k,
                                            int kmask) {
     double mu = sl.gam[k];
     if (md.acc) {
@@ -206,7 +207,8 @@ __device__ __forceinline__ double epoch_mu(const ModelDev& md, const SlotLDS& sl
 
 // Phase freq * t + phase of cosine term c, rounded as numpy rounds
 // `freq * X_ + phase` (no contraction).
-__device__ __forceinline__ double cos_phase(const SlotLDS& sl, int c, double t) {
+template <class SL>
+__device__ __forceinline__ double cos_phase(const SL& sl, int c, double t) {
 #pragma clang fp contract(off)
     return sl.cosw[c][1] * t + sl.cosw[c][2];
 }
@@ -227,7 +229,8 @@ __host__ __device__ inline bool needs_ext(const ModelDev& md) {
 // columns of the epoch's instrument times their coefficients (sai00.model),
 // then the cosine terms, whose angle is reduced in FP64 and whose cosine
 // comes from the FP64 table on both precision paths.
-__device__ __forceinline__ double epoch_post(const ModelDev& md, const SlotLDS& sl, const SinCosTab& tb, double t,
+template <class SL>
+__device__ __forceinline__ double epoch_post(const ModelDev& md, const SL& sl, const SinCosTab& tb, double t,
                                              int n, int k) {
     double v = 0.0;
     if (md.sai_w) {
@@ -314,8 +317,8 @@ __device__ __forceinline__ void affq_step(double (&m)[Q][Q], double (&v)[Q]) {
 // previous chunk (`carry`, updated).  Reads the tile only before the first
 // barrier.  Returns eps of the lane's epoch.  Whole wave active; with WPT > 1
 // all waves of the workgroup call this together (one barrier).
-template <int Q, int WPT>
-__device__ __forceinline__ double ma_chunk_q(const ModelDev& md, const SlotLDS& sl, const TileLDS& tl, bool valid,
+template <int Q, int WPT, class SL>
+__device__ __forceinline__ double ma_chunk_q(const ModelDev& md, const SL& sl, const TileLDS& tl, bool valid,
                                              int j, int tb, double tj, int k, double r, int lane, int wave,
                                              int wsub, int& parity, double (&carry)[4]) {
     __shared__ double qtot[2][MAX_WAVES][Q * Q + Q];
@@ -443,8 +446,8 @@ struct ScanLDS {
 // comes (fewer registers).  POST = false compiles out the features of
 // needs_ext: the terms added after the moving average (activity and cosine
 // terms) and MA orders above 1.
-template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true, bool POST = true>
-__device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, ScanLDS& sc,
+template <int PREC, int WPT, bool WRITE, bool WEPS = false, bool PF = true, bool POST = true, class SL>
+__device__ double block_lnl(const ModelDev& md, const SL& sl, TileLDS& tl, ScanLDS& sc,
                             bool live, int wsub, double* out_model, double* out_err2, bool resident = false,
                             double* out_eps = nullptr, const EpochRegs* first = nullptr) {
     const int lane = threadIdx.x & 63;
@@ -559,7 +562,8 @@ __device__ double block_lnl(const ModelDev& md, const SlotLDS& sl, TileLDS& tl, 
 
 // Expand the free vector (global memory, row `row`) into the slot's full
 // vector in LDS, re-inserting fixed values (emp/emp_model.py:709-711).
-__device__ __forceinline__ void expand_theta(const ModelDev& md, SlotLDS& sl, const double* row, int g, int G) {
+template <class SL>
+__device__ __forceinline__ void expand_theta(const ModelDev& md, SL& sl, const double* row, int g, int G) {
     for (int j = g; j < md.Dfull; j += G) {
         const int f = md.inv_free[j];
         sl.theta[j] = f >= 0 ? row[f] : md.fixed_val[j];

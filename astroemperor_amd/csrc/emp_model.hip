@@ -82,14 +82,14 @@ __global__ __launch_bounds__(BLOCK) void k_lnl(ModelDev md, const double* __rest
                                                int64_t ld, double* __restrict__ out,
                                                unsigned long long* nonfinite) {
     constexpr int SLOTS = NWAVES / WPT;
-    __shared__ SlotLDS s_slot[SLOTS];
+    __shared__ SlotLDS<SLOTS> s_slot[SLOTS];
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int slot = wave / WPT, wsub = wave % WPT;
     const int64_t b = (int64_t)blockIdx.x * SLOTS + slot;
     const bool live = b < B;
-    SlotLDS& sl = s_slot[slot];
+    SlotLDS<SLOTS>& sl = s_slot[slot];
     if (live) expand_theta(md, sl, theta + b * ld, wsub * 64 + lane, 64 * WPT);
     tab_load(md, s_tile);
     __syncthreads();
@@ -119,11 +119,11 @@ __global__ void k_prior(ModelDev md, const double* __restrict__ theta, int64_t B
 
 __global__ __launch_bounds__(BLOCK) void k_model_eval(ModelDev md, const double* __restrict__ theta,
                                                       double* out_model, double* out_err2) {
-    __shared__ SlotLDS s_slot[1];
+    __shared__ SlotLDS<1> s_slot[1];
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    SlotLDS& sl = s_slot[0];
+    SlotLDS<1>& sl = s_slot[0];
     expand_theta(md, sl, theta, threadIdx.x, BLOCK);
     tab_load(md, s_tile);
     __syncthreads();

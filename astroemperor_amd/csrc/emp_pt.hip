@@ -187,7 +187,7 @@ __global__ __launch_bounds__(PRIOR_BLOCK) void k_pt_prior(ModelDev md, PTDev pd,
 template <int PREC, int WPT, bool POST>
 __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int half, int step) {
     constexpr int SLOTS = NWAVES / WPT;
-    __shared__ SlotLDS s_slot[SLOTS];
+    __shared__ SlotLDS<SLOTS> s_slot[SLOTS];
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     __shared__ int s_acc[SLOTS];
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_half(ModelDev md, PTDev pd, int ha
         return;
     }
     const PTView pv = view_of(pd);
-    SlotLDS& sl = s_slot[slot];
+    SlotLDS<SLOTS>& sl = s_slot[slot];
     tab_load(md, s_tile);      // published by pt_eval_accept's first barrier
     if (live) {
         if (wsub == 0 && lane == 0) s_ops[slot] = pd.rec[b];

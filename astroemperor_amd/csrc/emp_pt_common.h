@@ -175,8 +175,8 @@ __device__ __forceinline__ TaskOps pt_task_ops(const PTView& pv, const Proposal&
 // in `tile`; with `resident` all epochs are too.  All threads of the
 // workgroup call this together (it contains barriers).  POST = false
 // compiles the terms added after the MA out (models without them).
-template <int PREC, int WPT, bool PF = true, bool POST = true>
-__device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskOps* s_ops, bool live, SlotLDS& sl,
+template <int PREC, int WPT, bool PF = true, bool POST = true, class SL>
+__device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskOps* s_ops, bool live, SL& sl,
                                TileLDS& tile, ScanLDS& scan, int* s_acc, int slot, int wsub, bool resident,
                                unsigned long long* n_lnl) {
     const int lane = threadIdx.x & 63;
@@ -189,7 +189,7 @@ __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskO
     __syncthreads();
     if (PREC == 1 && live) {       // per-vector Kepler tables, one planet per wave
         for (int p = wsub; p < md.K; p += WPT)
-            if (sl.planet[p].ktab) sl.ktab[p][lane] = ktab_node(sl.planet[p], lane);
+            if (sl.planet[p].ktab) ktab_fill(sl.planet[p], lane, sl.ktab[p][lane]);
     }
     __syncthreads();
     double ll = block_lnl<PREC, WPT, false, false, PF, POST>(md, sl, tile, scan, live, wsub, nullptr, nullptr,
@@ -220,9 +220,9 @@ __device__ void pt_eval_accept(const ModelDev& md, const PTView& pv, const TaskO
 // prior lp is already known (b < 0: no task for this slot in this pass):
 // proposal into LDS, then pt_eval_accept (without the tile prefetch: the
 // batched-systems kernel that calls this is register-bound).
-template <int PREC, int WPT, bool EXT = true>
+template <int PREC, int WPT, bool EXT = true, class SL>
 __device__ void pt_half_task(const ModelDev& md, const PTView& pv, long long sweep, int half, int step, int b,
-                             double lp, SlotLDS& sl, TileLDS& tile, ScanLDS& scan, int* s_acc, TaskOps* s_ops,
+                             double lp, SL& sl, TileLDS& tile, ScanLDS& scan, int* s_acc, TaskOps* s_ops,
                              int slot, int wsub, bool resident, unsigned long long* n_lnl) {
     const int lane = threadIdx.x & 63;
     const int g = wsub * 64 + lane, G = 64 * WPT;

@@ -138,7 +138,7 @@ __device__ inline void sys_priors(const ModelDev& md, const PTView& pv, long lon
 template <int PREC, int WPT, int NW, bool EXT>
 __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
     constexpr int NT = NW * 64, SLOTS = NW / WPT;
-    __shared__ SlotLDS s_slot[SLOTS];
+    __shared__ SlotLDS<SLOTS> s_slot[SLOTS];
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     __shared__ int s_acc[SLOTS];
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_sweep(SysDev sd) {
 template <int PREC, int WPT, int NW, bool EXT>
 __global__ __launch_bounds__(NW * 64) void k_sys_init(SysDev sd) {
     constexpr int NT = NW * 64, SLOTS = NW / WPT;
-    __shared__ SlotLDS s_slot[SLOTS];
+    __shared__ SlotLDS<SLOTS> s_slot[SLOTS];
     __shared__ TileLDS s_tile;
     __shared__ ScanLDS s_scan;
     __shared__ double s_lp[SLOTS];
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(NW * 64) void k_sys_init(SysDev sd) {
     const ModelDev& md = sys_model(sd, s);
     const PTView pv = sys_view(sd, s, md.D);
     const int TW = sd.T * sd.W;
-    SlotLDS& sl = s_slot[slot];
+    SlotLDS<SLOTS>& sl = s_slot[slot];
     tab_load(md, s_tile);
     for (int e = tid; e < TW; e += NT) pv.map[e] = e;
     for (int b0 = 0; b0 < TW; b0 += SLOTS) {
@@ -295,7 +295,10 @@ struct EmpSystems {
 
 // Static LDS of the sweep kernel's slot array (the swap round's working set
 // must fit in it).
-static size_t slot_bytes(int nw, int wpt) { return (size_t)(nw / wpt) * sizeof(SlotLDS); }
+static size_t slot_bytes(int nw, int wpt) {
+    const int slots = nw / wpt;
+    return (size_t)slots * (slots <= 2 ? sizeof(SlotLDS<2>) : sizeof(SlotLDS<4>));
+}
 static size_t swap_bytes(int T, int W) { return (size_t)(2 * T - 1) * W * (sizeof(double) + sizeof(int)); }
 
 #define EMP_SYS_CONFIGS(X) X(0, 1, 8) X(0, 1, 16) X(0, 2, 8) X(0, 2, 16) X(1, 1, 8) X(1, 1, 16) X(1, 2, 8) X(1, 2, 16)

@@ -4,10 +4,13 @@ The mixed likelihood starts each planet's Kepler solve from a 64-node cubic
 Hermite table of E(M) on [0, pi] held in FP32 (ktab_start in
 astroemperor_amd/csrc/emp_device.h), then takes one FP64 step: Newton up to
 e = 0.5, Halley up to e = 0.75, an FP32 Halley step and then the FP64 Halley
-step up to e = 0.95, and above that it does not use the table.  This test
-restates the interpolant and the FP32 step with FP32 rounding (sin and cos
-correctly rounded, where the kernel uses the hardware's) and the FP64
-steps, and checks the error bounds that choice rests on against the exact
+step up to e = 0.95, and above that it does not use the table.  The table
+comes in two forms: the nodes, with the interpolant's coefficients formed
+per epoch and x = M / h - j kept unrounded (one FMA), and the coefficients
+per interval, with the fraction of the rounded M / h and a linear interval
+past pi.  This test restates both with FP32 rounding (sin and cos correctly
+rounded, where the kernel uses the hardware's) and the FP64 steps, and
+checks the error bounds that the choice of step rests on against the exact
 root (Newton iterations to convergence), over a grid of mean anomalies.
 """
 import numpy as np
@@ -16,6 +19,7 @@ import pytest
 f32 = np.float32
 KT_N = 64
 H = np.pi / (KT_N - 1)
+SCALE = f32((KT_N - 1) / np.pi)
 
 
 def exact_E(M, e):
@@ -32,18 +36,48 @@ def table(e):
     return Ej.astype(f32), (H / (1.0 - e * np.cos(Ej))).astype(f32)
 
 
-def table_start(M, tab):
-    Ej, sl = tab
-    u = M.astype(f32) * f32((KT_N - 1) / np.pi)
-    j = np.minimum(u.astype(np.int32), KT_N - 2)
-    x = u - j.astype(f32)
-    ax, ay, bx, by = Ej[j], sl[j], Ej[j + 1], sl[j + 1]
+def coefficients(ax, ay, bx, by):
     d = bx - ax
     ss = ay + by
-    c3 = f32(-2.0) * d + ss
-    c2 = f32(3.0) * d - (ay + ss)
-    E = x * (x * (x * c3 + c2) + ay) + ax
-    return np.clip(E, f32(0.0), f32(np.pi)).astype(np.float64)
+    return f32(3.0) * d - (ay + ss), f32(-2.0) * d + ss     # c2, c3
+
+
+def intervals(tab):
+    """Interval form: (E_j, s_j, c2, c3) per interval, the last interval
+    (from node KT_N - 1) linear."""
+    Ej, sl = tab
+    c2, c3 = coefficients(Ej[:-1], sl[:-1], Ej[1:], sl[1:])
+    return Ej, sl, np.append(c2, f32(0.0)), np.append(c3, f32(0.0))
+
+
+def horner(x, c0, c1, c2, c3):
+    return (x * (x * (x * c3 + c2) + c1) + c0).astype(np.float64)
+
+
+def start_nodes(M, tab):
+    Ej, sl = tab
+    Mf = M.astype(f32)
+    j = np.minimum((Mf * SCALE).astype(np.int32), KT_N - 2)
+    x = (Mf.astype(np.float64) * np.float64(SCALE) - j).astype(f32)   # one rounding (the FMA)
+    c2, c3 = coefficients(Ej[j], sl[j], Ej[j + 1], sl[j + 1])
+    return horner(x, Ej[j], sl[j], c2, c3)
+
+
+def start_intervals(M, iv):
+    u = M.astype(f32) * SCALE
+    j = np.minimum(u.astype(np.int32), KT_N - 1)
+    x = u - np.floor(u)
+    return horner(x, *(a[j] for a in iv))
+
+
+def table_start(M, tab):
+    """The interval form, as the kernels with few vectors per workgroup use."""
+    return start_intervals(M, intervals(tab))
+
+
+def clamp32(E):
+    """The Halley steps' clamp of the FP32 start into [0, (float)pi]."""
+    return np.clip(E, 0.0, float(f32(np.pi)))
 
 
 def newton(M, e, E0):
@@ -59,7 +93,7 @@ def halley32(M, e, E0):
     f1 = f32(1.0) - ef * c
     dn = -f0 * (f32(1.0) / f1)
     E = E - f0 * (f32(1.0) / (f32(0.5) * dn * (ef * s) + f1))
-    return np.clip(E, f32(0.0), f32(np.pi)).astype(np.float64)
+    return clamp32(E.astype(np.float64))
 
 
 def halley(M, e, E0):
@@ -74,18 +108,35 @@ def grid():
     return np.linspace(0.0, np.pi, 400_001)
 
 
-@pytest.mark.parametrize("e", [0.1, 0.3, 0.5])
-def test_newton_from_table_up_to_half(grid, e):
+@pytest.mark.parametrize("form", ["nodes", "intervals"])
+@pytest.mark.parametrize("e", [0.0, 0.1, 0.3, 0.5])
+def test_newton_from_table_up_to_half(grid, e, form):
     Et = exact_E(grid, e)
-    E0 = table_start(grid, table(e))
+    tab = table(e)
+    E0 = start_nodes(grid, tab) if form == "nodes" else start_intervals(grid, intervals(tab))
     assert np.max(np.abs(E0 - Et)) < 7e-7
+    # no clamp before the Newton step: the start stays inside the sin/cos
+    # table's range [-h/2, pi + h/2] (h = pi / 256)
+    assert E0.min() > -np.pi / 512 and E0.max() < np.pi * (1 + 1 / 512)
     assert np.max(np.abs(newton(grid, e, E0) - Et)) < 1e-13
+
+
+def test_interval_past_pi():
+    """|M| can round above pi (FP32 pi is 8.7e-8 above it): the last interval
+    continues the table linearly, E(M) = pi + (M - pi) / (1 + e) to first
+    order, instead of clamping to the previous interval."""
+    for e in (0.0, 0.3, 0.5, 0.9):
+        iv = intervals(table(e))
+        M = np.array([np.pi, np.nextafter(np.pi, 4.0), float(f32(np.pi))])
+        assert (M.astype(f32) * SCALE).astype(np.int32).max() == KT_N - 1
+        E0 = start_intervals(M, iv)
+        assert np.max(np.abs(E0 - (np.pi + (M - np.pi) / (1 + e)))) < 1e-6
 
 
 @pytest.mark.parametrize("e", [0.6, 0.7, 0.75])
 def test_halley_from_table_up_to_three_quarters(grid, e):
     Et = exact_E(grid, e)
-    E0 = table_start(grid, table(e))
+    E0 = clamp32(table_start(grid, table(e)))
     assert np.max(np.abs(E0 - Et)) < 4e-5
     assert np.max(np.abs(halley(grid, e, E0) - Et)) < 3e-14
 
@@ -100,7 +151,7 @@ def test_newton_would_not_do_above_half(grid):
 def test_fp64_halley_alone_stops_at_three_quarters(grid):
     e = 0.8
     Et = exact_E(grid, e)
-    E0 = table_start(grid, table(e))
+    E0 = clamp32(table_start(grid, table(e)))
     assert np.max(np.abs(halley(grid, e, E0) - Et)) > 1e-13
 
 
