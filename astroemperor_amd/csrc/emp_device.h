@@ -72,6 +72,18 @@ __device__ __forceinline__ double dpp_shr_zero_f64(double src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// row_shr:n with every row enabled and 1.0 in the lanes that have no
+// source: the low word of 1.0 is zero (bound_ctrl zero fill), only the high
+// word needs an old value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr_one_f64(double src) {
+    static_assert(CTRL > 0x110 && CTRL < 0x120, "row_shr only");
+    const long long s = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_mov_dpp((int)s, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0x3ff00000, (int)(s >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Value of lane k (uniform k) in every lane.
 __device__ __forceinline__ double readlane_f64(double v, int k) {
     const long long bits = __double_as_longlong(v);
@@ -126,6 +138,8 @@ __device__ __forceinline__ double reduce_2pi(double M) {
 // Per-planet constants, computed once per theta (per planet) and kept in LDS.
 // ----------------------------------------------------------------------------
 struct PlanetC {
+    double frev;     // 1 / P: the mean motion in revolutions (mixed path)
+    double phrev;    // phase / (2 pi)
     double freq;     // 2 pi / P
     double phase;    // M = freq * t + phase         (tp_mode == 0)
     double tp;       // M = freq * (t - tp)          (tp_mode == 1)
@@ -186,6 +200,8 @@ __device__ inline PlanetC planet_setup(const double* th, int param, int use_c) {
     if (!(e >= 0.0 && e < 1.0)) pc.bad = 1;
     double q = sqrt((1.0 - e) * (1.0 + e));
     pc.freq = 2.0 * PI_D / per;
+    pc.frev = 1.0 / per;
+    pc.phrev = pc.phase * INV_TWO_PI;
     pc.e = e;
     pc.c1 = A * (1.0 - e) * (1.0 + e) * cw;
     pc.c2 = -A * q * sw;
@@ -509,12 +525,16 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
 template <int PREC, class KTE>
 __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const KTE* kt, double t,
                                           double& sE, double& cE, bool tp, int kmode) {
-    const double M = reduce_2pi(planet_M(pc, t, tp));
     if (PREC == 0) {
-        kepler_fp64(M, pc, tb, sE, cE);
+        kepler_fp64(reduce_2pi(planet_M(pc, t, tp)), pc, tb, sE, cE);
         return;
     }
-    const double Ma = __builtin_fabs(M);
+    // The mean anomaly in revolutions, its fraction in [-1/2, 1/2] exact,
+    // then radians: one operation fewer than the reduction of M in radians,
+    // at the same accuracy (both carry the rounding of M itself, ~1e-16 |M|).
+    const double Mr = tp ? (t - pc.tp) * pc.frev : __builtin_fma(pc.frev, t, pc.phrev);
+    const double M = Mr - __builtin_rint(Mr);
+    const double Ma = __builtin_fabs(M) * TWO_PI_HI;
     const float Mf = (float)Ma;
     float Ef;
     if (kmode & KT_F_TABLE) Ef = ktab_start(kt, Mf);

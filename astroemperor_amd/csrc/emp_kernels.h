@@ -254,10 +254,14 @@ __device__ __forceinline__ double epoch_post(const ModelDev& md, const SL& sl, c
 // without a source compose with the identity (1, 0).  Whole wave active.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ void affine_step(double& a, double& b) {
-    const double pa = dpp_f64<CTRL, ROW_MASK>(1.0, a);
-    double pb;
-    if constexpr (ROW_MASK == 0xf && CTRL > 0x110 && CTRL < 0x120) pb = dpp_shr_zero_f64<CTRL>(b);
-    else pb = dpp_f64<CTRL, ROW_MASK>(0.0, b);
+    double pa, pb;
+    if constexpr (ROW_MASK == 0xf && CTRL > 0x110 && CTRL < 0x120) {
+        pa = dpp_shr_one_f64<CTRL>(a);
+        pb = dpp_shr_zero_f64<CTRL>(b);
+    } else {
+        pa = dpp_f64<CTRL, ROW_MASK>(1.0, a);
+        pb = dpp_f64<CTRL, ROW_MASK>(0.0, b);
+    }
     b = __builtin_fma(a, pb, b);
     a = a * pa;
 }
@@ -518,8 +522,10 @@ __device__ double block_lnl(const ModelDev& md, const SL& sl, TileLDS& tl, ScanL
                     if (lane == 63) { sc.tot[parity][wave][0] = a; sc.tot[parity][wave][1] = b; }
                     __syncthreads();
                     const int w0 = wave - wsub;
-                    double pa = 1.0, pb = 0.0;
-                    for (int w = 0; w < WPT; ++w) {
+                    // (pa, pb): the maps of waves 0 .. w of the vector composed
+                    if (wsub == 0) eps = __builtin_fma(a, carry, b);
+                    double pa = sc.tot[parity][w0][0], pb = sc.tot[parity][w0][1];
+                    for (int w = 1; w < WPT; ++w) {
                         if (w == wsub) {
                             eps = __builtin_fma(a, __builtin_fma(pa, carry, pb), b);
                         }
