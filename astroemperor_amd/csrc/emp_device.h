@@ -60,24 +60,25 @@ __device__ __forceinline__ double dpp_mov_f64(double src) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// row_shr:n (0x110 + n) with every row enabled and zero in the lanes that
-// have no source (the first n of each row): update_dpp with `old` = 0.0,
-// without a register cleared to hold the 0.0.
+// row_shr:n (0x110 + n) or row_shl:n (0x100 + n) with every row enabled
+// and zero in the lanes that have no source (the first, or last, n of each
+// row): update_dpp with `old` = 0.0, without a register cleared to hold the
+// 0.0.
 template <int CTRL>
 __device__ __forceinline__ double dpp_shr_zero_f64(double src) {
-    static_assert(CTRL > 0x110 && CTRL < 0x120, "row_shr only");
+    static_assert((CTRL > 0x110 && CTRL < 0x120) || (CTRL > 0x100 && CTRL < 0x110), "row_shr or row_shl only");
     const long long s = __double_as_longlong(src);
     const int lo = __builtin_amdgcn_mov_dpp((int)s, CTRL, 0xf, 0xf, true);
     const int hi = __builtin_amdgcn_mov_dpp((int)(s >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// row_shr:n with every row enabled and 1.0 in the lanes that have no
-// source: the low word of 1.0 is zero (bound_ctrl zero fill), only the high
-// word needs an old value.
+// row_shr:n or row_shl:n with every row enabled and 1.0 in the lanes that
+// have no source: the low word of 1.0 is zero (bound_ctrl zero fill), only
+// the high word needs an old value.
 template <int CTRL>
 __device__ __forceinline__ double dpp_shr_one_f64(double src) {
-    static_assert(CTRL > 0x110 && CTRL < 0x120, "row_shr only");
+    static_assert((CTRL > 0x110 && CTRL < 0x120) || (CTRL > 0x100 && CTRL < 0x110), "row_shr or row_shl only");
     const long long s = __double_as_longlong(src);
     const int lo = __builtin_amdgcn_mov_dpp((int)s, CTRL, 0xf, 0xf, true);
     const int hi = __builtin_amdgcn_update_dpp(0x3ff00000, (int)(s >> 32), CTRL, 0xf, 0xf, false);

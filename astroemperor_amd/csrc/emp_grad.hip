@@ -932,8 +932,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
             wave_affine_scan(a, bb, lane);
             if (lane == 63) { s_scan.tot[parity][wave][0] = a; s_scan.tot[parity][wave][1] = bb; }
             __syncthreads();
-            double pa = 1.0, pb = 0.0, e0 = 0.0;
-            for (int w = 0; w < WPT; ++w) {
+            // (pa, pb): the maps of waves 0 .. w composed
+            double e0 = carry;
+            if (wave == 0) eps = __builtin_fma(a, carry, bb);
+            double pa = s_scan.tot[parity][0][0], pb = s_scan.tot[parity][0][1];
+            for (int w = 1; w < WPT; ++w) {
                 if (w == wave) {
                     e0 = __builtin_fma(pa, carry, pb);
                     eps = __builtin_fma(a, e0, bb);
@@ -983,8 +986,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
             wave_affine_suffix_scan(sa, sb, lane);
             if (lane == 0) { s_g1.suf[sparity][wave][0] = sa; s_g1.suf[sparity][wave][1] = sb; }
             __syncthreads();
-            double pa = 1.0, pb = 0.0, lB = 0.0, lA = 0.0;
-            for (int w = WPT - 1; w >= 0; --w) {
+            // (pa, pb): the maps of waves WPT - 1 down to w composed
+            double lB = sb, lA = sa;
+            double pa = s_g1.suf[sparity][WPT - 1][0], pb = s_g1.suf[sparity][WPT - 1][1];
+            for (int w = WPT - 2; w >= 0; --w) {
                 if (w == wave) {
                     lB = __builtin_fma(sa, pb, sb);
                     lA = sa * pa;

@@ -416,15 +416,40 @@ __device__ __forceinline__ double ma_chunk_q(const ModelDev& md, const SL& sl, c
     return st[0];
 }
 
+// One suffix step within rows of 16 lanes: lane i composes with lane i + n
+// of its row (row_shl:n), the last n lanes of a row with the identity.
+template <int CTRL>
+__device__ __forceinline__ void affine_suffix_step(double& a, double& b) {
+    const double pa = dpp_shr_one_f64<CTRL>(a);
+    const double pb = dpp_shr_zero_f64<CTRL>(b);
+    b = __builtin_fma(a, pb, b);
+    a = a * pa;
+}
+
 // Suffix composition for the adjoint of that recurrence, lambda_i = b_i +
 // a_i lambda_{i+1}: afterwards lane i holds the map of lanes i..63, so that
-// lambda_i = b + a lambda_in with lambda_in the value after lane 63.
+// lambda_i = b + a lambda_in with lambda_in the value after lane 63.  Within
+// rows by DPP, after which lane i holds the map of lanes i to the end of its
+// row and the first lane of a row that of the whole row; then across rows as
+// the prefix scan does it mirrored: rows 0 and 2 compose with the first lane
+// of the next row, rows 0 and 1 with lane 32 (by then rows 2 and 3).  Whole
+// wave active.
 __device__ __forceinline__ void wave_affine_suffix_scan(double& a, double& b, int lane) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        double pa = __shfl_down(a, o, 64);
-        double pb = __shfl_down(b, o, 64);
-        if (lane + o < 64) {
+    affine_suffix_step<0x101>(a, b);
+    affine_suffix_step<0x102>(a, b);
+    affine_suffix_step<0x104>(a, b);
+    affine_suffix_step<0x108>(a, b);
+    {
+        const int src = (lane & 48) + 16;
+        const double pa = __shfl(a, src, 64), pb = __shfl(b, src, 64);
+        if (!(lane & 16)) {
+            b = __builtin_fma(a, pb, b);
+            a = a * pa;
+        }
+    }
+    {
+        const double pa = readlane_f64(a, 32), pb = readlane_f64(b, 32);
+        if (lane < 32) {
             b = __builtin_fma(a, pb, b);
             a = a * pa;
         }
