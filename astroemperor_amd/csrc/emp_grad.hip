@@ -1058,22 +1058,23 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
                     f[3] = -pc.K * __builtin_fma(e, pc.sw, sfw);
                     f[4] = __builtin_fma(e, pc.cw, cfw);
                 }
+                // the B parts (one reduction for both cases), then with the MA
+                // the A parts, only where some lane's A is not negligible
                 const double b5[5] = {wB * f[0], wB * f[1], wB * f[2], wB * f[3], wB * f[4]};
-                if (!ma) {
-                    wave_reduce_add5(b5, wnat + 5 * p, lane, ri5);
-                } else {
-                    // the B parts, then the A parts only where some lane's A is not negligible
-                    const double sb = wave_reduce5(b5, lane);
-                    double sa = 0.0;
-                    if (needA) {
-                        const double a5[5] = {wA * f[0], wA * f[1], wA * f[2], wA * f[3], wA * f[4]};
-                        sa = wave_reduce5(a5, lane);
-                    }
-                    if (ri5 >= 0) {
-                        const int e = 5 * p + ri5;
+                const double sb = wave_reduce5(b5, lane);
+                double sa = 0.0;
+                if (ma && needA) {
+                    const double a5[5] = {wA * f[0], wA * f[1], wA * f[2], wA * f[3], wA * f[4]};
+                    sa = wave_reduce5(a5, lane);
+                }
+                if (ri5 >= 0) {
+                    const int e = 5 * p + ri5;
+                    if (ma) {
                         const double R0 = wnatR[e];
                         wnat[e] += __builtin_fma(tB, R0, sb);
                         wnatR[e] = __builtin_fma(tA, R0, sa);
+                    } else {
+                        wnat[e] += sb;
                     }
                 }
             }
