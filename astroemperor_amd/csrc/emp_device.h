@@ -264,14 +264,20 @@ __device__ __forceinline__ int round_half_up(float v) {
 // Mixed path: sin/cos of E = (double)Ef for an FP32 estimate Ef in
 // [-h/2, pi + h/2].  The table index is Ef / h rounded half up (|d| <= h/2
 // either way at a tie); d = E - k h in one FMA (the double h = pi / SC_N is
-// exact to 1e-19, k <= SC_N); sin d to d^3 (d^5 / 120 < 7e-14 dropped).
+// within 4.8e-19 of pi / SC_N, so d within k 4.8e-19 <= 1.2e-16, largest
+// near pi); sin d to d^3 (d^5 / 120 < 7e-14 dropped) and cos d to d^4
+// (d^6 / 720 < 8e-17 dropped), or, with D5, both to the sincos_tab degree
+// (d^5 and d^6).
+template <bool D5 = false>
 __device__ __forceinline__ void sincos_tab_f(const SinCosTab& tb, float Ef, double E, double& s, double& c) {
     const int k = round_half_up(Ef * (float)SC_INV_H);
     const double d = __builtin_fma(-(double)k, SC_H, E);
     const double2 sc = tb.v[k];
     const double d2 = d * d;
-    const double sd = __builtin_fma(d * d2, -1.0 / 6.0, d);
-    const double cd = __builtin_fma(d2, __builtin_fma(d2, 1.0 / 24.0, -0.5), 1.0);
+    const double sd = D5 ? __builtin_fma(d * d2, __builtin_fma(d2, 1.0 / 120.0, -1.0 / 6.0), d)
+                         : __builtin_fma(d * d2, -1.0 / 6.0, d);
+    const double cd = D5 ? __builtin_fma(d2, __builtin_fma(d2, __builtin_fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0)
+                         : __builtin_fma(d2, __builtin_fma(d2, 1.0 / 24.0, -0.5), 1.0);
     s = __builtin_fma(sc.x, cd, sc.y * sd);
     c = __builtin_fma(sc.y, cd, -sc.x * sd);
 }
@@ -477,22 +483,18 @@ __device__ __forceinline__ void ktab_fill(const PlanetC& pc, int lane, float4& o
 // from the FP32 Markley start.
 // The Newton step's Ef comes from the table, within 6.1e-7 rad of the root
 // in [0, pi] (or NaN), which sincos_tab_f takes as it is; the Halley step's
-// is clamped into [0, pi] first.
+// (also from the FP32 Markley start) is clamped into [0, (float)pi] first
+// (FP32 pi lies 8.7e-8 above pi, within half a sin/cos table step).
 template <bool HALLEY>
 __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, const SinCosTab& tb, double& sE,
                                                  double& cE) {
     if (HALLEY) Ef = __builtin_amdgcn_fmed3f(Ef, 0.0f, (float)PI_D);
-    double E0 = (double)Ef;
+    const double E0 = (double)Ef;
     double s, c;
     // Near periastron at high e the step amplifies an error in sin E0 by
-    // 1 / (1 - e cos E0): there the full-accuracy table evaluation is used
-    // (on [0, pi]: (float)pi lies above pi).
-    if (HALLEY) {
-        E0 = __builtin_fmin(E0, PI_D);
-        sincos_tab(tb, E0, s, c);
-    } else {
-        sincos_tab_f(tb, Ef, E0, s, c);
-    }
+    // 1 / (1 - e cos E0) (up to 20 at e = 0.95): the Halley step takes sin d
+    // to d^5; the error of d is largest near pi, where it is not amplified.
+    sincos_tab_f<HALLEY>(tb, Ef, E0, s, c);
     // 1 - e c as fma(e, -c, 1) (the same value): the negation then folds into
     // the operand of each use instead of a negated e kept between blocks.
     const double f0 = E0 - __builtin_fma(e, s, Ma);
