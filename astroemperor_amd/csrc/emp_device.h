@@ -139,8 +139,6 @@ __device__ __forceinline__ double reduce_2pi(double M) {
 // Per-planet constants, computed once per theta (per planet) and kept in LDS.
 // ----------------------------------------------------------------------------
 struct PlanetC {
-    double frev;     // 1 / P: the mean motion in revolutions (mixed path)
-    double phrev;    // phase / (2 pi)
     double freq;     // 2 pi / P
     double phase;    // M = freq * t + phase         (tp_mode == 0)
     double tp;       // M = freq * (t - tp)          (tp_mode == 1)
@@ -201,8 +199,6 @@ __device__ inline PlanetC planet_setup(const double* th, int param, int use_c) {
     if (!(e >= 0.0 && e < 1.0)) pc.bad = 1;
     double q = sqrt((1.0 - e) * (1.0 + e));
     pc.freq = 2.0 * PI_D / per;
-    pc.frev = 1.0 / per;
-    pc.phrev = pc.phase * INV_TWO_PI;
     pc.e = e;
     pc.c1 = A * (1.0 - e) * (1.0 + e) * cw;
     pc.c2 = -A * q * sw;
@@ -528,16 +524,15 @@ __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, 
 template <int PREC, class KTE>
 __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb, const KTE* kt, double t,
                                           double& sE, double& cE, bool tp, int kmode) {
+    // Both paths reduce the same FP64 mean anomaly: the mixed path's M is
+    // rounded exactly as the FP64 path's, so its error near periastron at
+    // high e (amplified by dE/dM = 1 / (1 - e cos E)) stays the FP64 path's.
+    const double M = reduce_2pi(planet_M(pc, t, tp));
     if (PREC == 0) {
-        kepler_fp64(reduce_2pi(planet_M(pc, t, tp)), pc, tb, sE, cE);
+        kepler_fp64(M, pc, tb, sE, cE);
         return;
     }
-    // The mean anomaly in revolutions, its fraction in [-1/2, 1/2] exact,
-    // then radians: one operation fewer than the reduction of M in radians,
-    // at the same accuracy (both carry the rounding of M itself, ~1e-16 |M|).
-    const double Mr = tp ? (t - pc.tp) * pc.frev : __builtin_fma(pc.frev, t, pc.phrev);
-    const double M = Mr - __builtin_rint(Mr);
-    const double Ma = __builtin_fabs(M) * TWO_PI_HI;
+    const double Ma = __builtin_fabs(M);
     const float Mf = (float)Ma;
     float Ef;
     if (kmode & KT_F_TABLE) Ef = ktab_start(kt, Mf);

@@ -196,23 +196,39 @@ def test_large_config_mixed_vs_fp64():
 
 
 @pytest.mark.parametrize("lo,hi", [(0.0, 0.5), (0.5, 0.75), (0.75, 0.95), (0.95, 0.99)])
-def test_eccentricity_bands_mixed_vs_fp64(lo, hi):
+@pytest.mark.parametrize("wpt", [1, 4])
+def test_eccentricity_bands_mixed_vs_fp64(lo, hi, wpt):
     """Every planet at e in [lo, hi]: one band per Kepler path of the mixed
     path (the per-vector table with an FP64 Newton step up to e = 0.5, the
     table with an FP64 Halley step up to 0.75, the table with an FP32 and
     an FP64 Halley step up to 0.95, the FP32 Markley start with the same two
     steps above), on draws that fit the data badly, which magnifies every
-    model error: still within 1e-6 of the FP64 path."""
+    model error: still within 1e-6 of the FP64 path.  At 1 wave per vector
+    a workgroup holds 8 vectors and keeps the tables as nodes, at 4 it holds
+    2 and keeps them per interval; the two forms evaluate the same FP32
+    cubic up to the rounding of M / h, so they agree far inside the bar."""
     from astroemperor_amd import synth
     m = synth.c2()["model"]
     rng = np.random.default_rng(17)
     th = _draws(m, 4096, rng)
     ecol = [i for i, n in enumerate(m.names_free()) if n.startswith("Eccentricity")]
     th[:, ecol] = rng.uniform(lo, hi, size=(len(th), len(ecol)))
-    a = m.lnl_batch(th, "fp64")
-    b = m.lnl_batch(th, "mixed")
+    m.set_waves_per_theta(wpt)
+    try:
+        a = m.lnl_batch(th, "fp64")
+        b = m.lnl_batch(th, "mixed")
+        if wpt == 4:
+            m.set_waves_per_theta(1)
+            b1 = m.lnl_batch(th, "mixed")
+    finally:
+        m.set_waves_per_theta(0)
     fin = np.isfinite(a)
     assert fin.sum() > 1000 and np.array_equal(fin, np.isfinite(b))
     dm = np.abs(a[fin] - b[fin])
-    print(f"e in [{lo}, {hi}] mixed vs fp64: max {dm.max():.2e} p99 {np.percentile(dm, 99):.2e}", flush=True)
+    print(f"e in [{lo}, {hi}] wpt {wpt} mixed vs fp64: max {dm.max():.2e} p99 {np.percentile(dm, 99):.2e}",
+          flush=True)
     assert dm.max() < 1e-6
+    if wpt == 4:
+        dn = np.abs(b1[fin] - b[fin])
+        print(f"e in [{lo}, {hi}] nodes vs intervals: max {dn.max():.2e}", flush=True)
+        assert dn.max() < 1e-6
