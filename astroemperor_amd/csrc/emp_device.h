@@ -477,14 +477,14 @@ __device__ __forceinline__ void ktab_fill(const PlanetC& pc, int lane, float4& o
 // table start (<= 6e-7 rad, so sin E, cos E follow from a first-order
 // rotation, neglected d^2 / 2 < 2e-13), Halley with a third-order rotation
 // from the FP32 Markley start.
-// The Newton step's Ef comes from the table, within 6.1e-7 rad of the root
-// in [0, pi] (or NaN), which sincos_tab_f takes as it is; the Halley step's
-// (also from the FP32 Markley start) is clamped into [0, (float)pi] first
-// (FP32 pi lies 8.7e-8 above pi, within half a sin/cos table step).
+// Ef: the start, from the table within 3.5e-5 rad of the root in [0, pi]
+// (6.1e-7 for the Newton step), the FP32 Markley start clamped into
+// [0, (float)pi], or NaN; sincos_tab_f takes it as it is (its index stays
+// in [0, SC_N] for any start within h/2 = 6.1e-3 rad of [0, pi], and is 0
+// for NaN), and the step converges from either side of the root.
 template <bool HALLEY>
 __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, const SinCosTab& tb, double& sE,
                                                  double& cE) {
-    if (HALLEY) Ef = __builtin_amdgcn_fmed3f(Ef, 0.0f, (float)PI_D);
     const double E0 = (double)Ef;
     double s, c;
     // Near periastron at high e the step amplifies an error in sin E0 by
@@ -535,9 +535,14 @@ __device__ __forceinline__ void planet_sc(const PlanetC& pc, const SinCosTab& tb
     const double Ma = __builtin_fabs(M);
     const float Mf = (float)Ma;
     float Ef;
-    if (kmode & KT_F_TABLE) Ef = ktab_start(kt, Mf);
-    else Ef = kepler_fp32_markley(Mf, pc);
-    if (kmode & KT_F_H32) Ef = kepler_fp32_halley(Ef, Mf, pc);
+    if (kmode & KT_F_TABLE) {
+        Ef = ktab_start(kt, Mf);
+        if (kmode & KT_F_H32) Ef = kepler_fp32_halley(Ef, Mf, pc);
+    } else {
+        // the Markley start always takes the FP32 Halley step (kt_flags);
+        // near periastron at e -> 1 it may leave [0, pi], so it is clamped
+        Ef = __builtin_amdgcn_fmed3f(kepler_fp32_start(Mf, pc), 0.0f, (float)PI_D);
+    }
     if (kmode & KT_F_H64) kepler_fp64_step<true>(Ma, Ef, pc.e, tb, sE, cE);
     else kepler_fp64_step<false>(Ma, Ef, pc.e, tb, sE, cE);
     // E(-M) = -E(M), and sin E >= 0 on [0, pi]: the sign of M, one bit insert
