@@ -892,29 +892,32 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
         const int j = g;
         const bool valid = j < tn;
         // Stage 1: every planet's (sin E, cos E), the model and the residual.
+        // Every lane solves the planets, a lane past the series' end at
+        // t = 0: its values are finite and its weights (wB, wA) exactly zero,
+        // so neither these arrays nor the derivatives below are cleared.
         double sE[KMAX], cE[KMAX];
-        double t = 0.0, r = 0.0, err2 = 1.0;
+        double t = 0.0, r = 0.0, err2 = 1.0, mu = 0.0;
         int k = 0;
         if (valid) {
             t = s_tile.t[j];
             k = s_tile.ins[j];
-            double mu = sl.gam[k];
+            mu = sl.gam[k];
             if (md.acc) {
                 const double x = t - md.t0;
                 double v = 0.0;
                 for (int a = 0; a < md.acc; ++a) v = __builtin_fma(v, x, sl.acc[a]);
                 mu += v * x;
             }
+        }
 #pragma unroll
-            for (int p = 0; p < KMAX; ++p) {
-                sE[p] = cE[p] = 0.0;
-                if (p < md.K) {
-                    const PlanetC& pc = sl.planet[p];
-                    planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p], tp, kt_mode(kmask, p));
-                    mu += __builtin_fma(pc.c1, cE[p], pc.c2 * sE[p])
-                          * rcp_path<PREC>(__builtin_fma(pc.e, -cE[p], 1.0));
-                }
+        for (int p = 0; p < KMAX; ++p) {
+            if (p < md.K) {
+                const PlanetC& pc = sl.planet[p];
+                planet_sc<PREC>(pc, s_tile.tab, sl.ktab[p], t, sE[p], cE[p], tp, kt_mode(kmask, p));
+                mu += __builtin_fma(pc.c1, cE[p], pc.c2 * sE[p]) * rcp_path<PREC>(__builtin_fma(pc.e, -cE[p], 1.0));
             }
+        }
+        if (valid) {
             err2 = s_tile.s2[j] + sl.jit2[k];
             r = s_tile.y[j] - mu;
             ia.to<EXT>(k, md.ma_global ? 0 : k, wacc, wrst);
@@ -1043,8 +1046,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(EXT == 
         for (int p = 0; p < KMAX; ++p) {
             if (p < md.K) {
                 const PlanetC& pc = sl.planet[p];
-                double f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                if (valid) {
+                double f[5];    // every lane (see stage 1)
+                {
                     const double e = pc.e;
                     const double rD = rcp_path<PREC>(__builtin_fma(e, -cE[p], 1.0));
                     const double uu = cE[p] - e, vv = pc.bq * sE[p];
