@@ -474,14 +474,15 @@ __device__ __forceinline__ void ktab_fill(const PlanetC& pc, int lane, float4& o
 }
 
 // One FP64 step from the FP32 root Ef (Ma = |M| in [0, pi]): Newton from the
-// table start (<= 6e-7 rad, so sin E, cos E follow from a first-order
-// rotation, neglected d^2 / 2 < 2e-13), Halley with a third-order rotation
-// from the FP32 Markley start.
-// Ef: the start, from the table within 3.5e-5 rad of the root in [0, pi]
-// (6.1e-7 for the Newton step), the FP32 Markley start clamped into
-// [0, (float)pi], or NaN; sincos_tab_f takes it as it is (its index stays
-// in [0, SC_N] for any start within h/2 = 6.1e-3 rad of [0, pi], and is 0
-// for NaN), and the step converges from either side of the root.
+// table start at e <= KT_ENEWTON (<= 6.1e-7 rad, so sin E, cos E follow
+// from a first-order rotation, neglected d^2 / 2 < 2e-13), otherwise Halley
+// with a third-order rotation (from the table start, after the FP32 Halley
+// step above KT_EH64, or from the FP32 Markley start and Halley step).
+// Ef is within 3.5e-5 rad of the root in [0, pi] when it comes from the
+// table, clamped into [0, (float)pi] when it comes from the Markley start,
+// or NaN; sincos_tab_f takes it as it is (its index stays in [0, SC_N] for
+// any start within h/2 = 6.1e-3 rad of [0, pi], and is 0 for NaN), and the
+// step converges from either side of the root.
 template <bool HALLEY>
 __device__ __forceinline__ void kepler_fp64_step(double Ma, float Ef, double e, const SinCosTab& tb, double& sE,
                                                  double& cE) {
