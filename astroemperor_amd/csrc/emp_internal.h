@@ -142,7 +142,9 @@ bool emp_fault_injected();
     } while (0)
 
 // Likelihood launchers (emp_model.hip), used by the PT engine as well.
-int emp_pick_wpt(const EmpModel* m, int64_t B, int precision);
+// Waves per vector for B vectors: batched likelihoods, or (halfstep) the
+// PT half-step's proposals.
+int emp_pick_wpt(const EmpModel* m, int64_t B, int precision, bool halfstep = false);
 int emp_launch_lnl(const EmpModel* m, const double* d_theta, int64_t B, int64_t ld,
                    double* d_lnl, int precision, int wpt, hipStream_t s);
 int emp_launch_prior(const EmpModel* m, const double* d_theta, int64_t B, int64_t ld,

@@ -148,20 +148,26 @@ static void launch_lnl_t(const ModelDev& md, const double* th, int64_t B, int64_
                            nf);
 }
 
-int emp_pick_wpt(const EmpModel* m, int64_t B, int precision) {
-    (void)precision;
+int emp_pick_wpt(const EmpModel* m, int64_t B, int precision, bool halfstep) {
     if (m->wpt_override) return m->wpt_override;
-    // Enough waves to give every SIMD (4 per CU) about 16, at least two waves
-    // per vector for series of 1024 epochs or more, but never more lanes per
-    // vector than about half the epochs.  Measured batched lnL (mixed):
-    // 7 planets x 10^4 epochs at 8192 vectors 1.58 / 1.40 / 1.40 / 1.49 ms at
-    // 1 / 2 / 4 / 8 waves per vector; 5 planets x 2000 epochs at 65,536
-    // vectors 1.47 / 1.32 / 1.43 / 1.62 ms; 3 planets x 1530 epochs at 2560
-    // vectors 77 / 59 / 56 / 63 us.
-    const int64_t target_waves = (int64_t)m->n_cu * 4 * 16;
+    // Enough waves to give every SIMD (4 per CU) about 32 on the mixed path
+    // (from 4 waves per vector its Kepler tables are kept per interval) and
+    // 16 on the FP64 path, at least two waves per vector for series of 1024
+    // epochs or more, but never more lanes per vector than a third of the
+    // epochs for batched likelihoods, half for the PT half-step (whose few
+    // proposals make one walker's critical path the launch's length).
+    // Measured batched lnL at 1 / 2 / 4 / 8 waves per vector, mixed and FP64:
+    // 7 planets x 10^4 epochs at 8192 vectors 1.48 / 1.31 / 1.28 / 1.37 ms
+    // and 3.07 / 3.01 / 3.06 / 3.15 ms; 5 planets x 2000 epochs at 65,536
+    // vectors 1.40 / 1.26 / 1.34 / 1.53 ms and 3.42 / 3.40 / 3.48 / 3.66 ms;
+    // 3 planets x 1530 epochs at 2560 vectors 72 / 56 / 52 / 59 us and
+    // 121 / 92 / 81 / 90 us.  The rule picks 4, 2, 2, 2, 4 and 4.
+    const int per_simd = precision == EMP_MIXED ? 32 : 16;
+    const int64_t target_waves = (int64_t)m->n_cu * 4 * per_simd;
     const int wmin = m->dev.N >= 1024 ? 2 : 1;
+    const int max_lanes = halfstep ? m->dev.N / 2 : m->dev.N / 3;
     int wpt = 1;
-    while (wpt < 8 && (B * wpt < target_waves || wpt < wmin) && 64 * (wpt * 2) <= m->dev.N / 2) wpt *= 2;
+    while (wpt < 8 && (B * wpt < target_waves || wpt < wmin) && 64 * (wpt * 2) <= max_lanes) wpt *= 2;
     return wpt;
 }
 

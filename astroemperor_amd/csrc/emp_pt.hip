@@ -1159,7 +1159,7 @@ extern "C" int emp_pt_create(EmpModel* m, const EmpPTDesc* desc, const double* b
     EMP_TRY(hipMemset(v.rec_owed, 0, sizeof(int)));
     EMP_TRY(hipDeviceSynchronize());     // the engine stream does not wait for the null stream
 #undef EMP_TRY
-    p->wpt = emp_pick_wpt(m, (int64_t)v.T * v.W2, d.precision);
+    p->wpt = emp_pick_wpt(m, (int64_t)v.T * v.W2, d.precision, true);
     rc = eval_state(p);
     if (rc) { emp_pt_destroy(p); return rc; }
     *out = p;
