@@ -280,13 +280,11 @@ __device__ __forceinline__ void sincos_tab_f(const SinCosTab& tb, float Ef, doub
     c = __builtin_fma(sc.y, cd, -sc.x * sd);
 }
 
-// x^(2/3) for x >= 0 in FP64: hardware FP32 log2/exp2 seed (~1e-7) and one
-// Newton step on w^3 = x^2, w <- (2 w + x^2 / w^2) / 3 (quadratic: ~1e-14).
-__device__ __forceinline__ double pow23_fp64(double x) {
+// x^(2/3) for x >= 0 to FP32 accuracy (hardware log2 / exp2, ~3e-7
+// relative): the Markley starter's cube root (see kepler_fp64_E).
+__device__ __forceinline__ double pow23_start(double x) {
     const float xf = (float)x;
-    const double w0 = (double)__builtin_amdgcn_exp2f(0.6666667f * __builtin_amdgcn_logf(xf));
-    const double w = (2.0 * w0 + x * x * rcp64(w0 * w0)) * (1.0 / 3.0);
-    return xf > 0.0f ? w : 0.0;
+    return xf > 0.0f ? (double)__builtin_amdgcn_exp2f(0.6666667f * __builtin_amdgcn_logf(xf)) : 0.0;
 }
 
 // ----------------------------------------------------------------------------
@@ -305,8 +303,13 @@ __device__ __forceinline__ void kepler_fp64_E(double M, const PlanetC& pc, const
     const double q = __builtin_fma(2.0 * ad, ome, -M2);
     const double r = __builtin_fma(3.0 * ad * (d - ome), Ma, M2 * Ma);
     const double q2 = q * q;
-    const double w = pow23_fp64(__builtin_fabs(r) + sqrt(__builtin_fma(q2, q, r * r)));
-    double E1 = __builtin_fma(2.0 * r * w, rcp64(__builtin_fma(w, w + q, q2)), Ma) * rcp64(d);
+    // The starter's arithmetic is FP64 (q and r cancel near periastron at
+    // high e); its square root, cube root and reciprocals are the hardware's
+    // FP32-accurate ones (about 3e-7 relative): the start is up to 4e-4 rad
+    // from the root in any case, and the fifth-order correction below
+    // leaves (4e-4)^5 of that.
+    const double w = pow23_start(__builtin_fabs(r) + (double)__builtin_amdgcn_sqrtf((float)__builtin_fma(q2, q, r * r)));
+    double E1 = __builtin_fma(2.0 * r * w, __builtin_amdgcn_rcp(__builtin_fma(w, w + q, q2)), Ma) * __builtin_amdgcn_rcp(d);
     E1 = __builtin_fmin(__builtin_fmax(E1, 0.0), PI_D);
     double s, c;
     sincos_tab(tb, E1, s, c);
