@@ -60,13 +60,13 @@ PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
 # (workload, kernel, precision); values (FETCH_SIZE, WRITE_SIZE).  Source:
 # profiles/hbm_traffic_k_pt_half_k_lnl_grad1_k_sys_sweep.md.
 TRAFFIC_KIB = {
-    ("c5", "k_pt_half", "mixed"): (3393.5, 1446.3),       # k_pt_half<1, 4>, 8192 proposals
-    ("c5", "k_lnl_grad", "mixed"): (4971.8, 6676.1),      # k_lnl_grad1<1, 0, 4, false>, 8192 vectors
-    ("c5", "k_lnl_grad", "fp64"): (5051.8, 6676.8),       # k_lnl_grad1<0, 0, 4, false>
-    ("c2", "k_pt_half", "mixed"): (594.1, 84.5),          # k_pt_half<1, 8>, 1280 proposals
-    ("c2", "k_pt_half", "fp64"): (580.6, 84.5),           # k_pt_half<0, 8>
-    ("c3", "k_sys_sweep", "mixed"): (193554.4, 139848.4),  # k_sys_sweep<1, 1, 16>, 4096 systems
-    ("c3", "k_sys_sweep", "fp64"): (195304.1, 139739.2),   # k_sys_sweep<0, 1, 16>
+    ("c5", "k_pt_half", "mixed"): (3371.8, 1457.1),       # k_pt_half<1, 4>, 8192 proposals
+    ("c5", "k_lnl_grad", "mixed"): (4968.9, 6669.2),      # k_lnl_grad1<1, 0, 4, false>, 8192 vectors
+    ("c5", "k_lnl_grad", "fp64"): (5035.4, 6670.8),       # k_lnl_grad1<0, 0, 4, false>
+    ("c2", "k_pt_half", "mixed"): (593.6, 84.5),          # k_pt_half<1, 8>, 1280 proposals
+    ("c2", "k_pt_half", "fp64"): (585.6, 84.5),           # k_pt_half<0, 8>
+    ("c3", "k_sys_sweep", "mixed"): (193728.0, 139875.8),  # k_sys_sweep<1, 1, 16>, 4096 systems
+    ("c3", "k_sys_sweep", "fp64"): (194613.8, 135634.8),   # k_sys_sweep<0, 1, 16>
 }
 FETCH_SCALE, WRITE_SCALE = 2.00, 1.00
 
