@@ -54,7 +54,8 @@ extern "C" {
 #define EMP_GRAD_MIXED 2    /* emp_lnl_grad_batch: the mixed path's Kepler solve (EMP_MIXED accuracy) */
 
 /* Precision of the likelihood path. */
-#define EMP_FP64 0          /* every operation FP64: the drop-in path */
+#define EMP_FP64 0          /* FP64 arithmetic (the Kepler starter's roots and reciprocals at FP32 accuracy, which
+                                the FP64 fifth-order correction removes): the drop-in path */
 #define EMP_MIXED 1         /* FP32 Kepler iterations + FP64 phase, final step, accumulation */
 
 /* Prior kinds (support/priors/<Kind>.prior). */
