@@ -61,6 +61,7 @@ PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector, FMA = 2 flops
 # profiles/hbm_traffic_k_pt_half_k_lnl_grad1_k_sys_sweep.md.
 TRAFFIC_KIB = {
     ("c5", "k_pt_half", "mixed"): (3371.8, 1457.1),       # k_pt_half<1, 4>, 8192 proposals
+    ("c5", "k_pt_half", "fp64"): (3894.2, 1334.2),        # k_pt_half<0, 8>
     ("c5", "k_lnl_grad", "mixed"): (4968.9, 6669.2),      # k_lnl_grad1<1, 0, 4, false>, 8192 vectors
     ("c5", "k_lnl_grad", "fp64"): (5035.4, 6670.8),       # k_lnl_grad1<0, 0, 4, false>
     ("c2", "k_pt_half", "mixed"): (593.6, 84.5),          # k_pt_half<1, 8>, 1280 proposals
